@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""KRR fit + predict benchmark on one MI355X: the serving workload at N = 50,000.
+
+Workload: 50,000 seeded synthetic training molecules (up to 29 atoms, Coulomb matrix size 29,
+435 FP64 features), Gaussian kernel sigma = 4000, lambda = 1e-6. One step is one full fit with
+the raw molecules (Z, R) already resident in HBM: Coulomb matrices on the GPU, FP64 squared
+norms, the symmetric 50k x 50k Gaussian kernel (FP64 MFMA), K + lambda I, the blocked Cholesky
+and the two triangular solves. After the timed steps, prediction requests of 1-512 molecules
+(log-uniform) are served one at a time (host molecules in, predictions out) and their latency
+percentiles are reported.
+
+Prints one JSON line. `value` is the fit time in seconds (lower is better).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA_TFLOPS = 78.6
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n-train", type=int, default=50000)
+    p.add_argument("--sigma", type=float, default=4000.0)
+    p.add_argument("--lam", type=float, default=1e-6)
+    p.add_argument("--requests", type=int, default=300)
+    p.add_argument("--cpu-sample", type=int, default=10000,
+                   help="training molecules in the CPU-baseline sample (0 = skip)")
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_full, size, sigma, lam, n_sample):
+    """Reference path on the host cores over a bounded sample, extrapolated to n_full.
+
+    Coulomb matrices one molecule at a time (as the reference's callers loop), the training
+    kernel with the reference Fortran (oracle/_ref) where it was built, else the C
+    restatement, and cho_solve as LAPACK dpotrf + dpotrs. Times scale as n, n^2 and n^3.
+    """
+    import numpy as np
+
+    import oracle
+    from qml_amd import synthetic
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    tr, _, _ = synthetic.workload("krr_serve_50k", n_train=n_sample, n_test=0)
+    use_ref = oracle.ref_available()
+    t0 = time.perf_counter()
+    if use_ref:
+        X = np.array([oracle.ref_coulomb_matrix(*tr.molecule(i), size) for i in range(tr.nmol)])
+    else:
+        X = np.array([oracle.coulomb_matrix(*tr.molecule(i), size) for i in range(tr.nmol)])
+    t1 = time.perf_counter()
+    K = oracle.ref_kernel("gaussian", X, X, sigma) if use_ref else oracle.gaussian_kernel(X, X, sigma)
+    t2 = time.perf_counter()
+    K[np.diag_indices_from(K)] += lam
+    oracle.cho_solve(K, tr.labels.copy())
+    t3 = time.perf_counter()
+    r = n_full / n_sample
+    est = (t1 - t0) * r + (t2 - t1) * r * r + (t3 - t2) * r ** 3
+    return {
+        "value": est, "unit": "s", "cores": cores,
+        "kind": "reference" if use_ref else "port",
+        "sample": (f"fit at N={n_sample} on {cores} host threads: CM {t1 - t0:.2f}s, "
+                   f"Gaussian kernel {t2 - t1:.2f}s, cho_solve {t3 - t2:.2f}s; extrapolated "
+                   f"to N={n_full} as N, N^2, N^3"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "design is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from qml_amd import _lib, synthetic
+    from qml_amd.device import ragged_to_device
+    from qml_amd.krr import KRR
+    from qml_amd.representations import cm_length, generate_coulomb_matrices_device
+
+    torch.cuda.set_device(0)
+    _lib.lib()
+    N = args.n_train
+    tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)
+    d = cm_length(size)
+    Zd, Rd, Od = ragged_to_device(*tr.ragged())
+    yd = torch.from_numpy(tr.labels).cuda()
+    log(f"workload: N={N} size={size} d={d} sigma={args.sigma} lambda={args.lam}")
+
+    model = KRR("gaussian", args.sigma, args.lam)
+
+    def step():
+        Xd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size)
+        model.fit_device(Xd, d, yd)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done")
+    torch.cuda.synchronize()
+    _lib.timer_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timers = {k: _lib.timer_read(k) for k in _lib.TIMER_CLASSES}
+    _lib.timer_enable(False)
+    fit_s = (t1 - t0) / args.steps
+
+    # Dominant kernel: the FP64 MFMA trailing update of the Cholesky.
+    syrk_ms, syrk_n, syrk_flops = timers["syrk"]
+    achieved = syrk_flops / (syrk_ms * 1e-3) / 1e12 if syrk_ms > 0 else 0.0
+    phases = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
+                  "rate": (v[2] / (v[0] * 1e-3) if v[0] > 0 else 0.0)}
+              for k, v in timers.items() if v[1] > 0}
+
+    # Predict: requests of 1..512 molecules (log-uniform), served one at a time.
+    rng = np.random.default_rng(5)
+    sizes = np.clip(np.exp(rng.uniform(0, np.log(512), args.requests)).astype(int), 1, 512)
+    lat = []
+    start = 0
+    molecules = 0
+    for k in sizes:
+        sub = te.subset(start % (te.nmol - 512), start % (te.nmol - 512) + k)
+        start += k
+        Z, R, off = sub.ragged()
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        y = model.predict_molecules(Z, R, off, size)
+        tb = time.perf_counter()
+        lat.append((tb - ta) * 1e3)
+        molecules += k
+    lat = np.array(lat)
+    log(f"fit {fit_s:.3f}s; predict p50 {np.percentile(lat, 50):.2f} ms")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"cpu baseline sample N={args.cpu_sample} ...")
+        cpu = cpu_baseline(N, size, args.sigma, args.lam, args.cpu_sample)
+        log(f"cpu baseline: {cpu['value']:.1f}s (extrapolated)")
+
+    out = {
+        "metric": "KRR fit s + predict p50/p99 ms at N=50k, mixed vs FP64 GPU at equal error",
+        "value": fit_s,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fit_s * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded molecules, 5-29 atoms, H/C/N/O/S)",
+        "config": {"workload": "krr_serve_50k: KRR fit N=50000, Coulomb size 29 (d=435), "
+                               "Gaussian sigma=4000, lambda=1e-6; predict requests 1-512 molecules",
+                   "n_train": N, "d": d, "sigma": args.sigma, "lambda": args.lam,
+                   "precision": "fp64"},
+        "predict": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                    "requests": int(len(lat)), "molecules_per_s": float(molecules / (lat.sum() / 1e3)),
+                    "mode": "sequential requests, host molecules in, predictions out"},
+        "phases": phases,
+        "roofline": {"kernel": "gemm_update_kernel<lower> (Cholesky trailing update, FP64 MFMA)",
+                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS,
+                     "traffic": None, "launches": syrk_n / args.steps},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

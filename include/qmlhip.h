@@ -1,0 +1,162 @@
+/*
+ * libqmlhip: C ABI of the MI355X (gfx950) KRR hot path.
+ *
+ * Conventions
+ *  - Every pointer is a device pointer (HBM) unless its name ends in _h (host memory).
+ *    The caller owns every buffer; no entry point allocates device memory.
+ *  - Representations are row-major (n, ld) FP64: row i at X + i*ld, features 0..d-1.
+ *  - Kernel, distance and Cholesky matrices are column-major: (i, j) at K[i + j*ldk],
+ *    i.e. the Fortran-ordered (na, nb) array the reference returns (qml/kernels.py:58).
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = the null stream) and returns
+ *    a status: 0 ok, < 0 error (message in qmlh_last_error(), thread local). Numerical outcomes
+ *    that LAPACK reports as `info` are written to a device int32 the caller reads back.
+ *  - Calls on different streams may run concurrently; the library keeps no mutable global
+ *    state besides the constant diagonal table of qmlh_init.
+ */
+#ifndef QMLHIP_H
+#define QMLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *qmlh_stream_t;
+
+enum {
+    QMLH_OK = 0,
+    QMLH_ERR_ARG = -1,
+    QMLH_ERR_HIP = -2,
+    QMLH_ERR_UNSUPPORTED = -3,
+    QMLH_ERR_WORKSPACE = -4
+};
+
+/* Library version (major*10000 + minor*100 + patch). */
+int qmlh_version(void);
+
+/* Message of the last failing call on this thread ("" when none). */
+const char *qmlh_last_error(void);
+
+/* Upload the Coulomb diagonal table 0.5*Z**2.4 for integer Z in [0, 255] computed with the
+ * host libm pow, the same call the reference Fortran makes (qml/frepresentations.f90:95).
+ * Called once per device before qmlh_cm_batch; idempotent. */
+int qmlh_init(void);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Coulomb matrices.
+ * Replaces fgenerate_coulomb_matrix (qml/frepresentations.f90:52-140, sorted = 1) and
+ * fgenerate_unsorted_coulomb_matrix (qml/frepresentations.f90:142-205, sorted = 0), which
+ * qml/representations.py:109-115 calls once per molecule; here one call covers a batch.
+ * Z: (total_atoms,) nuclear charges as FP64 (the f2py coercion of the reference).
+ * R: (total_atoms, 3) coordinates, row-major. offsets: (nmol + 1,) int64, molecule m owns
+ * atoms offsets[m] .. offsets[m+1]-1. out: (nmol, ldo) row-major, ldo >= size*(size+1)/2;
+ * columns beyond size*(size+1)/2 are written as zero. size <= QMLH_CM_MAX_SIZE.
+ * status (device int32, may be NULL): set to 1 when some molecule has natoms > size, a case
+ * the reference leaves undefined (it writes out of bounds); that molecule's row is zeroed.
+ * ---------------------------------------------------------------------------------------- */
+#define QMLH_CM_MAX_SIZE 256
+int qmlh_cm_batch(const double *Z, const double *R, const int64_t *offsets, int64_t nmol,
+                  int32_t size, int32_t sorted, double *out, int64_t ldo, int32_t *status,
+                  qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Pairwise kernels and distances, VALU tile kernels (direct differences).
+ * kind QMLH_PAIR_LAPLACIAN: K(i,j) = exp((-1/sigma) * sum_k |A_ik - B_jk|)
+ *      replaces flaplacian_kernel (qml/fkernels.f90:374-400) behind qml/kernels.py:36-63.
+ * kind QMLH_PAIR_GAUSSIAN:  K(i,j) = exp((-0.5/(sigma*sigma)) * sum_k (A_ik - B_jk)^2)
+ *      replaces fgaussian_kernel (qml/fkernels.f90:340-372) behind qml/kernels.py:65-92.
+ * kind QMLH_PAIR_MANHATTAN: D(i,j) = sum_k |A_ik - B_jk|
+ *      replaces fmanhattan_distance (qml/fdistance.f90:23-46) behind qml/distance.py:28-59.
+ * kind QMLH_PAIR_L2: D(i,j) = sqrt(sum_k (A_ik - B_jk)^2)
+ *      replaces fl2_distance (qml/fdistance.f90:49-80) behind qml/distance.py:61-92.
+ * The feature sum runs k = 0..d-1 in order for every entry, as the reference loops do.
+ * flags QMLH_SYMMETRIC: A == B (same rows); only tiles on or above the diagonal are computed
+ * and mirrored, which is value-identical because |a-b| = |b-a| and (a-b)^2 = (b-a)^2.
+ * ---------------------------------------------------------------------------------------- */
+enum {
+    QMLH_PAIR_LAPLACIAN = 0,
+    QMLH_PAIR_GAUSSIAN = 1,
+    QMLH_PAIR_MANHATTAN = 2,
+    QMLH_PAIR_L2 = 3
+};
+enum { QMLH_SYMMETRIC = 1 };
+int qmlh_pairwise(int32_t kind, const double *A, int64_t na, int64_t lda, const double *B,
+                  int64_t nb, int64_t ldb, int64_t d, double sigma, double *K, int64_t ldk,
+                  int32_t flags, qmlh_stream_t stream);
+
+/* Squared row norms norms[i] = sum_k X_ik^2 in FP64 (k in order). */
+int qmlh_row_sqnorms(const double *X, int64_t n, int64_t ldx, int64_t d, double *norms,
+                     qmlh_stream_t stream);
+
+/* Gaussian kernel in the expanded form on FP64 MFMA (v_mfma_f64_16x16x4f64):
+ * K(i,j) = exp((-0.5/(sigma*sigma)) * max(nA_i + nB_j - 2 A_i.B_j, 0)), with nA, nB from
+ * qmlh_row_sqnorms. Same result as fgaussian_kernel (qml/fkernels.f90:340-372) to within
+ * 4 u (|A_i|^2 + |B_j|^2) / (2 sigma^2) in the exponent; the drop-in picks this form only
+ * where that bound is below 1e-13. Requires lda, ldb multiples of 4, d <= lda, ldb and the
+ * pad columns [d, round_up(d, 16)) zero (or readable and zero in both operands).
+ * With QMLH_SYMMETRIC the diagonal is exactly 1. */
+int qmlh_gaussian_mfma(const double *A, int64_t na, int64_t lda, const double *normsA,
+                       const double *B, int64_t nb, int64_t ldb, const double *normsB,
+                       int64_t d, double sigma, double *K, int64_t ldk, int32_t flags,
+                       qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Cholesky factorisation and solve.
+ * qmlh_potrf replaces LAPACK dpotrf("U") at qml/fcho_solve.f90:35. The reference factors the
+ * upper triangle (numpy indices i <= j) of a matrix Fortran sees column-major; the same bytes
+ * read row-major (a C-ordered numpy array) are that matrix's transpose, so this routine
+ * factors the LOWER triangle of a column-major matrix in place: A = L L^T, L in the lower
+ * triangle, the strictly upper part untouched. info (device int32): 0, or j+1 when the
+ * leading minor of order j+1 is not positive definite (LAPACK's convention); the remaining
+ * columns are then left unfactored, as dpotrf leaves them.
+ * qmlh_potrs replaces dpotrs("U", n, 1) at qml/fcho_solve.f90:46: x := (L L^T)^-1 x.
+ * ---------------------------------------------------------------------------------------- */
+int qmlh_potrf(double *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
+int qmlh_potrs(const double *L, int64_t n, int64_t lda, double *x, qmlh_stream_t stream);
+
+/* A(i,i) += value for i < n (the K[diag] += lambda step of tests/test_energy_krr_cmat.py:104). */
+int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Fused KRR prediction y_q = sum_j k(Xq_q, Xt_j) alpha_j without materialising the
+ * (m, n) kernel; the composition Ks.T.dot(alpha) of tests/test_energy_krr_cmat.py:108-109.
+ * kind: QMLH_PAIR_LAPLACIAN (VALU) or QMLH_PAIR_GAUSSIAN (MFMA expanded form; needs the
+ * norms and the padding rules of qmlh_gaussian_mfma). Every query's sum over the training
+ * set uses a fixed chunking that does not depend on m or on the other queries, so a query's
+ * result is bitwise the same alone or in any batch. ws: workspace of at least
+ * qmlh_predict_workspace(m, n) bytes.
+ * ---------------------------------------------------------------------------------------- */
+size_t qmlh_predict_workspace(int64_t m, int64_t n);
+int qmlh_krr_predict(int32_t kind, const double *Xq, int64_t m, int64_t ldq,
+                     const double *normsQ, const double *Xt, int64_t n, int64_t ldt,
+                     const double *normsT, const double *alpha, int64_t d, double sigma,
+                     double *y, void *ws, size_t ws_bytes, qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Measurement hooks (no counterpart in the reference; used by bench.py). When enabled, each
+ * heavy launch is bracketed by HIP events on its stream and its algorithmic work is added to
+ * its class: flops for the MFMA classes and the Cholesky, pair-features for the VALU pairwise
+ * class, molecules for the Coulomb matrices. qmlh_timer_read waits for the recorded events.
+ * ---------------------------------------------------------------------------------------- */
+enum {
+    QMLH_TIMER_SYRK = 0,      /* potrf trailing updates (lower, FP64 MFMA)        flops */
+    QMLH_TIMER_GEMM = 1,      /* potrf panel/diagonal-block GEMM updates         flops */
+    QMLH_TIMER_GAUSS = 2,     /* Gaussian kernel matrix, FP64 MFMA               flops */
+    QMLH_TIMER_PAIR = 3,      /* VALU pairwise kernels                  pair-features */
+    QMLH_TIMER_PREDICT = 4,   /* fused predict                            query-train */
+    QMLH_TIMER_POTRF = 5,     /* whole factorisation                      n^3/3 flops */
+    QMLH_TIMER_POTRS = 6,     /* both triangular solves                     2 n^2    */
+    QMLH_TIMER_CM = 7,        /* Coulomb matrices                           molecules */
+    QMLH_TIMER_CLASSES = 8
+};
+int qmlh_timer_enable(int32_t on);
+int qmlh_timer_read(int32_t timer_class, double *total_ms_h, int64_t *launches_h,
+                    double *work_h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMLHIP_H */

@@ -1,0 +1,229 @@
+"""CPU oracle for the KRR hot path.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline
+leg may import this package, and only as the checker or as the timed CPU baseline. The product
+path (``qml_amd``) never imports it and has no CPU fallback.
+
+Contents
+--------
+* ``liboracle.so`` (oracle.c): C restatement of the reference Fortran loops
+  (qml/frepresentations.f90:52-205, qml/fkernels.f90:340-400, qml/fdistance.f90:23-80),
+  built with ``make -C oracle``.
+* ``cho_solve``: qml/math.py:60-97 around qml/fcho_solve.f90:23-52, which is exactly LAPACK
+  ``dpotrf('U')`` + ``dpotrs('U', nrhs=1)``. LAPACK is a third-party dependency of the
+  reference (setup.py:24, unpinned ``-llapack``) and is absent from the reference tree; this
+  oracle calls the same two routines from scipy's bundled LAPACK.
+* ``ref``: the reference's own Fortran compiled from its sources by ``make -C oracle ref``
+  into ``oracle/_ref/libqmlref.so`` (present only where the reference tree was available at
+  build time). Used to pin the C restatement.
+
+Pinning (see tests/test_oracle.py): the reference's golden Coulomb-matrix fixtures
+(tests/test_representations.py:74-90 data), the water known answer
+(docs/source/examples.rst:37-56), the cho_solve fixture pair (tests/test_math.py:32-51),
+closed forms and brute force on small inputs, and bit-for-bit agreement with ``ref``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+_ref = None
+
+
+def _ptr(a, typ=_c_double_p):
+    return a.ctypes.data_as(typ)
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        i64 = ctypes.c_int64
+        for name in ("oracle_cm_sorted", "oracle_cm_unsorted"):
+            f = getattr(L, name)
+            f.argtypes = [_c_double_p, _c_double_p, i64, i64, _c_double_p]
+            f.restype = ctypes.c_int
+        L.oracle_cm_batch.argtypes = [_c_double_p, _c_double_p, _c_int64_p, i64, i64,
+                                      ctypes.c_int, _c_double_p]
+        L.oracle_cm_batch.restype = ctypes.c_int
+        for name in ("oracle_laplacian", "oracle_gaussian"):
+            f = getattr(L, name)
+            f.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, ctypes.c_double, _c_double_p]
+            f.restype = None
+        for name in ("oracle_manhattan", "oracle_l2"):
+            f = getattr(L, name)
+            f.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, _c_double_p]
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libqmlref.so"))
+
+
+def ref():
+    """Load the reference build (oracle/_ref/libqmlref.so)."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libqmlref.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference build missing: {path} (run make -C oracle ref)")
+        L = ctypes.CDLL(path)
+        ci = ctypes.c_int
+        L.ref_cm.argtypes = [_c_double_p, _c_double_p, ci, ci, ci, _c_double_p]
+        L.ref_cm.restype = None
+        L.ref_kernel.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p,
+                                 ctypes.c_double]
+        L.ref_kernel.restype = None
+        _ref = L
+    return _ref
+
+
+# --------------------------------------------------------------------------------------------
+# Coulomb matrices
+# --------------------------------------------------------------------------------------------
+
+def coulomb_matrix(Z, R, size=23, sorting="row-norm"):
+    """One molecule; mirrors qml/representations.py:69-119 (f64 coercion of Z and R)."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(size * (size + 1) // 2)
+    if sorting == "row-norm":
+        rc = lib().oracle_cm_sorted(_ptr(Z), _ptr(R), Z.shape[0], size, _ptr(out))
+    elif sorting == "unsorted":
+        rc = lib().oracle_cm_unsorted(_ptr(Z), _ptr(R), Z.shape[0], size, _ptr(out))
+    else:
+        raise ValueError(sorting)
+    if rc:
+        raise ValueError("natoms > size")
+    return out
+
+
+def coulomb_matrix_batch(Z, R, offsets, size, sorted_=True):
+    """Ragged batch (see qml_amd.synthetic); returns (nmol, size*(size+1)/2)."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    nmol = offsets.shape[0] - 1
+    out = np.zeros((nmol, size * (size + 1) // 2))
+    rc = lib().oracle_cm_batch(_ptr(Z), _ptr(R), _ptr(offsets, _c_int64_p), nmol, size,
+                               1 if sorted_ else 0, _ptr(out))
+    if rc:
+        raise ValueError("natoms > size")
+    return out
+
+
+def ref_coulomb_matrix(Z, R, size, sorting="row-norm"):
+    """The reference's own fgenerate_*coulomb_matrix via oracle/_ref."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(size * (size + 1) // 2)
+    ref().ref_cm(_ptr(Z), _ptr(R), Z.shape[0], size, 1 if sorting == "row-norm" else 0,
+                 _ptr(out))
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# Kernels and distances (outputs Fortran-ordered (na, nb), as qml/kernels.py:58)
+# --------------------------------------------------------------------------------------------
+
+def _prep(A, B):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    assert A.ndim == 2 and B.ndim == 2 and A.shape[1] == B.shape[1]
+    return A, B, np.empty((A.shape[0], B.shape[0]), order="F")
+
+
+def laplacian_kernel(A, B, sigma):
+    A, B, K = _prep(A, B)
+    lib().oracle_laplacian(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], sigma, _ptr(K))
+    return K
+
+
+def gaussian_kernel(A, B, sigma):
+    A, B, K = _prep(A, B)
+    lib().oracle_gaussian(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], sigma, _ptr(K))
+    return K
+
+
+def manhattan_distance(A, B):
+    A, B, D = _prep(A, B)
+    lib().oracle_manhattan(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], _ptr(D))
+    return D
+
+
+def l2_distance(A, B):
+    A, B, D = _prep(A, B)
+    lib().oracle_l2(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], _ptr(D))
+    return D
+
+
+REF_KIND = {"laplacian": 0, "gaussian": 1, "manhattan": 2, "l2": 3}
+
+
+def ref_kernel(kind, A, B, sigma=1.0):
+    """The reference's own f*_kernel / f*_distance via oracle/_ref."""
+    A, B, K = _prep(A, B)
+    ref().ref_kernel(REF_KIND[kind], _ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1],
+                     _ptr(K), float(sigma))
+    return K
+
+
+# --------------------------------------------------------------------------------------------
+# cho_solve: qml/math.py:60-97 + qml/fcho_solve.f90:23-52
+# --------------------------------------------------------------------------------------------
+
+def cho_solve(A, y):
+    """Solve A x = y with LAPACK dpotrf('U') + dpotrs('U'), restoring A as the reference does.
+
+    qml/fcho_solve.f90:35 factors the upper triangle (numpy indices i <= j) of the array
+    Fortran sees; qml/math.py:85-95 then restores the diagonal and copies lower -> upper.
+    Warnings on info != 0 are printed with the reference's wording and x is still returned.
+    """
+    from scipy.linalg import lapack
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("expected square matrix")
+    if len(y.shape) != 1 or y.shape[0] != A.shape[1]:
+        raise ValueError("expected matrix and vector of same stride size")
+    n = A.shape[0]
+    A_diag = A[np.diag_indices_from(A)]
+    c, info = lapack.dpotrf(A, lower=0, clean=0, overwrite_a=0)
+    if info > 0:
+        print(" WARNING: Error in LAPACK Cholesky decomposition DPOTRF().")
+        print(" WARNING: The", info, "-th leading order is not positive definite.")
+    elif info < 0:
+        print(" WARNING: Error in LAPACK Cholesky decomposition DPOTRF().")
+        print(" WARNING: The", -info, "-th argument had an illegal value.")
+    x, info2 = lapack.dpotrs(c, np.asarray(y, dtype=np.float64).reshape(n, 1), lower=0)
+    if info2 < 0:
+        print(" WARNING: Error in LAPACK Cholesky solver DPOTRS().")
+        print(" WARNING: The", -info2, "-th argument had an illegal value.")
+    A[np.diag_indices_from(A)] = A_diag
+    i_lower = np.tril_indices_from(A)
+    A.T[i_lower] = A[i_lower]
+    return x.reshape(n)
+
+
+def krr_fit_predict(X, y, Xs, kernel, sigma, lam):
+    """KRR composition of tests/test_energy_krr_cmat.py:101-111: returns (alpha, y_pred)."""
+    kern = laplacian_kernel if kernel == "laplacian" else gaussian_kernel
+    K = kern(X, X, sigma)
+    K[np.diag_indices_from(K)] += lam
+    alpha = cho_solve(K, np.asarray(y, dtype=np.float64))
+    Ks = kern(X, Xs, sigma)
+    return alpha, Ks.T.dot(alpha)

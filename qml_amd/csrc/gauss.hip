@@ -1,0 +1,140 @@
+// Gaussian kernel matrix in the expanded form on FP64 MFMA:
+//   K(i, j) = exp(scale * max(|a_i|^2 + |b_j|^2 - 2 a_i.b_j, 0)),  scale = -0.5 / (sigma*sigma),
+// the same quantity qml/fkernels.f90:340-372 computes with direct differences. The cross term
+// is the one dense contraction of the path; norms and the exp argument stay FP64.
+//
+// A 256-lane workgroup owns a 128 x 128 output tile (2 x 2 wavefronts of 64 x 64). The MFMA is
+// fed (B rows, A rows) so that the 16 lanes holding one accumulator element run down 16
+// consecutive output rows: every store is a 128-byte run of the column-major K. In symmetric
+// mode only tiles with bi <= bj run; a diagonal tile writes i <= j and mirrors, an
+// off-diagonal tile writes both (i, j) and (j, i), so K is exactly symmetric with unit diagonal.
+#include "mfma_f64.h"
+#include "timer.h"
+
+namespace qmlh {
+
+static constexpr int GK_TILE = 128;
+
+__global__ __launch_bounds__(256) void row_sqnorms_kernel(const double* __restrict__ X,
+                                                          int64_t n, int64_t ld, int64_t d,
+                                                          double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* x = X + i * ld;
+    double s = 0.0;
+    for (int64_t k = 0; k < d; ++k) s = fma(x[k], x[k], s);
+    out[i] = s;
+}
+
+__device__ __forceinline__ void tile_coords(int64_t bid, int symmetric, int64_t tiles_m,
+                                            int64_t& bi, int64_t& bj) {
+    if (symmetric) {
+        int64_t j = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+        while (j * (j + 1) / 2 > bid) --j;
+        while ((j + 1) * (j + 2) / 2 <= bid) ++j;
+        bj = j;
+        bi = bid - j * (j + 1) / 2;
+    } else {
+        bi = bid % tiles_m;
+        bj = bid / tiles_m;
+    }
+}
+
+__global__ __launch_bounds__(256) void gaussian_mfma_kernel(
+    const double* __restrict__ A, int64_t lda, int64_t na, const double* __restrict__ nA,
+    const double* __restrict__ B, int64_t ldb, int64_t nb, const double* __restrict__ nB,
+    int64_t d, double scale, double* __restrict__ K, int64_t ldk, int symmetric,
+    int64_t tiles_m) {
+    int64_t bi, bj;
+    tile_coords(blockIdx.x, symmetric, tiles_m, bi, bj);
+    const int w = threadIdx.x >> 6;
+    const int64_t row0 = bi * GK_TILE + 64 * (w & 1);  // output rows (A)
+    const int64_t col0 = bj * GK_TILE + 64 * (w >> 1);  // output columns (B)
+    if (row0 >= na || col0 >= nb) return;
+
+    Operand<LAYOUT_ROW> opA{A, lda, na, d};
+    Operand<LAYOUT_ROW> opB{B, ldb, nb, d};
+    WaveTile t;
+    wave_tile_zero(t);
+    // First operand indexes output columns, second indexes output rows (see header note).
+    wave_tile_mma(t, opB, opA, col0, row0, d);
+
+    const int lane = threadIdx.x & 63;
+    const bool diag_tile = symmetric && bi == bj;
+    const bool mirror = symmetric && bi != bj;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + (lane & 15);
+            const int64_t j0 = col0 + 16 * a + 4 * (lane >> 4);
+            if (i >= na) continue;
+            const double ni = nA[i];
+            double v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = j0 + e;
+                double r = 0.0;
+                if (j < nb) {
+                    double d2 = ni + nB[j] - 2.0 * t.acc[a][b][e];
+                    d2 = d2 > 0.0 ? d2 : 0.0;
+                    if (symmetric && i == j) d2 = 0.0;
+                    r = exp(scale * d2);
+                }
+                v[e] = r;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = j0 + e;
+                if (j >= nb) continue;
+                if (diag_tile && i > j) continue;
+                K[i + j * ldk] = v[e];
+                if (mirror || (diag_tile && i < j)) K[j + i * ldk] = v[e];
+            }
+        }
+    }
+}
+
+}  // namespace qmlh
+
+extern "C" int qmlh_row_sqnorms(const double* X, int64_t n, int64_t ldx, int64_t d,
+                                double* norms, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || d < 0 || ldx < d) return set_error(QMLH_ERR_ARG, "row_sqnorms: bad shape");
+    if (n == 0) return QMLH_OK;
+    if (!X || !norms) return set_error(QMLH_ERR_ARG, "row_sqnorms: null pointer");
+    hipLaunchKernelGGL(row_sqnorms_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, X, n, ldx, d, norms);
+    QMLH_CHECK_LAUNCH("row_sqnorms launch");
+    return QMLH_OK;
+}
+
+extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, const double* normsA,
+                                  const double* B, int64_t nb, int64_t ldb,
+                                  const double* normsB, int64_t d, double sigma, double* K,
+                                  int64_t ldk, int32_t flags, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (na < 0 || nb < 0 || d < 0) return set_error(QMLH_ERR_ARG, "gaussian_mfma: negative size");
+    if (lda % 4 || ldb % 4) return set_error(QMLH_ERR_ARG, "gaussian_mfma: lda, ldb must be multiples of 4");
+    if (lda < d || ldb < d) return set_error(QMLH_ERR_ARG, "gaussian_mfma: ld < d");
+    if (ldk < na) return set_error(QMLH_ERR_ARG, "gaussian_mfma: ldk < na");
+    if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15)
+        return set_error(QMLH_ERR_ARG, "gaussian_mfma: A and B must be 16-byte aligned");
+    bool sym = (flags & QMLH_SYMMETRIC) != 0;
+    if (sym && na != nb) return set_error(QMLH_ERR_ARG, "gaussian_mfma: symmetric needs na == nb");
+    if (na == 0 || nb == 0) return QMLH_OK;
+    if (!A || !B || !K || !normsA || !normsB)
+        return set_error(QMLH_ERR_ARG, "gaussian_mfma: null pointer");
+    int64_t tm = ceil_div(na, GK_TILE), tn = ceil_div(nb, GK_TILE);
+    int64_t ntiles = sym ? tm * (tm + 1) / 2 : tm * tn;
+    if (ntiles > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "gaussian_mfma: grid too large");
+    // Work actually performed: full tiles, or the upper tile triangle in symmetric mode.
+    double work = 2.0 * (double)d * (sym ? (double)na * (double)(na + 1) / 2.0
+                                         : (double)na * (double)nb);
+    TimerScope ts(QMLH_TIMER_GAUSS, work, (hipStream_t)stream);
+    hipLaunchKernelGGL(gaussian_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0,
+                       (hipStream_t)stream, A, lda, na, normsA, B, ldb, nb, normsB, d,
+                       -0.5 / (sigma * sigma), K, ldk, sym ? 1 : 0, tm);
+    QMLH_CHECK_LAUNCH("gaussian_mfma launch");
+    return QMLH_OK;
+}

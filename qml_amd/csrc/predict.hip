@@ -1,0 +1,207 @@
+// Fused KRR prediction: y_q = sum_j k(q, x_j) alpha_j without materialising the kernel,
+// the Ks.T.dot(alpha) step of tests/test_energy_krr_cmat.py:108-109.
+//
+// Batch invariance: the training set is cut into fixed chunks (256 rows for the Gaussian,
+// 128 for the Laplacian); a workgroup reduces one chunk for a block of queries in an order
+// that depends only on the training index, writes one partial per (chunk, query), and a second
+// kernel adds the partials of a query in chunk order. Nothing in a query's arithmetic depends
+// on the other queries or on m, so a query's result is bitwise the same alone or in a batch.
+#include "mfma_f64.h"
+#include "timer.h"
+
+namespace qmlh {
+
+static constexpr int PG_Q = 64;     // queries per Gaussian workgroup
+static constexpr int PG_T = 256;    // training rows per Gaussian workgroup (4 waves x 64)
+static constexpr int PL_Q = 128;    // queries per Laplacian workgroup
+static constexpr int PL_T = 128;    // training rows per Laplacian workgroup
+static constexpr int PL_BK = 16;
+static constexpr int PL_LDS = 130;
+
+// Gaussian: MFMA cross term, FP64 epilogue, per-chunk partials part[chunk * m + q].
+__global__ __launch_bounds__(256) void predict_gauss_kernel(
+    const double* __restrict__ Q, int64_t ldq, int64_t m, const double* __restrict__ nQ,
+    const double* __restrict__ T, int64_t ldt, int64_t n, const double* __restrict__ nT,
+    const double* __restrict__ alpha, int64_t d, double scale, double* __restrict__ part) {
+    __shared__ double red[4][PG_Q];
+    const int64_t q0 = (int64_t)blockIdx.x * PG_Q;
+    const int64_t chunk = blockIdx.y;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t t0 = chunk * PG_T + 64 * w;
+
+    double qsum[4] = {0.0, 0.0, 0.0, 0.0};  // per query block b, this lane's query (lane & 15)
+    if (t0 < n) {
+        Operand<LAYOUT_ROW> opT{T, ldt, n, d};
+        Operand<LAYOUT_ROW> opQ{Q, ldq, m, d};
+        WaveTile t;
+        wave_tile_zero(t);
+        // First operand: training rows (accumulator rows), second: queries (columns).
+        wave_tile_mma(t, opT, opQ, t0, q0, d);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t q = q0 + 16 * b + (lane & 15);
+            const int64_t qc = q < m ? q : m - 1;
+            const double nq = nQ[qc];
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t j = t0 + 16 * a + 4 * (lane >> 4) + e;
+                    if (j < n) {
+                        double d2 = nq + nT[j] - 2.0 * t.acc[a][b][e];
+                        d2 = d2 > 0.0 ? d2 : 0.0;
+                        s = fma(exp(scale * d2), alpha[j], s);
+                    }
+                }
+            }
+            // Lanes l, l+16, l+32, l+48 hold the same query: fixed-order butterfly.
+            s = s + __shfl_xor(s, 16, 64);
+            s = s + __shfl_xor(s, 32, 64);
+            qsum[b] = s;
+        }
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) red[w][16 * b + lane] = qsum[b];
+    }
+    __syncthreads();
+    if (threadIdx.x < PG_Q) {
+        const int64_t q = q0 + threadIdx.x;
+        if (q < m) {
+            const int qi = threadIdx.x;
+            double s = ((red[0][qi] + red[1][qi]) + red[2][qi]) + red[3][qi];
+            part[chunk * m + q] = s;
+        }
+    }
+}
+
+// Laplacian: VALU L1 tile (as the pairwise kernel), exp * alpha, per-chunk partials.
+__global__ __launch_bounds__(256, 2) void predict_laplace_kernel(
+    const double* __restrict__ Q, int64_t ldq, int64_t m, const double* __restrict__ T,
+    int64_t ldt, int64_t n, const double* __restrict__ alpha, int64_t d, double scale,
+    double* __restrict__ part) {
+    __shared__ __attribute__((aligned(16))) double sA[PL_BK][PL_LDS];
+    __shared__ __attribute__((aligned(16))) double sB[PL_BK][PL_LDS];
+    __shared__ double red[16][PL_Q];
+    const int64_t row0 = (int64_t)blockIdx.x * PL_Q;  // queries
+    const int64_t chunk = blockIdx.y;
+    const int64_t col0 = chunk * PL_T;  // training rows
+    const int t = threadIdx.x;
+    const int tx = t % 16, ty = t / 16;
+    double acc[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+    for (int64_t k0 = 0; k0 < d; k0 += PL_BK) {
+        const int k = t % PL_BK;
+        const int rb = t / PL_BK;
+#pragma unroll
+        for (int qd = 0; qd < 8; ++qd) {
+            const int r = rb + 16 * qd;
+            const int64_t kk = k0 + k;
+            const int64_t ra = row0 + r, rt = col0 + r;
+            sA[k][r] = (ra < m && kk < d) ? Q[ra * ldq + kk] : 0.0;
+            sB[k][r] = (rt < n && kk < d) ? T[rt * ldt + kk] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kq = 0; kq < PL_BK; ++kq) {
+            double a[8], b[8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 va = *reinterpret_cast<const double2*>(&sA[kq][2 * tx + 32 * r]);
+                const double2 vb = *reinterpret_cast<const double2*>(&sB[kq][2 * ty + 32 * r]);
+                a[2 * r] = va.x; a[2 * r + 1] = va.y;
+                b[2 * r] = vb.x; b[2 * r + 1] = vb.y;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[r][c] = acc[r][c] + fabs(a[r] - b[c]);
+        }
+        __syncthreads();
+    }
+    // Row sums over this lane's 8 training columns, in column order.
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int64_t j = col0 + 2 * ty + 32 * (c / 2) + (c % 2);
+            if (j < n) s = fma(exp(scale * acc[r][c]), alpha[j], s);
+        }
+        red[ty][2 * tx + 32 * (r / 2) + (r % 2)] = s;
+    }
+    __syncthreads();
+    if (t < PL_Q) {
+        const int64_t q = row0 + t;
+        if (q < m) {
+            double s = 0.0;
+            for (int g = 0; g < 16; ++g) s = s + red[g][t];
+            part[chunk * m + q] = s;
+        }
+    }
+}
+
+__global__ void predict_reduce_kernel(const double* __restrict__ part, int64_t m, int64_t nchunks,
+                                      double* __restrict__ y) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    double s = 0.0;
+    for (int64_t c = 0; c < nchunks; ++c) s = s + part[c * m + q];
+    y[q] = s;
+}
+
+}  // namespace qmlh
+
+extern "C" size_t qmlh_predict_workspace(int64_t m, int64_t n) {
+    using namespace qmlh;
+    if (m <= 0 || n <= 0) return 0;
+    int64_t chunks = ceil_div(n, PL_T);  // the finer of the two chunkings
+    return (size_t)(chunks * m) * sizeof(double);
+}
+
+extern "C" int qmlh_krr_predict(int32_t kind, const double* Xq, int64_t m, int64_t ldq,
+                                const double* normsQ, const double* Xt, int64_t n, int64_t ldt,
+                                const double* normsT, const double* alpha, int64_t d,
+                                double sigma, double* y, void* ws, size_t ws_bytes,
+                                qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (m < 0 || n < 0 || d < 0) return set_error(QMLH_ERR_ARG, "krr_predict: negative size");
+    if (ldq < d || ldt < d) return set_error(QMLH_ERR_ARG, "krr_predict: ld < d");
+    if (m == 0) return QMLH_OK;
+    if (!Xq || !Xt || !alpha || !y) return set_error(QMLH_ERR_ARG, "krr_predict: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) {
+        return check_hip(hipMemsetAsync(y, 0, sizeof(double) * m, s), "krr_predict: zero y");
+    }
+    if (ws_bytes < qmlh_predict_workspace(m, n) || !ws)
+        return set_error(QMLH_ERR_WORKSPACE, "krr_predict: workspace too small");
+    double* part = static_cast<double*>(ws);
+    int64_t nchunks;
+    TimerScope ts(QMLH_TIMER_PREDICT, (double)m * (double)n, s);
+    if (kind == QMLH_PAIR_GAUSSIAN) {
+        if (ldq % 4 || ldt % 4)
+            return set_error(QMLH_ERR_ARG, "krr_predict: ldq, ldt must be multiples of 4");
+        if (!normsQ || !normsT) return set_error(QMLH_ERR_ARG, "krr_predict: norms missing");
+        nchunks = ceil_div(n, PG_T);
+        dim3 grid((unsigned)ceil_div(m, PG_Q), (unsigned)nchunks);
+        hipLaunchKernelGGL(predict_gauss_kernel, grid, dim3(256), 0, s, Xq, ldq, m, normsQ, Xt,
+                           ldt, n, normsT, alpha, d, -0.5 / (sigma * sigma), part);
+    } else if (kind == QMLH_PAIR_LAPLACIAN) {
+        nchunks = ceil_div(n, PL_T);
+        dim3 grid((unsigned)ceil_div(m, PL_Q), (unsigned)nchunks);
+        hipLaunchKernelGGL(predict_laplace_kernel, grid, dim3(256), 0, s, Xq, ldq, m, Xt, ldt, n,
+                           alpha, d, -1.0 / sigma, part);
+    } else {
+        return set_error(QMLH_ERR_ARG, "krr_predict: kind must be Gaussian or Laplacian");
+    }
+    QMLH_CHECK_LAUNCH("krr_predict launch");
+    hipLaunchKernelGGL(predict_reduce_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s,
+                       part, m, nchunks, y);
+    QMLH_CHECK_LAUNCH("krr_predict reduce launch");
+    return QMLH_OK;
+}

@@ -1,0 +1,152 @@
+"""Device-resident kernel ridge regression: the KRR composition of
+tests/test_energy_krr_cmat.py:101-111 (K = kernel(X, X); K[diag] += lambda;
+alpha = cho_solve(K, y); y_pred = kernel(X, Xs).T . alpha) with every array in HBM.
+
+The training representations, their FP64 squared norms and alpha stay on the device after
+``fit``; ``predict`` runs the fused kernel-times-alpha reduction, so the (m, n) prediction
+kernel is never materialised and K never crosses PCIe.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .device import device, padded_ld, ragged_to_device, rows_to_device
+from .kernels import gaussian_mfma_device, pairwise_device, row_sqnorms_device
+from .math import add_diagonal_device, potrf_device, potrs_device, print_potrf_warning
+from .representations import cm_length, generate_coulomb_matrices_device
+
+KERNELS = {"gaussian": _lib.PAIR_GAUSSIAN, "laplacian": _lib.PAIR_LAPLACIAN}
+
+
+class CholeskyError(RuntimeError):
+    """The regularised kernel matrix is not numerically positive definite (LAPACK info > 0)."""
+
+
+@dataclass
+class FitReport:
+    n: int
+    info: int
+    seconds: dict = field(default_factory=dict)
+
+
+class KRR:
+    """Kernel ridge regression with the training set resident in HBM.
+
+    kernel: "gaussian" (FP64 MFMA expanded form) or "laplacian" (FP64 VALU).
+    """
+
+    def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9):
+        if kernel not in KERNELS:
+            raise ValueError(f"kernel must be one of {sorted(KERNELS)}")
+        self.kernel = kernel
+        self.sigma = float(sigma)
+        self.lam = float(lam)
+        self.Xd = None
+        self.norms = None
+        self.alpha = None
+        self.n = 0
+        self.d = 0
+
+    # -- kernel matrix ------------------------------------------------------------------------
+    def build_kernel(self, Xd, n, d, norms=None, out=None, stream=None):
+        """Symmetric (n, n) training kernel, column-major, as an (n, n) device tensor."""
+        if self.kernel == "gaussian":
+            if norms is None:
+                norms = row_sqnorms_device(Xd, n, d, stream)
+            return gaussian_mfma_device(Xd, norms, n, Xd, norms, n, d, self.sigma, True, out,
+                                        stream)
+        return pairwise_device(_lib.PAIR_LAPLACIAN, Xd, n, Xd, n, d, self.sigma, True, out, stream)
+
+    # -- fit ----------------------------------------------------------------------------------
+    def fit_device(self, Xd, d: int, y, raise_on_error: bool = True, stream=None) -> FitReport:
+        """Fit from an (n, ld) device representation tensor; y host or device (n,)."""
+        import torch
+
+        n = Xd.shape[0]
+        rep = FitReport(n=n, info=0)
+        t0 = time.perf_counter()
+        norms = row_sqnorms_device(Xd, n, d, stream) if self.kernel == "gaussian" else None
+        K = self.build_kernel(Xd, n, d, norms, stream=stream)
+        add_diagonal_device(K, n, self.lam, stream)
+        info = potrf_device(K, n, stream)
+        alpha = torch.as_tensor(y, dtype=torch.float64).to(Xd.device).clone()
+        potrs_device(K, n, alpha, stream)
+        rep.info = int(info.item())
+        rep.seconds["fit"] = time.perf_counter() - t0
+        del K
+        if rep.info != 0:
+            print_potrf_warning(rep.info)
+            if raise_on_error:
+                raise CholeskyError(f"Cholesky failed: info={rep.info}")
+        self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d
+        return rep
+
+    def fit(self, X, y, **kw) -> FitReport:
+        """Fit from host (n, d) representations."""
+        X = np.asarray(X, dtype=np.float64)
+        return self.fit_device(rows_to_device(X), X.shape[1], y, **kw)
+
+    def fit_molecules(self, Z, R, offsets, size: int, y, sorting: str = "row-norm", **kw):
+        """Fit straight from raw molecules: Coulomb matrices are generated on the GPU."""
+        Zd, Rd, Od = ragged_to_device(Z, R, offsets)
+        Xd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size, sorting)
+        return self.fit_device(Xd, cm_length(size), y, **kw)
+
+    # -- predict ------------------------------------------------------------------------------
+    def predict_device(self, Qd, m: int, out=None, workspace=None, qnorms=None, stream=None):
+        """Predictions for (m, ld) device query rows; returns an (m,) device tensor."""
+        import torch
+
+        if self.alpha is None:
+            raise RuntimeError("fit first")
+        if Qd.shape[1] != self.Xd.shape[1]:
+            raise ValueError("query rows must use the training feature layout")
+        y = torch.empty(m, dtype=torch.float64, device=Qd.device) if out is None else out
+        if m == 0:
+            return y
+        L = _lib.lib()
+        ws_bytes = L.qmlh_predict_workspace(m, self.n)
+        if workspace is None or workspace.numel() * workspace.element_size() < ws_bytes:
+            workspace = torch.empty(max(1, ws_bytes // 8), dtype=torch.float64, device=Qd.device)
+        if self.kernel == "gaussian" and qnorms is None:
+            qnorms = row_sqnorms_device(Qd, m, self.d, stream)
+        _lib.check(L.qmlh_krr_predict(
+            KERNELS[self.kernel], _lib.ptr(Qd), m, Qd.shape[1], _lib.ptr(qnorms),
+            _lib.ptr(self.Xd), self.n, self.Xd.shape[1], _lib.ptr(self.norms),
+            _lib.ptr(self.alpha), self.d, self.sigma, _lib.ptr(y), _lib.ptr(workspace),
+            workspace.numel() * workspace.element_size(), _lib.stream_handle(stream)))
+        return y
+
+    def predict(self, Xq) -> np.ndarray:
+        Xq = np.asarray(Xq, dtype=np.float64)
+        Qd = rows_to_device(Xq, ld=self.Xd.shape[1])
+        return self.predict_device(Qd, Xq.shape[0]).cpu().numpy()
+
+    def predict_molecules(self, Z, R, offsets, size: int, sorting: str = "row-norm"):
+        Zd, Rd, Od = ragged_to_device(Z, R, offsets)
+        Qd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size, sorting, ld=self.Xd.shape[1])
+        return self.predict_device(Qd, Qd.shape[0]).cpu().numpy()
+
+    # -- checkpoint ---------------------------------------------------------------------------
+    def save(self, path: str) -> None:
+        """npz snapshot of what serving needs: representations, norms, alpha, hyperparameters."""
+        np.savez(path, X=self.Xd[:, :self.d].cpu().numpy(), alpha=self.alpha.cpu().numpy(),
+                 kernel=self.kernel, sigma=self.sigma, lam=self.lam, d=self.d)
+
+    @classmethod
+    def load(cls, path: str) -> "KRR":
+        z = np.load(path, allow_pickle=False)
+        m = cls(str(z["kernel"]), float(z["sigma"]), float(z["lam"]))
+        m.d = int(z["d"])
+        m.Xd = rows_to_device(z["X"])
+        m.n = m.Xd.shape[0]
+        m.norms = row_sqnorms_device(m.Xd, m.n, m.d) if m.kernel == "gaussian" else None
+        import torch
+
+        m.alpha = torch.from_numpy(z["alpha"]).to(m.Xd.device)
+        return m

@@ -1,0 +1,81 @@
+"""Cholesky solve on the GPU: the drop-in for qml.math.cho_solve (qml/math.py:60-97).
+
+The reference calls LAPACK dpotrf("U") and dpotrs("U", n, 1) (qml/fcho_solve.f90:35,46) on the
+upper triangle (numpy indices i <= j), prints a WARNING on a non-zero info and still returns x,
+then restores the diagonal of A and copies its lower triangle onto the upper one.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .device import device
+
+
+def potrf_device(Ad, n: int, stream=None):
+    """In-place lower Cholesky of a column-major (n, n) device matrix (an (n, n) tensor whose
+    row-major bytes are the column-major matrix). Returns the device info tensor."""
+    import torch
+
+    info = torch.zeros(1, dtype=torch.int32, device=Ad.device)
+    _lib.check(_lib.lib().qmlh_potrf(_lib.ptr(Ad), n, n, _lib.ptr(info),
+                                     _lib.stream_handle(stream)))
+    return info
+
+
+def potrs_device(Ld, n: int, xd, stream=None):
+    """x := (L L^T)^-1 x with L from potrf_device; x is an (n,) FP64 device tensor."""
+    _lib.check(_lib.lib().qmlh_potrs(_lib.ptr(Ld), n, n, _lib.ptr(xd), _lib.stream_handle(stream)))
+    return xd
+
+
+def add_diagonal_device(Ad, n: int, value: float, stream=None):
+    _lib.check(_lib.lib().qmlh_add_diagonal(_lib.ptr(Ad), n, n, float(value),
+                                            _lib.stream_handle(stream)))
+
+
+def print_potrf_warning(info: int) -> None:
+    """The messages of qml/fcho_solve.f90:36-42 (Fortran list-directed output)."""
+    if info > 0:
+        print(" WARNING: Error in LAPACK Cholesky decomposition DPOTRF().")
+        print(f" WARNING: The {info:11d} -th leading order is not positive definite.")
+    elif info < 0:
+        print(" WARNING: Error in LAPACK Cholesky decomposition DPOTRF().")
+        print(f" WARNING: The {-info:11d} -th argument had an illegal value.")
+
+
+def _copy_lower_to_upper(A, block: int = 2048) -> None:
+    """A.T[tril] = A[tril] (qml/math.py:94-95) without the N(N+1)/2 index arrays."""
+    n = A.shape[0]
+    for r0 in range(0, n, block):
+        r1 = min(n, r0 + block)
+        D = A[r0:r1, r0:r1]
+        iu = np.triu_indices(r1 - r0, 1)
+        D[iu] = D.T[iu]
+        if r1 < n:
+            A[r0:r1, r1:] = A[r1:, r0:r1].T
+
+
+def cho_solve(A, y):
+    """Solve A x = y by Cholesky on the GPU; same signature, errors and side effects as
+    qml.math.cho_solve (qml/math.py:60-97)."""
+    import torch
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError('expected square matrix')
+    if len(y.shape) != 1 or y.shape[0] != A.shape[1]:
+        raise ValueError('expected matrix and vector of same stride size')
+    n = A.shape[0]
+    x = np.zeros(n)
+    if n == 0:
+        return x
+    # C-ordered bytes of A read column-major are A^T, whose lower triangle is A's upper one.
+    Ad = torch.from_numpy(np.ascontiguousarray(A, dtype=np.float64)).to(device())
+    info = potrf_device(Ad, n)
+    xd = torch.from_numpy(np.array(y, dtype=np.float64)).to(device())
+    potrs_device(Ad, n, xd)
+    x[:] = xd.cpu().numpy()
+    print_potrf_warning(int(info.item()))
+    _copy_lower_to_upper(A)
+    return x

@@ -1,0 +1,46 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libqmlhip.so)")
+
+
+@pytest.fixture(scope="session")
+def qm7():
+    z = np.load(os.path.join(GOLDEN, "qm7_subset.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def ref_vectors():
+    z = np.load(os.path.join(GOLDEN, "ref_vectors.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def cho_pair():
+    z = np.load(os.path.join(GOLDEN, "cho_solve_pair.npz"), allow_pickle=False)
+    return z["K"], z["y"]
+
+
+def qm7_molecules(qm7, names):
+    index = {str(n): i for i, n in enumerate(qm7["names"])}
+    out = []
+    for name in names:
+        i = index[str(name)]
+        a, b = qm7["offsets"][i], qm7["offsets"][i + 1]
+        out.append((qm7["Z"][a:b], qm7["R"][a:b], qm7["hof"][i]))
+    return out
+
+
+def bits(x):
+    return np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)

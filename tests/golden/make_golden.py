@@ -1,0 +1,108 @@
+"""Write the golden fixtures under tests/golden/ (run once, in a container that has the
+reference tree at /root/reference and the reference build oracle/_ref).
+
+Outputs (data only):
+  qm7_subset.npz        QM7 molecules from the reference's tests/qm7/*.xyz: the ten files
+                        0101-0110 used by tests/test_representations.py:47-58 and the first
+                        1000 names of tests/data/hof_qm7.txt used by
+                        tests/test_energy_krr_cmat.py:59-75 (Z, R ragged, heats of formation).
+  cm_row-norm_sorted.txt, cm_unsorted.txt
+                        the reference's tests/data/coulomb_matrix_representation_*.txt.
+  cho_solve_pair.npz    tests/data/K_local_gaussian.txt and y_cho_solve.txt (%.18e text,
+                        stored losslessly as float64).
+  ref_vectors.npz       outputs of the reference Fortran (oracle/_ref, one OpenMP thread) on
+                        seeded synthetic molecules and random matrices: sorted and unsorted
+                        Coulomb matrices (size 29) and the four pairwise matrices.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+REF = "/root/reference"
+NUCLEAR_CHARGE = {"H": 1, "C": 6, "N": 7, "O": 8, "S": 16, "F": 9, "Cl": 17, "P": 15}
+
+
+def read_xyz(path):
+    with open(path) as f:
+        lines = f.readlines()
+    n = int(lines[0])
+    Z, R = [], []
+    for line in lines[2:n + 2]:
+        tok = line.split()
+        if len(tok) < 4:
+            break
+        Z.append(NUCLEAR_CHARGE[tok[0]])
+        R.append([float(t) for t in tok[1:4]])
+    return np.array(Z, dtype=np.int32), np.array(R, dtype=np.float64)
+
+
+def qm7_subset():
+    names_rep = [f"{i:04d}.xyz" for i in range(101, 111)]
+    hof = {}
+    with open(os.path.join(REF, "tests/data/hof_qm7.txt")) as f:
+        for line in f:
+            tok = line.split()
+            hof[tok[0]] = float(tok[1])
+    names_krr = sorted(hof.keys())[:1000]
+    names = names_rep + [n for n in names_krr if n not in names_rep]
+    Zs, Rs, nat = [], [], []
+    for name in names:
+        Z, R = read_xyz(os.path.join(REF, "tests/qm7", name))
+        Zs.append(Z)
+        Rs.append(R)
+        nat.append(len(Z))
+    offsets = np.zeros(len(names) + 1, dtype=np.int64)
+    np.cumsum(nat, out=offsets[1:])
+    np.savez_compressed(os.path.join(HERE, "qm7_subset.npz"), names=np.array(names),
+                        Z=np.concatenate(Zs), R=np.concatenate(Rs), offsets=offsets,
+                        hof=np.array([hof.get(n, np.nan) for n in names]),
+                        rep_names=np.array(names_rep), krr_names=np.array(names_krr))
+
+
+def copy_reference_fixtures():
+    data = os.path.join(REF, "tests/data")
+    shutil.copyfile(os.path.join(data, "coulomb_matrix_representation_row-norm_sorted.txt"),
+                    os.path.join(HERE, "cm_row-norm_sorted.txt"))
+    shutil.copyfile(os.path.join(data, "coulomb_matrix_representation_unsorted.txt"),
+                    os.path.join(HERE, "cm_unsorted.txt"))
+    K = np.loadtxt(os.path.join(data, "K_local_gaussian.txt"))
+    y = np.loadtxt(os.path.join(data, "y_cho_solve.txt"))
+    np.savez_compressed(os.path.join(HERE, "cho_solve_pair.npz"), K=K, y=y)
+
+
+def ref_vectors():
+    os.environ["OMP_NUM_THREADS"] = "1"
+    sys.path.insert(0, ROOT)
+    import oracle
+    from qml_amd import synthetic
+
+    mols = synthetic.generate(64, 2024, 5, 29)
+    Z, R, off = mols.ragged()
+    cm_sorted = np.stack([oracle.ref_coulomb_matrix(*mols.molecule(i), 29, "row-norm")
+                          for i in range(mols.nmol)])
+    cm_unsorted = np.stack([oracle.ref_coulomb_matrix(*mols.molecule(i), 29, "unsorted")
+                            for i in range(mols.nmol)])
+    rng = np.random.default_rng(2024)
+    A = rng.random((45, 77)) * 40.0
+    B = rng.random((33, 77)) * 40.0
+    out = dict(Z=Z, R=R, offsets=off, cm_sorted=cm_sorted, cm_unsorted=cm_unsorted, A=A, B=B,
+               sigma_lap=55.0, sigma_gauss=90.0)
+    out["K_laplacian"] = oracle.ref_kernel("laplacian", A, B, 55.0)
+    out["K_gaussian"] = oracle.ref_kernel("gaussian", A, B, 90.0)
+    out["D_manhattan"] = oracle.ref_kernel("manhattan", A, B)
+    out["D_l2"] = oracle.ref_kernel("l2", A, B)
+    np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
+
+
+if __name__ == "__main__":
+    qm7_subset()
+    copy_reference_fixtures()
+    ref_vectors()
+    print("golden fixtures written to", HERE)
