@@ -67,13 +67,12 @@ __global__ __launch_bounds__(256) void gaussian_mfma_kernel(
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int64_t i = row0 + 16 * b + (lane & 15);
-            const int64_t j0 = col0 + 16 * a + 4 * (lane >> 4);
             if (i >= na) continue;
             const double ni = nA[i];
             double v[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = j0 + e;
+                const int64_t j = col0 + tile_row(a, e);
                 double r = 0.0;
                 if (j < nb) {
                     double d2 = ni + nB[j] - 2.0 * t.acc[a][b][e];
@@ -85,7 +84,7 @@ __global__ __launch_bounds__(256) void gaussian_mfma_kernel(
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = j0 + e;
+                const int64_t j = col0 + tile_row(a, e);
                 if (j >= nb) continue;
                 if (diag_tile && i > j) continue;
                 K[i + j * ldk] = v[e];

@@ -2,7 +2,8 @@
 //
 // One wavefront owns a 64 x 64 tile held as 4 x 4 MFMA blocks (16 x double4 accumulators).
 // Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A(l % 16, l / 16) and
-// B(l / 16, l % 16); the result block D(i, j) sits in lane j + 16 * (i / 4), element i % 4.
+// B(l / 16, l % 16); the result block D(i, j) sits in lane j + 16 * (i % 4), element i / 4
+// (rows interleave across the four lane groups; checked on gfx950 by the parity tests).
 //
 // Fragments are read straight from global memory (the FP64 MFMA issues once per 64 cycles, so
 // a 16-deep k chunk keeps a wave busy ~4k cycles, long enough to cover the loads of the next
@@ -120,7 +121,7 @@ __device__ __forceinline__ void wave_tile_mma(WaveTile& t, const Operand<LA>& A,
 // Row/column of accumulator element e (0..3) of block (a, b) for this lane.
 __device__ __forceinline__ int tile_row(int a, int e) {
     const int lane = threadIdx.x & 63;
-    return 16 * a + 4 * (lane >> 4) + e;
+    return 16 * a + (lane >> 4) + 4 * e;
 }
 __device__ __forceinline__ int tile_col(int b) {
     const int lane = threadIdx.x & 63;

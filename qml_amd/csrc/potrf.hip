@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(
             if (i >= M) continue;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + 16 * a + 4 * (lane >> 4) + e;
+                const int64_t j = col0 + tile_row(a, e);
                 if (j >= N || (LOWER && j > i)) continue;
                 double* c = C + i + j * ldc;
                 *c = *c + alpha * t.acc[a][b][e];

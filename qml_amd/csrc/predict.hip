@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void predict_gauss_kernel(
             for (int a = 0; a < 4; ++a) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int64_t j = t0 + 16 * a + 4 * (lane >> 4) + e;
+                    const int64_t j = t0 + tile_row(a, e);
                     if (j < n) {
                         double d2 = nq + nT[j] - 2.0 * t.acc[a][b][e];
                         d2 = d2 > 0.0 ? d2 : 0.0;
