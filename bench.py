@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--requests", type=int, default=300)
     p.add_argument("--cpu-sample", type=int, default=10000,
                    help="training molecules in the CPU-baseline sample (0 = skip)")
+    p.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    p.add_argument("--tol", type=float, default=1e-10)
     return p.parse_args()
 
 
@@ -104,27 +106,49 @@ def main():
     yd = torch.from_numpy(tr.labels).cuda()
     log(f"workload: N={N} size={size} d={d} sigma={args.sigma} lambda={args.lam}")
 
-    model = KRR("gaussian", args.sigma, args.lam)
+    def run_fits(model, tag):
+        reps = []
 
-    def step():
-        Xd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size)
-        model.fit_device(Xd, d, yd)
+        def step():
+            Xd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size)
+            reps.append(model.fit_device(Xd, d, yd))
 
-    for i in range(args.warmup):
-        step()
-        log(f"warmup {i} done")
-    torch.cuda.synchronize()
-    _lib.timer_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        log(f"step {i} done")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    timers = {k: _lib.timer_read(k) for k in _lib.TIMER_CLASSES}
-    _lib.timer_enable(False)
-    fit_s = (t1 - t0) / args.steps
+        for i in range(args.warmup):
+            step()
+            log(f"{tag} warmup {i} done")
+        torch.cuda.synchronize()
+        _lib.timer_enable(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+            log(f"{tag} step {i} done ({reps[-1].iterations} refinement iterations)")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timers = {k: _lib.timer_read(k) for k in _lib.TIMER_CLASSES}
+        _lib.timer_enable(False)
+        return (t1 - t0) / args.steps, timers, reps[-1]
+
+    model64 = KRR("gaussian", args.sigma, args.lam, "fp64")
+    fit64_s, timers, _ = run_fits(model64, "fp64")
+    # Predictions of the FP64 fit on a fixed query set, the reference for the mixed fit's error.
+    Zq, Rq, offq = te.subset(0, 2048).ragged()
+    pred64 = model64.predict_molecules(Zq, Rq, offq, size)
+    alpha64 = model64.alpha.clone()
+    modelmx = None
+    fitmx_s, timers_mx, rep_mx = None, None, None
+    if args.precision in ("mixed", "both"):
+        modelmx = KRR("gaussian", args.sigma, args.lam, "mixed", tol=args.tol)
+        fitmx_s, timers_mx, rep_mx = run_fits(modelmx, "mixed")
+        predmx = modelmx.predict_molecules(Zq, Rq, offq, size)
+        std = float(np.std(tr.labels))
+        mixed_err = {
+            "pred_max_abs_over_std": float(np.max(np.abs(predmx - pred64)) / std),
+            "alpha_max_rel": float((modelmx.alpha - alpha64).abs().max() / alpha64.abs().max()),
+            "refinement_iterations": rep_mx.iterations, "relres": rep_mx.relres,
+            "fp32_shift": rep_mx.shift, "reference": "FP64 GPU fit (same data)"}
+    fit_s = fitmx_s if fitmx_s is not None else fit64_s
+    model = modelmx if modelmx is not None else model64
 
     # Dominant kernel: the FP64 MFMA trailing update of the Cholesky.
     syrk_ms, syrk_n, syrk_flops = timers["syrk"]
@@ -169,16 +193,21 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32 (mixed)" if fitmx_s is not None else "f64",
         "data": "synthetic (seeded molecules, 5-29 atoms, H/C/N/O/S)",
         "config": {"workload": "krr_serve_50k: KRR fit N=50000, Coulomb size 29 (d=435), "
                                "Gaussian sigma=4000, lambda=1e-6; predict requests 1-512 molecules",
                    "n_train": N, "d": d, "sigma": args.sigma, "lambda": args.lam,
-                   "precision": "fp64"},
+                   "precision": "mixed" if fitmx_s is not None else "fp64"},
+        "fit_s": {"mixed": fitmx_s, "fp64": fit64_s,
+                  "speedup_mixed_vs_fp64": (fit64_s / fitmx_s) if fitmx_s else None},
+        "mixed_error": mixed_err if fitmx_s is not None else None,
         "predict": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                     "requests": int(len(lat)), "molecules_per_s": float(molecules / (lat.sum() / 1e3)),
                     "mode": "sequential requests, host molecules in, predictions out"},
-        "phases": phases,
+        "phases_fp64": phases,
+        "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
+                          for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),
         "roofline": {"kernel": "gemm_update_kernel<lower> (Cholesky trailing update, FP64 MFMA)",
                      "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS,

@@ -111,13 +111,49 @@ int qmlh_gaussian_mfma(const double *A, int64_t na, int64_t lda, const double *n
  * triangle, the strictly upper part untouched. info (device int32): 0, or j+1 when the
  * leading minor of order j+1 is not positive definite (LAPACK's convention); the remaining
  * columns are then left unfactored, as dpotrf leaves them.
- * qmlh_potrs replaces dpotrs("U", n, 1) at qml/fcho_solve.f90:46: x := (L L^T)^-1 x.
+ * qmlh_potrs replaces dpotrs("U", n, 1) at qml/fcho_solve.f90:46: x := (L L^T)^-1 x; ws holds
+ * qmlh_potrs_workspace(n) bytes of scratch.
  * ---------------------------------------------------------------------------------------- */
 int qmlh_potrf(double *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
-int qmlh_potrs(const double *L, int64_t n, int64_t lda, double *x, qmlh_stream_t stream);
+size_t qmlh_potrs_workspace(int64_t n);
+int qmlh_potrs(const double *L, int64_t n, int64_t lda, double *x, void *ws, size_t ws_bytes,
+               qmlh_stream_t stream);
 
 /* A(i,i) += value for i < n (the K[diag] += lambda step of tests/test_energy_krr_cmat.py:104). */
 int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Mixed precision (no single reference counterpart: together these replace the FP64
+ * dpotrf/dpotrs pair of qml/fcho_solve.f90:35,46 in the KRR fit at FP64 accuracy).
+ *
+ * qmlh_spotrf: the Cholesky of qmlh_potrf on FP32 storage; trailing updates on FP32 MFMA,
+ * diagonal blocks and substitutions in FP64 arithmetic. Same info convention.
+ *
+ * qmlh_gaussian_deflated: from one FP64 MFMA pass over X (n, ldx), with norms from
+ * qmlh_row_sqnorms and s = -0.5/sigma^2, writes
+ *   u_i = exp(s * norms_i)                                       (n FP64)
+ *   K(i, j) = exp(s * max(n_i + n_j - 2 x_i.x_j, 0)) + lambda [i==j]   (full, FP64; K may be NULL)
+ *   B32(i, j) = u_i u_j expm1(x_i.x_j / sigma^2) + (lambda + shift) [i==j]  (lower, FP32)
+ * so that K + lambda I = u u^T + B32 - shift I: a rank-one term plus the small-entry
+ * remainder that the FP32 factorisation resolves.
+ *
+ * qmlh_pcg_solve: FP64 preconditioned CG on A x = y (A symmetric, column-major, full), with
+ * preconditioner (L L^T + u u^T)^-1 (u may be NULL), L the qmlh_spotrf factor. Iterates until
+ * |y - A x| <= tol |y| or max_iter; reports iterations and the final relative residual.
+ * Synchronises the stream once per iteration (reads the residual norm).
+ * ---------------------------------------------------------------------------------------- */
+int qmlh_spotrf(float *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
+int qmlh_gaussian_deflated(const double *X, int64_t n, int64_t ldx, const double *norms,
+                           int64_t d, double sigma, double lambda, double shift, double *K,
+                           int64_t ldk, float *B32, int64_t ldb, double *u, qmlh_stream_t stream);
+/* B(i, j) = (float)(A(i, j) + shift [i == j]) on the lower triangle (column-major). */
+int qmlh_lower_to_f32(const double *A, int64_t n, int64_t lda, float *B, int64_t ldb,
+                      double shift, qmlh_stream_t stream);
+size_t qmlh_pcg_workspace(int64_t n);
+int qmlh_pcg_solve(const double *A, int64_t n, int64_t lda, const float *L, int64_t ldl,
+                   const double *u, const double *y, double *x, double tol, int32_t max_iter,
+                   void *ws, size_t ws_bytes, int32_t *iters_h, double *relres_h,
+                   qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Fused KRR prediction y_q = sum_j k(Xq_q, Xt_j) alpha_j without materialising the

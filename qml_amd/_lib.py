@@ -12,7 +12,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqmlhip.so")
+# QMLH_LIBRARY may name another build of the same C ABI (e.g. a tuning build).
+LIB_PATH = os.environ.get("QMLH_LIBRARY", os.path.join(_HERE, "libqmlhip.so"))
 
 QMLH_OK = 0
 PAIR_LAPLACIAN, PAIR_GAUSSIAN, PAIR_MANHATTAN, PAIR_L2 = 0, 1, 2, 3
@@ -46,7 +47,15 @@ _SIGNATURES = {
     "qmlh_gaussian_mfma": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64,
                                           _f64, _vp, _i64, _i32, _vp]),
     "qmlh_potrf": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
-    "qmlh_potrs": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
+    "qmlh_potrs_workspace": (_sz, [_i64]),
+    "qmlh_potrs": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "qmlh_spotrf": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
+    "qmlh_gaussian_deflated": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _vp,
+                                              _i64, _vp, _i64, _vp, _vp]),
+    "qmlh_lower_to_f32": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _vp]),
+    "qmlh_pcg_workspace": (_sz, [_i64]),
+    "qmlh_pcg_solve": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _f64, _i32,
+                                      _vp, _sz, ctypes.POINTER(_i32), ctypes.POINTER(_f64), _vp]),
     "qmlh_add_diagonal": (ctypes.c_int, [_vp, _i64, _i64, _f64, _vp]),
     "qmlh_predict_workspace": (_sz, [_i64, _i64]),
     "qmlh_krr_predict": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp,

@@ -6,7 +6,7 @@
 #include <stdio.h>
 #include <stdarg.h>
 
-#include "../../include/qmlhip.h"
+#include "qmlhip.h"
 
 namespace qmlh {
 

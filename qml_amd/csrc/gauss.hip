@@ -3,17 +3,26 @@
 // the same quantity qml/fkernels.f90:340-372 computes with direct differences. The cross term
 // is the one dense contraction of the path; norms and the exp argument stay FP64.
 //
-// A 256-lane workgroup owns a 128 x 128 output tile (2 x 2 wavefronts of 64 x 64). The MFMA is
-// fed (B rows, A rows) so that the 16 lanes holding one accumulator element run down 16
-// consecutive output rows: every store is a 128-byte run of the column-major K. In symmetric
-// mode only tiles with bi <= bj run; a diagonal tile writes i <= j and mirrors, an
-// off-diagonal tile writes both (i, j) and (j, i), so K is exactly symmetric with unit diagonal.
-#include "mfma_f64.h"
+// A 256-lane workgroup owns a 128 x 128 output tile (2 x 2 wavefronts of 64 x 64) with both
+// operands staged through LDS. The MFMA is fed (B rows, A rows) so that the 16 lanes holding
+// one accumulator element run down 16 consecutive output rows: every store is a 128-byte run
+// of the column-major K. In symmetric mode only tiles with bi <= bj run; a diagonal tile
+// writes i <= j and mirrors, an off-diagonal tile writes both (i, j) and (j, i), so K is
+// exactly symmetric with unit diagonal.
+//
+// The deflated build (mixed-precision KRR) writes, from the same cross term c = x_i.x_j:
+//   K64(i, j) = exp(scale * max(n_i + n_j - 2c, 0)) + lambda [i == j]     (full, FP64)
+//   B32(i, j) = u_i u_j expm1(c / sigma^2) + (lambda + shift) [i == j]   (lower, FP32)
+// with u_i = exp(scale * n_i), so that K = u u^T + u_i u_j expm1(c / sigma^2): the Gaussian
+// kernel is a rank-one term plus a positive semidefinite remainder (expm1 of a Gram matrix is
+// a sum of Hadamard powers), and B32 is that remainder -- O(|x|^2 / sigma^2) instead of O(1)
+// entries -- which an FP32 Cholesky resolves far better than K itself.
+#include "mfma_core.h"
 #include "timer.h"
 
 namespace qmlh {
 
-static constexpr int GK_TILE = 128;
+static constexpr int GK_TILE = WG_T;
 
 __global__ __launch_bounds__(256) void row_sqnorms_kernel(const double* __restrict__ X,
                                                           int64_t n, int64_t ld, int64_t d,
@@ -24,6 +33,12 @@ __global__ __launch_bounds__(256) void row_sqnorms_kernel(const double* __restri
     double s = 0.0;
     for (int64_t k = 0; k < d; ++k) s = fma(x[k], x[k], s);
     out[i] = s;
+}
+
+__global__ void exp_scale_kernel(const double* __restrict__ norms, int64_t n, double scale,
+                                 double* __restrict__ u) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = exp(scale * norms[i]);
 }
 
 __device__ __forceinline__ void tile_coords(int64_t bid, int symmetric, int64_t tiles_m,
@@ -40,26 +55,26 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int symmetric, int64_t 
     }
 }
 
-__global__ __launch_bounds__(256) void gaussian_mfma_kernel(
+__global__ __launch_bounds__(256, 2) void gaussian_mfma_kernel(
     const double* __restrict__ A, int64_t lda, int64_t na, const double* __restrict__ nA,
     const double* __restrict__ B, int64_t ldb, int64_t nb, const double* __restrict__ nB,
     int64_t d, double scale, double* __restrict__ K, int64_t ldk, int symmetric,
     int64_t tiles_m) {
+    __shared__ __attribute__((aligned(16))) double lds[WG_LDS_ELEMS];
     int64_t bi, bj;
     tile_coords(blockIdx.x, symmetric, tiles_m, bi, bj);
-    const int w = threadIdx.x >> 6;
-    const int64_t row0 = bi * GK_TILE + 64 * (w & 1);  // output rows (A)
-    const int64_t col0 = bj * GK_TILE + 64 * (w >> 1);  // output columns (B)
-    if (row0 >= na || col0 >= nb) return;
-
-    Operand<LAYOUT_ROW> opA{A, lda, na, d};
-    Operand<LAYOUT_ROW> opB{B, ldb, nb, d};
-    WaveTile t;
+    Operand<double, LAYOUT_ROW> opA{A, lda, na, d};
+    Operand<double, LAYOUT_ROW> opB{B, ldb, nb, d};
+    WaveTile<double> t;
     wave_tile_zero(t);
-    // First operand indexes output columns, second indexes output rows (see header note).
-    wave_tile_mma(t, opB, opA, col0, row0, d);
+    // First operand indexes output columns (B), second output rows (A).
+    wg_tile_mma_lds(t, lds, opB, opA, bj * GK_TILE, bi * GK_TILE, d);
 
+    const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
+    const int64_t col0 = bj * GK_TILE + 64 * (w & 1);
+    const int64_t row0 = bi * GK_TILE + 64 * (w >> 1);
+    if (row0 >= na || col0 >= nb) return;
     const bool diag_tile = symmetric && bi == bj;
     const bool mirror = symmetric && bi != bj;
 #pragma unroll
@@ -69,26 +84,68 @@ __global__ __launch_bounds__(256) void gaussian_mfma_kernel(
             const int64_t i = row0 + 16 * b + (lane & 15);
             if (i >= na) continue;
             const double ni = nA[i];
-            double v[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + tile_row(a, e);
-                double r = 0.0;
-                if (j < nb) {
-                    double d2 = ni + nB[j] - 2.0 * t.acc[a][b][e];
-                    d2 = d2 > 0.0 ? d2 : 0.0;
-                    if (symmetric && i == j) d2 = 0.0;
-                    r = exp(scale * d2);
-                }
-                v[e] = r;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + tile_row(a, e);
+                const int64_t j = col0 + tile_row<double>(a, e);
                 if (j >= nb) continue;
                 if (diag_tile && i > j) continue;
-                K[i + j * ldk] = v[e];
-                if (mirror || (diag_tile && i < j)) K[j + i * ldk] = v[e];
+                double d2 = ni + nB[j] - 2.0 * t.acc[a][b][e];
+                d2 = d2 > 0.0 ? d2 : 0.0;
+                if (symmetric && i == j) d2 = 0.0;
+                const double v = exp(scale * d2);
+                K[i + j * ldk] = v;
+                if (mirror || (diag_tile && i < j)) K[j + i * ldk] = v;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void gaussian_deflated_kernel(
+    const double* __restrict__ X, int64_t ldx, int64_t n, const double* __restrict__ nrm,
+    const double* __restrict__ u, int64_t d, double scale, double lambda, double shift,
+    double* __restrict__ K, int64_t ldk, float* __restrict__ B32, int64_t ldb) {
+    __shared__ __attribute__((aligned(16))) double lds[WG_LDS_ELEMS];
+    int64_t bi, bj;
+    tile_coords(blockIdx.x, 1, 0, bi, bj);  // bi <= bj
+    Operand<double, LAYOUT_ROW> op{X, ldx, n, d};
+    WaveTile<double> t;
+    wave_tile_zero(t);
+    wg_tile_mma_lds(t, lds, op, op, bj * GK_TILE, bi * GK_TILE, d);
+
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t col0 = bj * GK_TILE + 64 * (w & 1);
+    const int64_t row0 = bi * GK_TILE + 64 * (w >> 1);
+    if (row0 >= n || col0 >= n) return;
+    const double inv_s2 = -2.0 * scale;  // 1 / sigma^2
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + (lane & 15);  // i <= j: row index of the upper half
+            if (i >= n) continue;
+            const double ni = nrm[i], ui = u[i];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + tile_row<double>(a, e);
+                if (j >= n || i > j) continue;
+                const double c = t.acc[a][b][e];
+                double kv, bv;
+                if (i == j) {
+                    kv = 1.0 + lambda;
+                    bv = ui * ui * expm1(c * inv_s2) + (lambda + shift);
+                } else {
+                    double d2 = ni + nrm[j] - 2.0 * c;
+                    d2 = d2 > 0.0 ? d2 : 0.0;
+                    kv = exp(scale * d2);
+                    bv = ui * u[j] * expm1(c * inv_s2);
+                }
+                if (K) {
+                    K[i + j * ldk] = kv;
+                    K[j + i * ldk] = kv;
+                }
+                // B32 keeps the lower triangle (row >= column): element (j, i).
+                B32[j + i * ldb] = (float)bv;
             }
         }
     }
@@ -135,5 +192,33 @@ extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, cons
                        (hipStream_t)stream, A, lda, na, normsA, B, ldb, nb, normsB, d,
                        -0.5 / (sigma * sigma), K, ldk, sym ? 1 : 0, tm);
     QMLH_CHECK_LAUNCH("gaussian_mfma launch");
+    return QMLH_OK;
+}
+
+extern "C" int qmlh_gaussian_deflated(const double* X, int64_t n, int64_t ldx,
+                                      const double* norms, int64_t d, double sigma,
+                                      double lambda, double shift, double* K, int64_t ldk,
+                                      float* B32, int64_t ldb, double* u,
+                                      qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || d < 0 || ldx < d || ldx % 4)
+        return set_error(QMLH_ERR_ARG, "gaussian_deflated: bad shape (ldx multiple of 4, >= d)");
+    if ((K && ldk < n) || ldb < n) return set_error(QMLH_ERR_ARG, "gaussian_deflated: ld < n");
+    if (reinterpret_cast<uintptr_t>(X) & 15)
+        return set_error(QMLH_ERR_ARG, "gaussian_deflated: X must be 16-byte aligned");
+    if (n == 0) return QMLH_OK;
+    if (!X || !norms || !B32 || !u) return set_error(QMLH_ERR_ARG, "gaussian_deflated: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const double scale = -0.5 / (sigma * sigma);
+    hipLaunchKernelGGL(exp_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, norms,
+                       n, scale, u);
+    QMLH_CHECK_LAUNCH("gaussian_deflated scale launch");
+    int64_t tm = ceil_div(n, GK_TILE);
+    int64_t ntiles = tm * (tm + 1) / 2;
+    if (ntiles > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "gaussian_deflated: grid too large");
+    TimerScope ts(QMLH_TIMER_GAUSS, 2.0 * (double)d * (double)n * (double)(n + 1) / 2.0, s);
+    hipLaunchKernelGGL(gaussian_deflated_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, X, ldx,
+                       n, norms, u, d, scale, lambda, shift, K, ldk, B32, ldb);
+    QMLH_CHECK_LAUNCH("gaussian_deflated launch");
     return QMLH_OK;
 }

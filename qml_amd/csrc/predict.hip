@@ -6,7 +6,7 @@
 // that depends only on the training index, writes one partial per (chunk, query), and a second
 // kernel adds the partials of a query in chunk order. Nothing in a query's arithmetic depends
 // on the other queries or on m, so a query's result is bitwise the same alone or in a batch.
-#include "mfma_f64.h"
+#include "mfma_core.h"
 #include "timer.h"
 
 namespace qmlh {
@@ -32,12 +32,12 @@ __global__ __launch_bounds__(256) void predict_gauss_kernel(
 
     double qsum[4] = {0.0, 0.0, 0.0, 0.0};  // per query block b, this lane's query (lane & 15)
     if (t0 < n) {
-        Operand<LAYOUT_ROW> opT{T, ldt, n, d};
-        Operand<LAYOUT_ROW> opQ{Q, ldq, m, d};
-        WaveTile t;
+        Operand<double, LAYOUT_ROW> opT{T, ldt, n, d};
+        Operand<double, LAYOUT_ROW> opQ{Q, ldq, m, d};
+        WaveTile<double> t;
         wave_tile_zero(t);
         // First operand: training rows (accumulator rows), second: queries (columns).
-        wave_tile_mma(t, opT, opQ, t0, q0, d);
+        wave_tile_mma_rows(t, opT, opQ, t0, q0, d);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int64_t q = q0 + 16 * b + (lane & 15);
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void predict_gauss_kernel(
             for (int a = 0; a < 4; ++a) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int64_t j = t0 + tile_row(a, e);
+                    const int64_t j = t0 + tile_row<double>(a, e);
                     if (j < n) {
                         double d2 = nq + nT[j] - 2.0 * t.acc[a][b][e];
                         d2 = d2 > 0.0 ? d2 : 0.0;

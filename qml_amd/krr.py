@@ -32,20 +32,37 @@ class FitReport:
     n: int
     info: int
     seconds: dict = field(default_factory=dict)
+    precision: str = "fp64"
+    iterations: int = 0
+    relres: float = 0.0
+    shift: float = 0.0
 
 
 class KRR:
     """Kernel ridge regression with the training set resident in HBM.
 
     kernel: "gaussian" (FP64 MFMA expanded form) or "laplacian" (FP64 VALU).
+    precision:
+      "fp64"  -- K in FP64, FP64 MFMA Cholesky, two triangular solves (the drop-in path);
+      "mixed" -- K in FP64 for the residuals, an FP32 Cholesky (FP32 MFMA trailing updates,
+                 FP64 diagonal blocks) as preconditioner, FP64 conjugate-gradient refinement to
+                 |y - K alpha| <= tol |y|. For the Gaussian kernel the factor is of the
+                 deflated matrix B = K - u u^T (u_i = exp(-|x_i|^2 / 2 sigma^2)), with the
+                 rank-one term restored by Sherman-Morrison inside the preconditioner.
     """
 
-    def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9):
+    def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9,
+                 precision: str = "fp64", tol: float = 1e-10, max_iter: int = 200):
         if kernel not in KERNELS:
             raise ValueError(f"kernel must be one of {sorted(KERNELS)}")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.kernel = kernel
         self.sigma = float(sigma)
         self.lam = float(lam)
+        self.precision = precision
+        self.tol = float(tol)
+        self.max_iter = int(max_iter)
         self.Xd = None
         self.norms = None
         self.alpha = None
@@ -65,6 +82,8 @@ class KRR:
     # -- fit ----------------------------------------------------------------------------------
     def fit_device(self, Xd, d: int, y, raise_on_error: bool = True, stream=None) -> FitReport:
         """Fit from an (n, ld) device representation tensor; y host or device (n,)."""
+        if self.precision == "mixed":
+            return self._fit_mixed(Xd, d, y, raise_on_error, stream)
         import torch
 
         n = Xd.shape[0]
@@ -84,6 +103,66 @@ class KRR:
             if raise_on_error:
                 raise CholeskyError(f"Cholesky failed: info={rep.info}")
         self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d
+        return rep
+
+    def _fit_mixed(self, Xd, d: int, y, raise_on_error: bool, stream) -> FitReport:
+        import ctypes
+
+        import torch
+
+        n = Xd.shape[0]
+        dev = Xd.device
+        L = _lib.lib()
+        h = _lib.stream_handle(stream)
+        rep = FitReport(n=n, info=0, precision="mixed")
+        t0 = time.perf_counter()
+        K = torch.empty((n, n), dtype=torch.float64, device=dev)
+        B = torch.empty((n, n), dtype=torch.float32, device=dev)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        norms = u = None
+        if self.kernel == "gaussian":
+            norms = row_sqnorms_device(Xd, n, d, stream)
+            u = torch.empty(n, dtype=torch.float64, device=dev)
+        else:
+            self.build_kernel(Xd, n, d, out=K, stream=stream)
+            add_diagonal_device(K, n, self.lam, stream)
+        # FP32 factor; on breakdown retry with a diagonal shift (the CG on the FP64 matrix
+        # still converges to the unshifted solution, only more slowly).
+        shifts = [0.0] + [max(self.lam, 1e-12) * 10.0 ** k for k in range(1, 9)]
+        for attempt, shift in enumerate(shifts):
+            if self.kernel == "gaussian":
+                _lib.check(L.qmlh_gaussian_deflated(
+                    _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(norms), d, self.sigma, self.lam, shift,
+                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), n, _lib.ptr(u), h))
+            else:
+                _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), n, shift, h))
+            _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
+            rep.info = int(info.item())
+            rep.shift = shift
+            if rep.info == 0:
+                break
+        if rep.info != 0:
+            if raise_on_error:
+                raise CholeskyError(f"FP32 factorisation failed at every shift: info={rep.info}")
+            return rep
+        alpha = torch.zeros(n, dtype=torch.float64, device=dev)
+        yd = torch.as_tensor(y, dtype=torch.float64).to(dev).contiguous()
+        ws_bytes = L.qmlh_pcg_workspace(n)
+        ws = torch.empty(max(1, ws_bytes // 8), dtype=torch.float64, device=dev)
+        iters = ctypes.c_int32()
+        relres = ctypes.c_double()
+        _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), n, _lib.ptr(u), _lib.ptr(yd),
+                                    _lib.ptr(alpha), self.tol, self.max_iter, _lib.ptr(ws),
+                                    ws.numel() * 8, ctypes.byref(iters), ctypes.byref(relres), h))
+        rep.iterations, rep.relres = iters.value, relres.value
+        rep.seconds["fit"] = time.perf_counter() - t0
+        del K, B, ws
+        if norms is None and self.kernel == "gaussian":
+            norms = row_sqnorms_device(Xd, n, d, stream)
+        self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d
+        if not (rep.relres <= self.tol) and raise_on_error:
+            raise CholeskyError(f"refinement stopped at relative residual {rep.relres:.3e} "
+                                f"after {rep.iterations} iterations")
         return rep
 
     def fit(self, X, y, **kw) -> FitReport:

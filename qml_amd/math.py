@@ -24,9 +24,14 @@ def potrf_device(Ad, n: int, stream=None):
     return info
 
 
-def potrs_device(Ld, n: int, xd, stream=None):
+def potrs_device(Ld, n: int, xd, stream=None, workspace=None):
     """x := (L L^T)^-1 x with L from potrf_device; x is an (n,) FP64 device tensor."""
-    _lib.check(_lib.lib().qmlh_potrs(_lib.ptr(Ld), n, n, _lib.ptr(xd), _lib.stream_handle(stream)))
+    import torch
+
+    if workspace is None:
+        workspace = torch.empty(max(1, n), dtype=torch.float64, device=xd.device)
+    _lib.check(_lib.lib().qmlh_potrs(_lib.ptr(Ld), n, n, _lib.ptr(xd), _lib.ptr(workspace),
+                                     workspace.numel() * 8, _lib.stream_handle(stream)))
     return xd
 
 

@@ -1,0 +1,143 @@
+"""GPU tests of the mixed-precision fit: FP32 factor, deflated Gaussian build, CG refinement.
+
+Bars: the FP32 factor matches LAPACK's (on the FP32-rounded matrix) normwise to FP32 accuracy;
+the deflated build reproduces the FP64 Gaussian kernel bit for bit and B32 = K - u u^T to FP32
+rounding; the refined solution's predictions agree with the FP64 path (and with the oracle's
+LAPACK path) within 1e-8 of std(y) where the problem's FP64 noise floor allows.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from qml_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+    from qml_amd import _lib
+
+    _lib.lib()
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 257, 777, 1500])
+def test_spotrf_fp32_factor_vs_lapack(n):
+    from scipy.linalg import lapack
+
+    from qml_amd import _lib
+
+    rng = np.random.default_rng(n + 7)
+    M = rng.standard_normal((n, n))
+    A = (M @ M.T / n + np.eye(n)).astype(np.float32)
+    Ad = torch.from_numpy(A.copy()).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().qmlh_spotrf(Ad.data_ptr(), n, n, info.data_ptr(), None))
+    assert int(info.item()) == 0
+    L = np.tril(Ad.cpu().numpy().T.astype(np.float64))
+    Lref, inf = lapack.dpotrf(A.astype(np.float64), lower=1)
+    assert inf == 0
+    assert np.linalg.norm(L - np.tril(Lref)) <= 2e-6 * np.linalg.norm(Lref)
+    # the column-major strictly upper part (row-major strictly lower) is untouched
+    assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
+
+
+def test_spotrf_info_matches_lapack():
+    from scipy.linalg import lapack
+
+    from qml_amd import _lib
+
+    n = 300
+    A = np.eye(n, dtype=np.float32)
+    A[130, 130] = -1.0
+    Ad = torch.from_numpy(A.copy()).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().qmlh_spotrf(Ad.data_ptr(), n, n, info.data_ptr(), None))
+    assert int(info.item()) == lapack.dpotrf(A.astype(np.float64), lower=1)[1] == 131
+
+
+def test_gaussian_deflated_build_identities():
+    from qml_amd import _lib
+    from qml_amd.device import rows_to_device
+    from qml_amd.kernels import gaussian_mfma_device, row_sqnorms_device
+
+    tr, _, size = synthetic.workload("krr_serve_50k", n_train=777, n_test=0)
+    X = oracle.coulomb_matrix_batch(*tr.ragged(), size)
+    n, d = X.shape
+    Xd = rows_to_device(X)
+    nrm = row_sqnorms_device(Xd, n, d)
+    K = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    B = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    u = torch.empty(n, dtype=torch.float64, device="cuda")
+    lam, shift, sigma = 1e-6, 3e-5, 4000.0
+    _lib.check(_lib.lib().qmlh_gaussian_deflated(Xd.data_ptr(), n, Xd.shape[1], nrm.data_ptr(),
+                                                 d, sigma, lam, shift, K.data_ptr(), n,
+                                                 B.data_ptr(), n, u.data_ptr(), None))
+    Kref = gaussian_mfma_device(Xd, nrm, n, Xd, nrm, n, d, sigma, True).cpu().numpy().T
+    Kg = K.cpu().numpy().T
+    off = ~np.eye(n, dtype=bool)
+    assert np.array_equal(Kg[off], Kref[off])
+    assert np.all(np.diag(Kg) == 1.0 + lam)
+    Kor = oracle.gaussian_kernel(X, X, sigma)
+    assert np.max(np.abs(Kg - lam * np.eye(n) - Kor) / Kor) <= 1e-12
+    ug = u.cpu().numpy()
+    np.testing.assert_allclose(ug, np.exp(-(X * X).sum(1) / (2 * sigma ** 2)), rtol=1e-14)
+    Bg = np.tril(B.cpu().numpy().T.astype(np.float64))
+    Bexp = np.tril(Kor - np.outer(ug, ug) + (lam + shift) * np.eye(n))
+    # B's entries are O(1e-2); FP32 rounding plus the cancellation-free expm1 form.
+    assert np.max(np.abs(Bg - Bexp)) <= 1e-8
+    assert np.all(np.triu(B.cpu().numpy().T, 1) == 0)
+
+
+def test_pcg_solve_random_spd():
+    from qml_amd import _lib
+
+    n = 900
+    rng = np.random.default_rng(0)
+    M = rng.standard_normal((n, n))
+    A = M @ M.T / n + 0.1 * np.eye(n)
+    y = rng.standard_normal(n)
+    L = _lib.lib()
+    Ad = torch.from_numpy(A.copy()).cuda()
+    Bd = torch.from_numpy(A.astype(np.float32)).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.check(L.qmlh_spotrf(Bd.data_ptr(), n, n, info.data_ptr(), None))
+    assert int(info.item()) == 0
+    yd = torch.from_numpy(y).cuda()
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    ws = torch.empty(L.qmlh_pcg_workspace(n) // 8 + 1, dtype=torch.float64, device="cuda")
+    it, rr = ctypes.c_int32(), ctypes.c_double()
+    _lib.check(L.qmlh_pcg_solve(Ad.data_ptr(), n, n, Bd.data_ptr(), n, None, yd.data_ptr(),
+                                x.data_ptr(), 1e-13, 50, ws.data_ptr(), ws.numel() * 8,
+                                ctypes.byref(it), ctypes.byref(rr), None))
+    assert rr.value <= 1e-13 and it.value <= 10
+    xs = np.linalg.solve(A, y)
+    assert np.linalg.norm(x.cpu().numpy() - xs) <= 1e-11 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("kernel,sigma,lam,n", [("gaussian", 4000.0, 1e-6, 3000),
+                                                ("laplacian", 724.0, 1e-8, 2000)])
+def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n):
+    from qml_amd.krr import KRR
+
+    tr, te, size = synthetic.workload("krr_serve_50k", n_train=n, n_test=300)
+    X = oracle.coulomb_matrix_batch(*tr.ragged(), size)
+    Xs = oracle.coulomb_matrix_batch(*te.ragged(), size)
+    std = np.std(tr.labels)
+    m64 = KRR(kernel, sigma, lam, "fp64")
+    m64.fit(X, tr.labels)
+    p64 = m64.predict(Xs)
+    mx = KRR(kernel, sigma, lam, "mixed", tol=1e-11)
+    rep = mx.fit(X, tr.labels)
+    pmx = mx.predict(Xs)
+    _, pref = oracle.krr_fit_predict(X, tr.labels, Xs, kernel, sigma, lam)
+    assert rep.relres <= 1e-11 and rep.iterations < 60, rep
+    assert np.max(np.abs(pmx - p64)) <= 1e-8 * std
+    assert np.max(np.abs(pmx - pref)) <= 1e-8 * std
