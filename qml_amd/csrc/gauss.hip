@@ -11,12 +11,15 @@
 // exactly symmetric with unit diagonal.
 //
 // The deflated build (mixed-precision KRR) writes, from the same cross term c = x_i.x_j:
-//   K64(i, j) = exp(scale * max(n_i + n_j - 2c, 0)) + lambda [i == j]     (full, FP64)
-//   B32(i, j) = u_i u_j expm1(c / sigma^2) + (lambda + shift) [i == j]   (lower, FP32)
-// with u_i = exp(scale * n_i), so that K = u u^T + u_i u_j expm1(c / sigma^2): the Gaussian
-// kernel is a rank-one term plus a positive semidefinite remainder (expm1 of a Gram matrix is
-// a sum of Hadamard powers), and B32 is that remainder -- O(|x|^2 / sigma^2) instead of O(1)
-// entries -- which an FP32 Cholesky resolves far better than K itself.
+//   K64(i, j) = exp(scale * max(n_i + n_j - 2c, 0)) + lambda [i == j]              (full, FP64)
+//   B32(i, j) = u_i u_j (expm1(c / s2) - c / s2) + (lambda + shift) [i == j]     (lower, FP32)
+//   V(i, :)   = u_i [1, x_i / sigma]                                  (n x (d+1), FP32)
+// with s2 = sigma^2 and u_i = exp(scale * n_i). Since exp(c/s2) = 1 + c/s2 + (expm1(c/s2) -
+// c/s2), K = V V^T + B: a rank-(d+1) term (the constant and linear parts of the expansion)
+// plus a positive semidefinite remainder (the Hadamard powers >= 2 of the Gram matrix / s2),
+// whose entries are O((|x|^2 / s2)^2) -- about 1e-3 for Coulomb matrices at sigma = 4000,
+// against O(1) for K. A low-precision Cholesky of B + lambda I stays accurate relative to
+// lambda where one of K + lambda I does not, and V is kept exact in the preconditioner.
 #include "mfma_core.h"
 #include "timer.h"
 
@@ -39,6 +42,18 @@ __global__ void exp_scale_kernel(const double* __restrict__ norms, int64_t n, do
                                  double* __restrict__ u) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) u[i] = exp(scale * norms[i]);
+}
+
+// V(i, 0) = u_i, V(i, 1 + k) = u_i x_ik / sigma (column-major, FP32).
+__global__ void lowrank_kernel(const double* __restrict__ X, int64_t ldx, int64_t n, int64_t d,
+                               const double* __restrict__ u, double inv_sigma,
+                               float* __restrict__ V, int64_t ldv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ui = u[i];
+    V[i] = (float)ui;
+    const double* x = X + i * ldx;
+    for (int64_t k = 0; k < d; ++k) V[i + (k + 1) * ldv] = (float)(ui * x[k] * inv_sigma);
 }
 
 __device__ __forceinline__ void tile_coords(int64_t bid, int symmetric, int64_t tiles_m,
@@ -131,14 +146,15 @@ __global__ __launch_bounds__(256, 2) void gaussian_deflated_kernel(
                 if (j >= n || i > j) continue;
                 const double c = t.acc[a][b][e];
                 double kv, bv;
+                const double xv = c * inv_s2;
                 if (i == j) {
                     kv = 1.0 + lambda;
-                    bv = ui * ui * expm1(c * inv_s2) + (lambda + shift);
+                    bv = ui * ui * (expm1(xv) - xv) + (lambda + shift);
                 } else {
                     double d2 = ni + nrm[j] - 2.0 * c;
                     d2 = d2 > 0.0 ? d2 : 0.0;
                     kv = exp(scale * d2);
-                    bv = ui * u[j] * expm1(c * inv_s2);
+                    bv = ui * u[j] * (expm1(xv) - xv);
                 }
                 if (K) {
                     K[i + j * ldk] = kv;
@@ -198,12 +214,13 @@ extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, cons
 extern "C" int qmlh_gaussian_deflated(const double* X, int64_t n, int64_t ldx,
                                       const double* norms, int64_t d, double sigma,
                                       double lambda, double shift, double* K, int64_t ldk,
-                                      float* B32, int64_t ldb, double* u,
+                                      float* B32, int64_t ldb, float* V, int64_t ldv, double* u,
                                       qmlh_stream_t stream) {
     using namespace qmlh;
     if (n < 0 || d < 0 || ldx < d || ldx % 4)
         return set_error(QMLH_ERR_ARG, "gaussian_deflated: bad shape (ldx multiple of 4, >= d)");
-    if ((K && ldk < n) || ldb < n) return set_error(QMLH_ERR_ARG, "gaussian_deflated: ld < n");
+    if ((K && ldk < n) || ldb < n || (V && ldv < n))
+        return set_error(QMLH_ERR_ARG, "gaussian_deflated: ld < n");
     if (reinterpret_cast<uintptr_t>(X) & 15)
         return set_error(QMLH_ERR_ARG, "gaussian_deflated: X must be 16-byte aligned");
     if (n == 0) return QMLH_OK;
@@ -213,6 +230,11 @@ extern "C" int qmlh_gaussian_deflated(const double* X, int64_t n, int64_t ldx,
     hipLaunchKernelGGL(exp_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, norms,
                        n, scale, u);
     QMLH_CHECK_LAUNCH("gaussian_deflated scale launch");
+    if (V) {
+        hipLaunchKernelGGL(lowrank_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, X, ldx,
+                           n, d, u, 1.0 / sigma, V, ldv);
+        QMLH_CHECK_LAUNCH("gaussian_deflated low-rank launch");
+    }
     int64_t tm = ceil_div(n, GK_TILE);
     int64_t ntiles = tm * (tm + 1) / 2;
     if (ntiles > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "gaussian_deflated: grid too large");

@@ -1,0 +1,40 @@
+// Internal dense linear-algebra building blocks shared by the Cholesky, the refinement and
+// the batched solver (implemented in potrf.hip).
+#pragma once
+
+#include "common.h"
+
+namespace qmlh {
+
+// Lower Cholesky in place (column-major), storage T in {double, float}; LAPACK info rules.
+template <typename T>
+int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s);
+
+// Synchronisation words the single-launch triangular solve needs for order n.
+size_t trsv_sync_ints(int64_t n);
+
+// Single-launch triangular solve (forward z = L^-1 rhs, or backward x = L^-T rhs).
+template <typename T>
+int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
+                  int32_t* sync, bool forward, hipStream_t s);
+
+// x := (L L^T)^-1 x; tmp is n doubles of scratch, sync trsv_sync_ints(n) ints.
+template <typename T>
+int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32_t* sync,
+               hipStream_t s);
+
+// z := L^-1 r (r may be clobbered).
+template <typename T>
+int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
+                   hipStream_t s);
+
+// x := L^-T z (z may be clobbered).
+template <typename T>
+int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
+                   hipStream_t s);
+
+// V := L^-1 V for an (n, nrhs) column-major right-hand side (storage float), L lower float.
+int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
+                         int64_t nrhs, hipStream_t s);
+
+}  // namespace qmlh

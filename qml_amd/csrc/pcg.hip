@@ -1,21 +1,22 @@
 // Mixed-precision solve: FP64 preconditioned conjugate gradients on A x = y, preconditioned by
-// an FP32 Cholesky factor L (B = L L^T) and, when u is given, the rank-one correction
-// M = B + u u^T applied with Sherman-Morrison:
-//   M^-1 r = B^-1 r - (u . B^-1 r) / (1 + u . B^-1 u) B^-1 u.
+// M = L L^T + V V^T, with L a low-precision Cholesky factor (FP32 storage) and V (n x r) a
+// low-rank term kept out of the factorisation. With Y = L^-1 V and C = I + Y^T Y = R R^T
+// (Woodbury),
+//   M^-1 r = L^-T (q - Y C^-1 Y^T q),   q = L^-1 r,
+// i.e. two triangular solves with L plus two skinny products with Y per application.
 // Every residual r = y - A x is formed from the FP64 matrix, so the iteration converges to the
-// FP64 solution; the FP32 factor only sets how fast. (This is iterative refinement with the
-// correction equation solved by CG, i.e. CG-IR.)
+// FP64 solution; the preconditioner only sets how fast. (Iterative refinement with the
+// correction equation solved by CG: CG-IR.)
 //
 // Reductions are deterministic: a fixed number of workgroups each reduce a fixed contiguous
 // range in a fixed tree order, and one workgroup adds the partials in order. PCG scalars stay
 // on the device; the host reads one residual norm per iteration to decide convergence.
-#include "common.h"
+#include "linalg.h"
 #include "timer.h"
 
 namespace qmlh {
 
-template <typename T>
-int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, hipStream_t s);
+static constexpr int WB_MAX_RANK = 1024;
 
 static constexpr int RED_BLOCKS = 256;
 
@@ -75,14 +76,56 @@ __global__ void axpy2_kernel(double* __restrict__ x, double* __restrict__ r,
     r[i] = fma(-alpha, q[i], r[i]);
 }
 
-__global__ void sm_correct_kernel(double* __restrict__ z, const double* __restrict__ Bu,
-                                  const double* __restrict__ sc, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    z[i] = fma(-(sc[4] / sc[5]), Bu[i], z[i]);
+// t_a = sum_i Y(i, a) q_i: one workgroup per column a, fixed-order reduction.
+__global__ __launch_bounds__(256) void yt_q_kernel(const float* __restrict__ Y, int64_t ldy,
+                                                   int64_t n, const double* __restrict__ q,
+                                                   double* __restrict__ t) {
+    __shared__ double red[256];
+    const float* col = Y + (int64_t)blockIdx.x * ldy;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s = fma((double)col[i], q[i], s);
+    const double tot = block_reduce_256(s, red);
+    if (threadIdx.x == 0) t[blockIdx.x] = tot;
 }
 
-__global__ void den_kernel(double* sc) { sc[5] = 1.0 + sc[5]; }
+// q_i -= sum_a Y(i, a) t_a (rank r <= WB_MAX_RANK).
+__global__ __launch_bounds__(256) void q_minus_yt_kernel(const float* __restrict__ Y, int64_t ldy,
+                                                         int64_t n, int64_t r,
+                                                         const double* __restrict__ t,
+                                                         double* __restrict__ q) {
+    __shared__ double st[WB_MAX_RANK];
+    for (int64_t a = threadIdx.x; a < r; a += 256) st[a] = t[a];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t a = 0; a < r; ++a) s = fma((double)Y[i + a * ldy], st[a], s);
+    q[i] = q[i] - s;
+}
+
+// C = I + Y^T Y (r x r, FP64): a 16 x 16 block per workgroup, rows streamed through LDS.
+__global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ Y, int64_t ldy,
+                                                   int64_t n, int64_t r, double* __restrict__ C,
+                                                   int64_t ldc) {
+    __shared__ double sa[64][17], sb[64][17];
+    const int ta = threadIdx.x & 15, tb = threadIdx.x >> 4;
+    const int64_t a0 = (int64_t)blockIdx.x * 16, b0 = (int64_t)blockIdx.y * 16;
+    double acc = 0.0;
+    for (int64_t i0 = 0; i0 < n; i0 += 64) {
+        for (int idx = threadIdx.x; idx < 64 * 16; idx += 256) {
+            const int ii = idx % 64, cc = idx / 64;
+            const int64_t i = i0 + ii;
+            sa[ii][cc] = (i < n && a0 + cc < r) ? (double)Y[i + (a0 + cc) * ldy] : 0.0;
+            sb[ii][cc] = (i < n && b0 + cc < r) ? (double)Y[i + (b0 + cc) * ldy] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int ii = 0; ii < 64; ++ii) acc = fma(sa[ii][ta], sb[ii][tb], acc);
+        __syncthreads();
+    }
+    const int64_t a = a0 + ta, b = b0 + tb;
+    if (a < r && b < r) C[a + b * ldc] = acc + (a == b ? 1.0 : 0.0);
+}
 
 __global__ void p_update_kernel(double* __restrict__ p, const double* __restrict__ z,
                                 double* __restrict__ sc, int64_t n, int first) {
@@ -126,6 +169,7 @@ struct PcgCtx {
     hipStream_t s;
     int64_t n;
     double* partial;
+    int32_t* sync;  // triangular-solve synchronisation words
 };
 
 static inline unsigned grid1(int64_t n) { return (unsigned)ceil_div(n, 256); }
@@ -144,17 +188,31 @@ static int copy(const PcgCtx& c, const double* a, double* b) {
     return QMLH_OK;
 }
 
-// z = M^-1 r
-static int precond(const PcgCtx& c, const float* L, int64_t ldl, const double* u,
-                   const double* Bu, const double* r, double* z, double* tmp, double* sc) {
-    QMLH_TRY(copy(c, r, z));
-    QMLH_TRY(potrs_impl<float>(L, c.n, ldl, z, tmp, c.s));
-    if (u) {
-        QMLH_TRY(dot(c, u, z, sc + 4));
-        hipLaunchKernelGGL(sm_correct_kernel, dim3(grid1(c.n)), dim3(256), 0, c.s, z, Bu, sc, c.n);
-        QMLH_CHECK_LAUNCH("pcg correction launch");
+struct Woodbury {
+    const float* Y;
+    int64_t ldy;
+    int64_t r;
+    const double* C;  // Cholesky factor of I + Y^T Y
+    int64_t ldc;
+    double* t;        // r
+    double* tmp_r;    // r
+};
+
+// z = M^-1 r; tmp and q are n-vectors of scratch.
+static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury& wb,
+                   const double* r, double* z, double* q, double* tmp) {
+    QMLH_TRY(copy(c, r, tmp));
+    QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, c.sync, c.s));
+    if (wb.r > 0) {
+        hipLaunchKernelGGL(yt_q_kernel, dim3((unsigned)wb.r), dim3(256), 0, c.s, wb.Y, wb.ldy, c.n,
+                           q, wb.t);
+        QMLH_CHECK_LAUNCH("pcg Y^T q launch");
+        QMLH_TRY(potrs_impl<double>(wb.C, wb.r, wb.ldc, wb.t, wb.tmp_r, nullptr, c.s));
+        hipLaunchKernelGGL(q_minus_yt_kernel, dim3(grid1(c.n)), dim3(256), 0, c.s, wb.Y, wb.ldy,
+                           c.n, wb.r, wb.t, q);
+        QMLH_CHECK_LAUNCH("pcg q - Y t launch");
     }
-    return QMLH_OK;
+    return trsv_lower_bwd<float>(L, c.n, ldl, q, z, c.sync, c.s);
 }
 
 }  // namespace qmlh
@@ -184,34 +242,61 @@ extern "C" int qmlh_lower_to_f32(const double* A, int64_t n, int64_t lda, float*
     return QMLH_OK;
 }
 
-extern "C" size_t qmlh_pcg_workspace(int64_t n) {
+extern "C" int qmlh_woodbury_setup(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
+                                   int64_t r, double* C, int64_t ldc, int32_t* info,
+                                   qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || r < 0 || ldl < n || ldv < n || ldc < r)
+        return set_error(QMLH_ERR_ARG, "woodbury_setup: bad shape");
+    if (r > WB_MAX_RANK)
+        return set_error(QMLH_ERR_ARG, "woodbury_setup: rank %lld above %d", (long long)r,
+                         WB_MAX_RANK);
+    if (n == 0 || r == 0) return QMLH_OK;
+    if (!L || !V || !C || !info) return set_error(QMLH_ERR_ARG, "woodbury_setup: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    QMLH_TRY(trsm_lower_fwd_multi(L, n, ldl, V, ldv, r, s));
+    dim3 g((unsigned)ceil_div(r, 16), (unsigned)ceil_div(r, 16));
+    hipLaunchKernelGGL(gram_kernel, g, dim3(256), 0, s, V, ldv, n, r, C, ldc);
+    QMLH_CHECK_LAUNCH("woodbury gram launch");
+    return potrf_impl<double>(C, r, ldc, info, s);
+}
+
+extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
     if (n <= 0) return 0;
-    return (size_t)(6 * n + qmlh::RED_BLOCKS + 8) * sizeof(double);
+    if (r < 0) r = 0;
+    return (size_t)(5 * n + 2 * r + qmlh::RED_BLOCKS + 8) * sizeof(double) +
+           qmlh::trsv_sync_ints(n) * sizeof(int32_t);
 }
 
 extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const float* L,
-                              int64_t ldl, const double* u, const double* y, double* x,
+                              int64_t ldl, const float* Y, int64_t ldy, int64_t r,
+                              const double* C, int64_t ldc, const double* y, double* x,
                               double tol, int32_t max_iter, void* ws, size_t ws_bytes,
                               int32_t* iters_h, double* relres_h, qmlh_stream_t stream) {
     using namespace qmlh;
-    if (n < 0 || lda < n || ldl < n) return set_error(QMLH_ERR_ARG, "pcg_solve: bad shape");
+    if (n < 0 || lda < n || ldl < n || r < 0) return set_error(QMLH_ERR_ARG, "pcg_solve: bad shape");
+    if (r > 0 && (!Y || !C || ldy < n || ldc < r || r > WB_MAX_RANK))
+        return set_error(QMLH_ERR_ARG, "pcg_solve: bad low-rank term");
     if (iters_h) *iters_h = 0;
     if (relres_h) *relres_h = 0.0;
     if (n == 0) return QMLH_OK;
     if (!A || !L || !y || !x || !ws) return set_error(QMLH_ERR_ARG, "pcg_solve: null pointer");
-    if (ws_bytes < qmlh_pcg_workspace(n))
+    if (ws_bytes < qmlh_pcg_workspace(n, r))
         return set_error(QMLH_ERR_WORKSPACE, "pcg_solve: workspace too small");
     hipStream_t s = (hipStream_t)stream;
     double* w = static_cast<double*>(ws);
-    double* r = w;
-    double* z = r + n;
+    double* rv = w;
+    double* z = rv + n;
     double* p = z + n;
     double* q = p + n;
     double* tmp = q + n;
-    double* Bu = tmp + n;
-    double* partial = Bu + n;
-    double* sc = partial + RED_BLOCKS;  // 8 scalars
-    PcgCtx c{s, n, partial};
+    double* t = tmp + n;
+    double* tmp_r = t + r;
+    double* partial = tmp_r + r;
+    double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 6 rz_new
+    int32_t* sync = reinterpret_cast<int32_t*>(sc + 8);
+    PcgCtx c{s, n, partial, sync};
+    Woodbury wb{Y, ldy, r, C, ldc, t, tmp_r};
 
     double yy_h = 0.0;
     QMLH_TRY(dot(c, y, y, sc + 2));
@@ -222,16 +307,9 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     hipLaunchKernelGGL(fill_kernel, dim3(grid1(n)), dim3(256), 0, s, x, 0.0, n);
     QMLH_CHECK_LAUNCH("pcg fill launch");
     if (ynorm == 0.0) return QMLH_OK;
-    if (u) {  // Bu = B^-1 u, den = 1 + u.Bu
-        QMLH_TRY(copy(c, u, Bu));
-        QMLH_TRY(potrs_impl<float>(L, n, ldl, Bu, tmp, s));
-        QMLH_TRY(dot(c, u, Bu, sc + 5));
-        hipLaunchKernelGGL(den_kernel, dim3(1), dim3(1), 0, s, sc);
-        QMLH_CHECK_LAUNCH("pcg den launch");
-    }
-    QMLH_TRY(copy(c, y, r));
-    QMLH_TRY(precond(c, L, ldl, u, Bu, r, z, tmp, sc));
-    QMLH_TRY(dot(c, r, z, sc + 0));
+    QMLH_TRY(copy(c, y, rv));
+    QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
+    QMLH_TRY(dot(c, rv, z, sc + 0));
     hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 1);
     QMLH_CHECK_LAUNCH("pcg p launch");
     double relres = 1.0;
@@ -242,17 +320,17 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
                            lda, n, p, q);
         QMLH_CHECK_LAUNCH("pcg symv launch");
         QMLH_TRY(dot(c, p, q, sc + 1));
-        hipLaunchKernelGGL(axpy2_kernel, dim3(grid1(n)), dim3(256), 0, s, x, r, p, q, sc, n);
+        hipLaunchKernelGGL(axpy2_kernel, dim3(grid1(n)), dim3(256), 0, s, x, rv, p, q, sc, n);
         QMLH_CHECK_LAUNCH("pcg axpy launch");
-        QMLH_TRY(dot(c, r, r, sc + 2));
+        QMLH_TRY(dot(c, rv, rv, sc + 2));
         double rr_h = 0.0;
         QMLH_TRY(check_hip(hipMemcpyAsync(&rr_h, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s),
                            "pcg: read |r|"));
         QMLH_TRY(check_hip(hipStreamSynchronize(s), "pcg: sync"));
         relres = sqrt(rr_h) / ynorm;
         if (!(relres > tol)) break;  // converged (or NaN: stop and report)
-        QMLH_TRY(precond(c, L, ldl, u, Bu, r, z, tmp, sc));
-        QMLH_TRY(dot(c, r, z, sc + 6));
+        QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
+        QMLH_TRY(dot(c, rv, z, sc + 6));
         hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 0);
         QMLH_CHECK_LAUNCH("pcg p launch");
         hipLaunchKernelGGL(rz_roll_kernel, dim3(1), dim3(1), 0, s, sc);

@@ -15,6 +15,7 @@
 // A device int32 `info` follows LAPACK: the first non-positive (or NaN) pivot j sets
 // info = j + 1, its Schur-complement value is left on the diagonal, and every later kernel of
 // the factorisation returns without work.
+#include "linalg.h"
 #include "mfma_core.h"
 #include "timer.h"
 
@@ -25,10 +26,11 @@ static constexpr int PF_IB = 64;
 static constexpr int UP_TILE = WG_T;
 
 // ------------------------------------------------------------------------------------------
-// C(i, j) += alpha * sum_k A(i, k) B(j, k); A, B, C column-major. LOWER: only tiles with
-// bi >= bj run and only i >= j is written (a SYRK when A == B).
+// C(i, j) += alpha * sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL:
+// B(j, k) at B[j + k ldb]) or row-major (LB = ROW: B(j, k) at B[j ldb + k]). LOWER: only tiles
+// with bi >= bj run and only i >= j is written (a SYRK when A == B).
 // ------------------------------------------------------------------------------------------
-template <typename T, bool LOWER>
+template <typename T, bool LOWER, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
     int64_t N, int64_t Kd, T alpha, T* __restrict__ C, int64_t ldc,
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     // First MFMA operand: B rows (output columns); second: A rows (output rows), so the 16
     // lanes of one accumulator element run down 16 consecutive rows of column-major C.
     Operand<T, LAYOUT_COL> opA{A, lda, M, Kd};
-    Operand<T, LAYOUT_COL> opB{B, ldb, N, Kd};
+    Operand<T, LB> opB{B, ldb, N, Kd};
     WaveTile<T> t;
     wave_tile_zero(t);
     wg_tile_mma_lds(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     }
 }
 
-template <typename T>
+template <typename T, int LB = LAYOUT_COL>
 static int launch_update(bool lower, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
                          int64_t N, int64_t Kd, double alpha, T* C, int64_t ldc,
                          const int32_t* info, hipStream_t s) {
@@ -90,11 +92,11 @@ static int launch_update(bool lower, const T* A, int64_t lda, const T* B, int64_
                   s);
     if (lower) {
         int64_t nt = tm * (tm + 1) / 2;
-        hipLaunchKernelGGL((gemm_update_kernel<T, true>), dim3((unsigned)nt), dim3(256), 0, s, A,
-                           lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
+        hipLaunchKernelGGL((gemm_update_kernel<T, true, LB>), dim3((unsigned)nt), dim3(256), 0, s,
+                           A, lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
     } else {
-        hipLaunchKernelGGL((gemm_update_kernel<T, false>), dim3((unsigned)(tm * tn)), dim3(256),
-                           0, s, A, lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
+        hipLaunchKernelGGL((gemm_update_kernel<T, false, LB>), dim3((unsigned)(tm * tn)),
+                           dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
@@ -220,7 +222,7 @@ static int factor_diag(T* A, int64_t lda, int64_t kb, int64_t col_base, int32_t*
 __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 
 template <typename T>
-static int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s) {
+int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s) {
     if (n < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0");
     if (lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n");
     if (!info) return set_error(QMLH_ERR_ARG, "potrf: info is null");
@@ -350,45 +352,121 @@ __global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const T* __restrict_
     z[i] = z[i] - s;
 }
 
-// x := (L L^T)^-1 x with L lower, storage T; `tmp` is an n-vector of scratch.
+// Below this order the per-step kernels are used (few steps, no spin-waiting).
+static constexpr int64_t TRSV_SYNCFREE_MIN = 1024;
+
 template <typename T>
-int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, hipStream_t s) {
-    if (n < 0) return set_error(QMLH_ERR_ARG, "potrs: n < 0");
-    if (lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "potrs: lda < n");
-    if (n == 0) return QMLH_OK;
-    if (!L || !x || !tmp) return set_error(QMLH_ERR_ARG, "potrs: null pointer");
-    TimerScope ts(QMLH_TIMER_POTRS, 2.0 * (double)n * (double)n, s);
-    // Forward L z = y: x holds y and is consumed as scratch, z lands in tmp.
+int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
+                   hipStream_t s) {
+    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, r, z, sync, true, s);
     for (int64_t k = 0; k < n; k += PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
         const int64_t rest = n - k - w;
         const unsigned grid = (unsigned)(rest > 0 ? ceil_div(rest, 256) : 1);
         if (w == PF_IB)
             hipLaunchKernelGGL((trsv_fwd_step_kernel<T, true>), dim3(grid), dim3(256), 0, s, L,
-                               lda, n, k, w, x, tmp);
+                               lda, n, k, w, r, z);
         else
             hipLaunchKernelGGL((trsv_fwd_step_kernel<T, false>), dim3(grid), dim3(256), 0, s, L,
-                               lda, n, k, w, x, tmp);
+                               lda, n, k, w, r, z);
         QMLH_CHECK_LAUNCH("trsv forward launch");
     }
-    // Backward L^T x = z: tmp is consumed as scratch, the solution lands in x.
+    return QMLH_OK;
+}
+
+template <typename T>
+int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
+                   hipStream_t s) {
+    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, z, x, sync, false, s);
     const int64_t last = ((n - 1) / PF_IB) * PF_IB;
     for (int64_t k = last; k >= 0; k -= PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
         const unsigned grid = (unsigned)(k > 0 ? ceil_div(k, 256) : 1);
         if (w == PF_IB)
             hipLaunchKernelGGL((trsv_bwd_step_kernel<T, true>), dim3(grid), dim3(256), 0, s, L,
-                               lda, k, w, tmp, x);
+                               lda, k, w, z, x);
         else
             hipLaunchKernelGGL((trsv_bwd_step_kernel<T, false>), dim3(grid), dim3(256), 0, s, L,
-                               lda, k, w, tmp, x);
+                               lda, k, w, z, x);
         QMLH_CHECK_LAUNCH("trsv backward launch");
     }
     return QMLH_OK;
 }
 
-template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, hipStream_t);
-template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, hipStream_t);
+// x := (L L^T)^-1 x with L lower, storage T; `tmp` is an n-vector of scratch.
+template <typename T>
+int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32_t* sync,
+               hipStream_t s) {
+    if (n < 0) return set_error(QMLH_ERR_ARG, "potrs: n < 0");
+    if (lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "potrs: lda < n");
+    if (n == 0) return QMLH_OK;
+    if (!L || !x || !tmp) return set_error(QMLH_ERR_ARG, "potrs: null pointer");
+    TimerScope ts(QMLH_TIMER_POTRS, 2.0 * (double)n * (double)n, s);
+    // Forward L z = y: x holds y and is consumed as scratch, z lands in tmp; backward L^T x = z:
+    // tmp is consumed, the solution lands in x.
+    QMLH_TRY(trsv_lower_fwd<T>(L, n, lda, x, tmp, sync, s));
+    return trsv_lower_bwd<T>(L, n, lda, tmp, x, sync, s);
+}
+
+// ------------------------------------------------------------------------------------------
+// Multiple right-hand sides: V := L^-1 V, 64-row steps. A step solves the 64 x nrhs block
+// (one lane per right-hand side, FP64 substitution against the diagonal block in LDS), then
+// updates the rows below with an FP32 MFMA product.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void trsm_multi_diag_kernel(const float* __restrict__ L,
+                                                              int64_t ldl, int64_t k0, int w,
+                                                              float* __restrict__ V, int64_t ldv,
+                                                              int64_t nrhs) {
+    __shared__ double sL[PF_IB][PF_IB + 1];
+    __shared__ double sInv[PF_IB];
+    load_diag_block(sL, sInv, L, ldl, k0, w);
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nrhs) return;
+    float* col = V + k0 + j * ldv;
+    double x[PF_IB];
+#pragma unroll
+    for (int c = 0; c < PF_IB; ++c) {
+        if (c < w) {
+            double s = (double)col[c];
+#pragma unroll
+            for (int l = 0; l < c; ++l) s = s - sL[c][l] * x[l];
+            x[c] = s * sInv[c];
+            col[c] = (float)x[c];
+        }
+    }
+}
+
+int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
+                         int64_t nrhs, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0) return QMLH_OK;
+    for (int64_t k = 0; k < n; k += PF_IB) {
+        const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
+        hipLaunchKernelGGL(trsm_multi_diag_kernel, dim3((unsigned)ceil_div(nrhs, 256)), dim3(256),
+                           0, s, L, ldl, k, w, V, ldv, nrhs);
+        QMLH_CHECK_LAUNCH("trsm multi diag launch");
+        const int64_t rest = n - k - w;
+        // V[k+w:, :] -= L[k+w:, k:k+w] * V[k:k+w, :]; B(j, c) = V[(k + c) + j ldv] (row layout).
+        QMLH_TRY((launch_update<float, LAYOUT_ROW>(false, L + (k + w) + k * ldl, ldl, V + k, ldv,
+                                                   rest, nrhs, w, -1.0, V + (k + w), ldv, nullptr,
+                                                   s)));
+    }
+    return QMLH_OK;
+}
+
+template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t);
+template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t);
+template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, int32_t*,
+                                hipStream_t);
+template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
+                               hipStream_t);
+template int trsv_lower_fwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
+                                   hipStream_t);
+template int trsv_lower_bwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
+                                   hipStream_t);
+template int trsv_lower_fwd<double>(const double*, int64_t, int64_t, double*, double*,
+                                    int32_t*, hipStream_t);
+template int trsv_lower_bwd<double>(const double*, int64_t, int64_t, double*, double*,
+                                    int32_t*, hipStream_t);
 
 }  // namespace qmlh
 
@@ -403,7 +481,8 @@ extern "C" int qmlh_spotrf(float* A, int64_t n, int64_t lda, int32_t* info,
 }
 
 extern "C" size_t qmlh_potrs_workspace(int64_t n) {
-    return n > 0 ? (size_t)n * sizeof(double) : 0;
+    if (n <= 0) return 0;
+    return (size_t)n * sizeof(double) + qmlh::trsv_sync_ints(n) * sizeof(int32_t);
 }
 
 extern "C" int qmlh_potrs(const double* L, int64_t n, int64_t lda, double* x, void* ws,
@@ -411,5 +490,7 @@ extern "C" int qmlh_potrs(const double* L, int64_t n, int64_t lda, double* x, vo
     using namespace qmlh;
     if (n > 0 && ws_bytes < qmlh_potrs_workspace(n))
         return set_error(QMLH_ERR_WORKSPACE, "potrs: workspace too small");
-    return potrs_impl<double>(L, n, lda, x, static_cast<double*>(ws), (hipStream_t)stream);
+    double* tmp = static_cast<double*>(ws);
+    int32_t* sync = n > 0 ? reinterpret_cast<int32_t*>(tmp + n) : nullptr;
+    return potrs_impl<double>(L, n, lda, x, tmp, sync, (hipStream_t)stream);
 }

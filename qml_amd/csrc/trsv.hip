@@ -1,0 +1,197 @@
+// Single-launch triangular solves with a lower-triangular L (column-major, storage T):
+//   forward  z = L^-1 r        backward  x = L^-T z
+// One workgroup per 64-row block. A workgroup takes its block from an atomic ticket, so
+// blocks are claimed in launch order and a block only ever waits for blocks already claimed
+// by running workgroups (no deadlock, whatever the dispatch order). It accumulates the
+// contributions of the finished blocks as they are published -- loading the L tile before
+// waiting on the block's flag, so the loads overlap the wait -- then solves its 64 x 64
+// diagonal block in FP64 and publishes the result (stores, agent-scope release, flag).
+// Every wait is bounded: past the spin limit the workgroup records an error and stops
+// waiting, so the kernel always drains; the host turns the error into a failed status.
+//
+// The accumulation order is fixed (finished blocks in solve order, a fixed intra-thread
+// order, a fixed final reduction tree), so results are deterministic.
+#include "common.h"
+
+namespace qmlh {
+
+static constexpr int TS_B = 64;
+static constexpr unsigned TS_SPIN_LIMIT = 1u << 24;
+
+// sync: [0] ticket, [1] error, [2 .. 2 + nblocks) flags; cleared before each solve.
+__global__ void trsv_sync_clear_kernel(int32_t* sync, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) sync[i] = 0;
+}
+
+// Spin (bounded) until the block's flag is set. The flag and the published values are read
+// with agent-scope atomic loads, which go to the point of coherence, instead of an
+// agent-scope acquire fence: such a fence invalidates the whole L2 of the XCD and would throw
+// away the tiles every other workgroup has in flight.
+__device__ __forceinline__ bool wait_flag(int32_t* flag, int32_t* err) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (++spins > TS_SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+template <typename T, bool FWD>
+__device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L, int64_t ldl,
+                                            int64_t n, int64_t r0, int w, int64_t j, int lane,
+                                            int grp) {
+    const int64_t c0 = j * TS_B;
+    const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int cc = 16 * grp + c;
+        if (FWD)  // L(r0 + lane, c0 + cc)
+            v[c] = (lane < w && cc < wj) ? L[(r0 + lane) + (c0 + cc) * ldl] : T(0);
+        else      // L(c0 + lane, r0 + cc): rows of block j, columns of block b
+            v[c] = (lane < wj && cc < w) ? L[(c0 + lane) + (r0 + cc) * ldl] : T(0);
+    }
+}
+
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
+                                                            int64_t n, const double* __restrict__ rhs,
+                                                            double* __restrict__ out,
+                                                            int32_t* __restrict__ sync) {
+    __shared__ double sL[TS_B][TS_B + 1];
+    __shared__ double sInv[TS_B];
+    __shared__ double red[TS_B][TS_B + 1];
+    __shared__ double sv[TS_B];
+    __shared__ int s_block, s_fail;
+    const int t = threadIdx.x;
+    const int64_t nblocks = (n + TS_B - 1) / TS_B;
+    int32_t* flags = sync + 2;
+    if (t == 0) {
+        const int ticket = atomicAdd(&sync[0], 1);
+        s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
+        s_fail = 0;
+    }
+    __syncthreads();
+    const int64_t b = s_block;
+    const int64_t r0 = b * TS_B;
+    const int w = (int)(n - r0 < TS_B ? n - r0 : TS_B);
+    const int lane = t & 63, grp = t >> 6;  // 4 groups of 16 columns
+
+    // The diagonal block does not depend on other blocks: stage it first.
+    for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+        const int i = idx % TS_B, jj = idx / TS_B;
+        if (i < w && jj < w && i >= jj) sL[i][jj] = (double)L[(r0 + i) + (r0 + jj) * ldl];
+    }
+
+    // Forward: thread (row lane, columns 16 grp ..) keeps one partial sum.
+    // Backward: thread (row lane of block j, columns 16 grp .. of block b) keeps 16.
+    double acc1 = 0.0;
+    double acc16[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc16[c] = 0.0;
+    bool ok = true;
+    const int64_t nj = FWD ? b : nblocks - 1 - b;
+    T lv[16], ln[16];
+    if (nj > 0) load_tile16<T, FWD>(lv, L, ldl, n, r0, w, FWD ? 0 : nblocks - 1, lane, grp);
+    for (int64_t q = 0; q < nj; ++q) {
+        const int64_t j = FWD ? q : nblocks - 1 - q;
+        if (q + 1 < nj) load_tile16<T, FWD>(ln, L, ldl, n, r0, w, FWD ? q + 1 : j - 1, lane, grp);
+        if (ok) ok = wait_flag(&flags[j], &sync[1]);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int64_t c0 = j * TS_B;
+        const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
+        if (FWD) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int cc = 16 * grp + c;
+                const double zc = cc < wj ? __hip_atomic_load(&out[c0 + cc], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : 0.0;
+                acc1 = fma((double)lv[c], zc, acc1);
+            }
+        } else {
+            const double xj = lane < wj ? __hip_atomic_load(&out[c0 + lane], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc16[c] = fma((double)lv[c], xj, acc16[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) lv[c] = ln[c];
+    }
+    if (!ok) s_fail = 1;  // benign race: every writer stores 1
+
+    if (FWD) {
+        red[lane][grp] = acc1;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) red[lane][16 * grp + c] = acc16[c];
+    }
+    __syncthreads();
+    if (t < TS_B) {
+        double s = 0.0;
+        if (FWD) {
+            s = ((red[t][0] + red[t][1]) + red[t][2]) + red[t][3];
+        } else {
+            for (int i = 0; i < TS_B; ++i) s += red[i][t];
+        }
+        sv[t] = (t < w ? rhs[r0 + t] : 0.0) - s;
+        if (t < w) sInv[t] = 1.0 / sL[t][t];
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int i = t;
+        double xi = sv[i];
+        if (FWD) {
+            for (int c = 0; c < w; ++c) {
+                const double xc = __shfl(xi * sInv[c], c, 64);
+                if (i == c) xi = xc;
+                if (i > c && i < w) xi = xi - sL[i][c] * xc;
+            }
+        } else {
+            for (int c = w - 1; c >= 0; --c) {
+                const double xc = __shfl(xi * sInv[c], c, 64);
+                if (i == c) xi = xc;
+                if (i < c) xi = xi - sL[c][i] * xc;
+            }
+        }
+        if (i < w)
+            __hip_atomic_store(&out[r0 + i], s_fail ? __builtin_nan("") : xi, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (t == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+size_t trsv_sync_ints(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B + 2); }
+
+template <typename T>
+int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
+                  int32_t* sync, bool forward, hipStream_t s) {
+    if (n <= 0) return QMLH_OK;
+    const int64_t nblocks = (n + TS_B - 1) / TS_B;
+    const int64_t cnt = nblocks + 2;
+    hipLaunchKernelGGL(trsv_sync_clear_kernel, dim3((unsigned)ceil_div(cnt, 256)), dim3(256), 0, s,
+                       sync, cnt);
+    if (forward)
+        hipLaunchKernelGGL((trsv_syncfree_kernel<T, true>), dim3((unsigned)nblocks), dim3(256), 0,
+                           s, L, ldl, n, rhs, out, sync);
+    else
+        hipLaunchKernelGGL((trsv_syncfree_kernel<T, false>), dim3((unsigned)nblocks), dim3(256),
+                           0, s, L, ldl, n, rhs, out, sync);
+    QMLH_CHECK_LAUNCH("trsv launch");
+    return QMLH_OK;
+}
+
+template int trsv_syncfree<double>(const double*, int64_t, int64_t, const double*, double*,
+                                   int32_t*, bool, hipStream_t);
+template int trsv_syncfree<float>(const float*, int64_t, int64_t, const double*, double*,
+                                  int32_t*, bool, hipStream_t);
+
+}  // namespace qmlh

@@ -119,21 +119,26 @@ class KRR:
         K = torch.empty((n, n), dtype=torch.float64, device=dev)
         B = torch.empty((n, n), dtype=torch.float32, device=dev)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        norms = u = None
+        norms = u = V = C = None
+        r = 0
         if self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
             u = torch.empty(n, dtype=torch.float64, device=dev)
+            r = d + 1
+            V = torch.empty((r, n), dtype=torch.float32, device=dev)  # (n, r) column-major
+            C = torch.empty((r, r), dtype=torch.float64, device=dev)
         else:
             self.build_kernel(Xd, n, d, out=K, stream=stream)
             add_diagonal_device(K, n, self.lam, stream)
-        # FP32 factor; on breakdown retry with a diagonal shift (the CG on the FP64 matrix
-        # still converges to the unshifted solution, only more slowly).
+        # Low-precision factor; on breakdown retry with a diagonal shift (the CG on the FP64
+        # matrix still converges to the unshifted solution, only more slowly).
         shifts = [0.0] + [max(self.lam, 1e-12) * 10.0 ** k for k in range(1, 9)]
         for attempt, shift in enumerate(shifts):
             if self.kernel == "gaussian":
                 _lib.check(L.qmlh_gaussian_deflated(
                     _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(norms), d, self.sigma, self.lam, shift,
-                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), n, _lib.ptr(u), h))
+                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), n, _lib.ptr(V), n,
+                    _lib.ptr(u), h))
             else:
                 _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), n, shift, h))
             _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
@@ -145,18 +150,24 @@ class KRR:
             if raise_on_error:
                 raise CholeskyError(f"FP32 factorisation failed at every shift: info={rep.info}")
             return rep
+        if r:
+            _lib.check(L.qmlh_woodbury_setup(_lib.ptr(B), n, n, _lib.ptr(V), n, r, _lib.ptr(C), r,
+                                             _lib.ptr(info), h))
+            if int(info.item()) != 0:
+                raise CholeskyError("Woodbury capacitance matrix is not positive definite")
         alpha = torch.zeros(n, dtype=torch.float64, device=dev)
         yd = torch.as_tensor(y, dtype=torch.float64).to(dev).contiguous()
-        ws_bytes = L.qmlh_pcg_workspace(n)
-        ws = torch.empty(max(1, ws_bytes // 8), dtype=torch.float64, device=dev)
+        ws_bytes = L.qmlh_pcg_workspace(n, r)
+        ws = torch.empty(max(1, ws_bytes // 8 + 1), dtype=torch.float64, device=dev)
         iters = ctypes.c_int32()
         relres = ctypes.c_double()
-        _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), n, _lib.ptr(u), _lib.ptr(yd),
-                                    _lib.ptr(alpha), self.tol, self.max_iter, _lib.ptr(ws),
-                                    ws.numel() * 8, ctypes.byref(iters), ctypes.byref(relres), h))
+        _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), n, _lib.ptr(V), n, r,
+                                    _lib.ptr(C), r, _lib.ptr(yd), _lib.ptr(alpha), self.tol,
+                                    self.max_iter, _lib.ptr(ws), ws.numel() * 8,
+                                    ctypes.byref(iters), ctypes.byref(relres), h))
         rep.iterations, rep.relres = iters.value, relres.value
         rep.seconds["fit"] = time.perf_counter() - t0
-        del K, B, ws
+        del K, B, ws, V, C
         if norms is None and self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
         self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d

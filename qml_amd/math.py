@@ -28,10 +28,12 @@ def potrs_device(Ld, n: int, xd, stream=None, workspace=None):
     """x := (L L^T)^-1 x with L from potrf_device; x is an (n,) FP64 device tensor."""
     import torch
 
+    L = _lib.lib()
     if workspace is None:
-        workspace = torch.empty(max(1, n), dtype=torch.float64, device=xd.device)
-    _lib.check(_lib.lib().qmlh_potrs(_lib.ptr(Ld), n, n, _lib.ptr(xd), _lib.ptr(workspace),
-                                     workspace.numel() * 8, _lib.stream_handle(stream)))
+        nbytes = L.qmlh_potrs_workspace(n)
+        workspace = torch.empty(max(1, (nbytes + 7) // 8), dtype=torch.float64, device=xd.device)
+    _lib.check(L.qmlh_potrs(_lib.ptr(Ld), n, n, _lib.ptr(xd), _lib.ptr(workspace),
+                            workspace.numel() * 8, _lib.stream_handle(stream)))
     return xd
 
 

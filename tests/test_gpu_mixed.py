@@ -75,11 +75,13 @@ def test_gaussian_deflated_build_identities():
     nrm = row_sqnorms_device(Xd, n, d)
     K = torch.empty((n, n), dtype=torch.float64, device="cuda")
     B = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    V = torch.empty((d + 1, n), dtype=torch.float32, device="cuda")
     u = torch.empty(n, dtype=torch.float64, device="cuda")
     lam, shift, sigma = 1e-6, 3e-5, 4000.0
     _lib.check(_lib.lib().qmlh_gaussian_deflated(Xd.data_ptr(), n, Xd.shape[1], nrm.data_ptr(),
                                                  d, sigma, lam, shift, K.data_ptr(), n,
-                                                 B.data_ptr(), n, u.data_ptr(), None))
+                                                 B.data_ptr(), n, V.data_ptr(), n, u.data_ptr(),
+                                                 None))
     Kref = gaussian_mfma_device(Xd, nrm, n, Xd, nrm, n, d, sigma, True).cpu().numpy().T
     Kg = K.cpu().numpy().T
     off = ~np.eye(n, dtype=bool)
@@ -89,10 +91,19 @@ def test_gaussian_deflated_build_identities():
     assert np.max(np.abs(Kg - lam * np.eye(n) - Kor) / Kor) <= 1e-12
     ug = u.cpu().numpy()
     np.testing.assert_allclose(ug, np.exp(-(X * X).sum(1) / (2 * sigma ** 2)), rtol=1e-14)
+    Vg = V.cpu().numpy().T.astype(np.float64)  # (n, d+1)
+    np.testing.assert_allclose(Vg[:, 0], ug, rtol=1e-7)
+    np.testing.assert_allclose(Vg[:, 1:], ug[:, None] * X / sigma, rtol=1e-7, atol=1e-30)
     Bg = np.tril(B.cpu().numpy().T.astype(np.float64))
-    Bexp = np.tril(Kor - np.outer(ug, ug) + (lam + shift) * np.eye(n))
-    # B's entries are O(1e-2); FP32 rounding plus the cancellation-free expm1 form.
-    assert np.max(np.abs(Bg - Bexp)) <= 1e-8
+    # Remainder of the rank-(d+1) expansion, computed independently in FP64 on the host.
+    G = X @ X.T / sigma ** 2
+    Bexp = np.tril(np.outer(ug, ug) * (np.expm1(G) - G) + (lam + shift) * np.eye(n))
+    assert np.max(np.abs(Bexp)) < 5e-3  # small entries: the point of the deflation
+    assert np.max(np.abs(Bg - Bexp)) <= 2e-10
+    # K + lambda I = V V^T + B - shift I, to FP32 rounding of V.
+    VVt = Vg @ Vg.T
+    assert np.max(np.abs(np.tril(VVt) + Bexp - (lam + shift) * np.tril(np.eye(n))
+                         + lam * np.tril(np.eye(n)) - np.tril(Kor + lam * np.eye(n)))) <= 1e-6
     assert np.all(np.triu(B.cpu().numpy().T, 1) == 0)
 
 
@@ -112,14 +123,48 @@ def test_pcg_solve_random_spd():
     assert int(info.item()) == 0
     yd = torch.from_numpy(y).cuda()
     x = torch.empty(n, dtype=torch.float64, device="cuda")
-    ws = torch.empty(L.qmlh_pcg_workspace(n) // 8 + 1, dtype=torch.float64, device="cuda")
+    ws = torch.empty(L.qmlh_pcg_workspace(n, 0) // 8 + 1, dtype=torch.float64, device="cuda")
     it, rr = ctypes.c_int32(), ctypes.c_double()
-    _lib.check(L.qmlh_pcg_solve(Ad.data_ptr(), n, n, Bd.data_ptr(), n, None, yd.data_ptr(),
-                                x.data_ptr(), 1e-13, 50, ws.data_ptr(), ws.numel() * 8,
-                                ctypes.byref(it), ctypes.byref(rr), None))
+    _lib.check(L.qmlh_pcg_solve(Ad.data_ptr(), n, n, Bd.data_ptr(), n, None, n, 0, None, 1,
+                                yd.data_ptr(), x.data_ptr(), 1e-13, 50, ws.data_ptr(),
+                                ws.numel() * 8, ctypes.byref(it), ctypes.byref(rr), None))
     assert rr.value <= 1e-13 and it.value <= 10
     xs = np.linalg.solve(A, y)
     assert np.linalg.norm(x.cpu().numpy() - xs) <= 1e-11 * np.linalg.norm(xs)
+
+
+def test_woodbury_preconditioner_is_exact_for_exact_factor():
+    # With an exact factor of B, M = B + V V^T equals A, so PCG converges in one iteration.
+    from qml_amd import _lib
+
+    n, r = 700, 9
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((n, n)) / np.sqrt(n)
+    Bm = M @ M.T + 0.5 * np.eye(n)
+    Vm = rng.standard_normal((n, r)) * 3
+    A = Bm + Vm @ Vm.T
+    y = rng.standard_normal(n)
+    L = _lib.lib()
+    Bd = torch.from_numpy(Bm.astype(np.float32)).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.check(L.qmlh_spotrf(Bd.data_ptr(), n, n, info.data_ptr(), None))
+    Vd = torch.from_numpy(np.ascontiguousarray(Vm.T).astype(np.float32)).cuda()  # column-major
+    Cd = torch.empty((r, r), dtype=torch.float64, device="cuda")
+    _lib.check(L.qmlh_woodbury_setup(Bd.data_ptr(), n, n, Vd.data_ptr(), n, r, Cd.data_ptr(), r,
+                                     info.data_ptr(), None))
+    assert int(info.item()) == 0
+    Ad = torch.from_numpy(A.copy()).cuda()
+    yd = torch.from_numpy(y).cuda()
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    ws = torch.empty(L.qmlh_pcg_workspace(n, r) // 8 + 1, dtype=torch.float64, device="cuda")
+    it, rr = ctypes.c_int32(), ctypes.c_double()
+    _lib.check(L.qmlh_pcg_solve(Ad.data_ptr(), n, n, Bd.data_ptr(), n, Vd.data_ptr(), n, r,
+                                Cd.data_ptr(), r, yd.data_ptr(), x.data_ptr(), 1e-12, 50,
+                                ws.data_ptr(), ws.numel() * 8, ctypes.byref(it), ctypes.byref(rr),
+                                None))
+    assert rr.value <= 1e-12 and it.value <= 4  # FP32 factor: a couple of refinement steps
+    assert np.linalg.norm(x.cpu().numpy() - np.linalg.solve(A, y)) <= 1e-10 * np.linalg.norm(
+        np.linalg.solve(A, y))
 
 
 @pytest.mark.parametrize("kernel,sigma,lam,n", [("gaussian", 4000.0, 1e-6, 3000),
@@ -138,6 +183,6 @@ def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n):
     rep = mx.fit(X, tr.labels)
     pmx = mx.predict(Xs)
     _, pref = oracle.krr_fit_predict(X, tr.labels, Xs, kernel, sigma, lam)
-    assert rep.relres <= 1e-11 and rep.iterations < 60, rep
+    assert rep.relres <= 1e-11 and rep.iterations < 30, rep
     assert np.max(np.abs(pmx - p64)) <= 1e-8 * std
     assert np.max(np.abs(pmx - pref)) <= 1e-8 * std

@@ -241,6 +241,33 @@ def test_potrf_potrs_vs_lapack(n):
     assert np.linalg.norm(A @ x - y) <= 1e-13 * np.linalg.norm(A) * np.linalg.norm(x)
 
 
+@pytest.mark.parametrize("n", [1024, 4100, 9001])
+def test_potrs_single_launch_solves(n):
+    # Orders >= 1024 use the single-launch triangular solves (workgroups hand blocks to each
+    # other through flags); repeated solves must agree bit for bit and match LAPACK.
+    from scipy.linalg import lapack
+
+    from qml_amd.math import potrf_device, potrs_device
+
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, 64))
+    A = M @ M.T / 64 + np.eye(n)
+    Ad = torch.from_numpy(A.copy()).cuda()
+    assert int(potrf_device(Ad, n).item()) == 0
+    Lref = np.tril(lapack.dpotrf(A, lower=1)[0])
+    y = rng.standard_normal(n)
+    outs = []
+    for _ in range(4):
+        xd = torch.from_numpy(y.copy()).cuda()
+        potrs_device(Ad, n, xd)
+        outs.append(xd.cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(bits(o), bits(outs[0]))
+    xref = lapack.dpotrs(Lref, y.reshape(-1, 1), lower=1)[0].ravel()
+    assert np.linalg.norm(outs[0] - xref) <= 1e-11 * np.linalg.norm(xref)
+    assert np.linalg.norm(A @ outs[0] - y) <= 1e-13 * np.linalg.norm(A) * np.linalg.norm(outs[0])
+
+
 @pytest.mark.parametrize("bad", [0, 5, 64, 255, 256, 300])
 def test_potrf_info_matches_lapack(bad):
     from scipy.linalg import lapack
