@@ -148,6 +148,13 @@ int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stre
  * the iterations and the final relative residual. Synchronises the stream once per iteration.
  * ---------------------------------------------------------------------------------------- */
 int qmlh_spotrf(float *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
+/* qmlh_spotrf with the O(n^3) trailing updates in split BF16: P P^T ~ Ph Ph^T + Ph Pl^T +
+ * Pl Ph^T (Ph = bf16(P), Pl = bf16(P - Ph)) on BF16 MFMA with FP32 accumulation, about
+ * 2^-16 relative per update; for factors that only precondition (qmlh_pcg_solve).
+ * ws: qmlh_spotrf_bf16x3_workspace(n) bytes, 16-byte aligned. */
+size_t qmlh_spotrf_bf16x3_workspace(int64_t n);
+int qmlh_spotrf_bf16x3(float *A, int64_t n, int64_t lda, int32_t *info, void *ws,
+                       size_t ws_bytes, qmlh_stream_t stream);
 int qmlh_gaussian_deflated(const double *X, int64_t n, int64_t ldx, const double *norms,
                            int64_t d, double sigma, double lambda, double shift, double *K,
                            int64_t ldk, float *B32, int64_t ldb, float *V, int64_t ldv,

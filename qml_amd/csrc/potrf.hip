@@ -15,6 +15,8 @@
 // A device int32 `info` follows LAPACK: the first non-positive (or NaN) pivot j sets
 // info = j + 1, its Schur-complement value is left on the diagonal, and every later kernel of
 // the factorisation returns without work.
+#include <type_traits>
+
 #include "linalg.h"
 #include "mfma_core.h"
 #include "timer.h"
@@ -222,7 +224,7 @@ static int factor_diag(T* A, int64_t lda, int64_t kb, int64_t col_base, int32_t*
 __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 
 template <typename T>
-int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s) {
+int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* bf16_ws) {
     if (n < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0");
     if (lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n");
     if (!info) return set_error(QMLH_ERR_ARG, "potrf: info is null");
@@ -239,8 +241,14 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s) {
         if (m > 0) {
             T* P = Akk + kb;
             QMLH_TRY(trsm_panel(P, lda, m, (const T*)Akk, lda, kb, info, s));
-            QMLH_TRY(launch_update<T>(true, P, lda, P, lda, m, m, kb, -1.0, P + kb * lda, lda,
-                                      info, s));
+            if (std::is_same<T, float>::value && bf16_ws) {
+                QMLH_TRY(syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb, -1.0f,
+                                     reinterpret_cast<float*>(P + kb * lda), lda, info, bf16_ws,
+                                     s));
+            } else {
+                QMLH_TRY(launch_update<T>(true, P, lda, P, lda, m, m, kb, -1.0, P + kb * lda,
+                                          lda, info, s));
+            }
         }
     }
     return QMLH_OK;
@@ -453,8 +461,8 @@ int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64
     return QMLH_OK;
 }
 
-template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t);
-template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t);
+template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*);
+template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*);
 template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, int32_t*,
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
@@ -478,6 +486,20 @@ extern "C" int qmlh_potrf(double* A, int64_t n, int64_t lda, int32_t* info,
 extern "C" int qmlh_spotrf(float* A, int64_t n, int64_t lda, int32_t* info,
                            qmlh_stream_t stream) {
     return qmlh::potrf_impl<float>(A, n, lda, info, (hipStream_t)stream);
+}
+
+extern "C" size_t qmlh_spotrf_bf16x3_workspace(int64_t n) {
+    return n > 0 ? qmlh::syrk_bf16_workspace(n, qmlh::PF_NB) : 0;
+}
+
+extern "C" int qmlh_spotrf_bf16x3(float* A, int64_t n, int64_t lda, int32_t* info, void* ws,
+                                  size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n > 0 && (!ws || ws_bytes < qmlh_spotrf_bf16x3_workspace(n)))
+        return set_error(QMLH_ERR_WORKSPACE, "spotrf_bf16x3: workspace too small");
+    if (reinterpret_cast<uintptr_t>(ws) & 15)
+        return set_error(QMLH_ERR_ARG, "spotrf_bf16x3: workspace must be 16-byte aligned");
+    return potrf_impl<float>(A, n, lda, info, (hipStream_t)stream, ws);
 }
 
 extern "C" size_t qmlh_potrs_workspace(int64_t n) {

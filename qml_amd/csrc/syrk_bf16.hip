@@ -1,0 +1,197 @@
+// Split-BF16 trailing update for the mixed-precision factor: C -= P P^T (lower tiles) with
+// P = Ph + Pl, Ph = bf16(P), Pl = bf16(P - Ph), evaluated as three BF16 products accumulated
+// in FP32 on v_mfma_f32_16x16x32_bf16:
+//     P P^T ~ Ph Ph^T + Ph Pl^T + Pl Ph^T
+// (each BF16 x BF16 product is exact in FP32; the dropped Pl Pl^T term is ~2^-16 relative).
+// This is the dense O(n^3) part of the mixed factorisation; the panel, the diagonal blocks and
+// the substitutions stay in FP32 storage / FP64 arithmetic.
+//
+// Panel layout. The split kernel writes Ph and Pl in MFMA fragment order: element (row, k)
+// of the (m_pad x kb) panel at ((c * 4 + q) * m_pad + row) * 8 + e with k = 32 c + 8 q + e,
+// so that one 32-deep chunk of 128 rows is four contiguous 2 KB runs, read with 16-byte loads
+// and copied unchanged into LDS, where lane (r, g) of a wave finds its 8 k-values of row r
+// (k = 8 g .. 8 g + 7) as one 16-byte read.
+//
+// Tiles: 256 lanes own a 128 x 128 lower tile of C (2 x 2 waves of 64 x 64); chunks of 32
+// features are double buffered through LDS (4 panels x 8 KB per buffer).
+#include "common.h"
+#include "timer.h"
+
+namespace qmlh {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+static constexpr int SB_T = 128;          // tile rows
+static constexpr int SB_KC = 32;          // features per chunk
+static constexpr int SB_PANEL = SB_T * SB_KC;  // bf16 elements per panel per chunk (8 KB)
+
+__device__ __forceinline__ int64_t split_index(int64_t row, int64_t k, int64_t m_pad) {
+    const int64_t c = k >> 5, q = (k >> 3) & 3, e = k & 7;
+    return ((c * 4 + q) * m_pad + row) * 8 + e;
+}
+
+// Ph, Pl <- split of P (column-major FP32, m x kb); rows m .. m_pad-1 and features
+// kb .. kb_pad-1 are zero.
+__global__ void split_panel_kernel(const float* __restrict__ P, int64_t ldp, int64_t m,
+                                   int64_t m_pad, int64_t kb, int64_t kb_pad,
+                                   __bf16* __restrict__ Ph, __bf16* __restrict__ Pl) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = blockIdx.y;
+    if (row >= m_pad || k >= kb_pad) return;
+    const float v = (row < m && k < kb) ? P[row + k * ldp] : 0.0f;
+    const __bf16 h = (__bf16)v;
+    const __bf16 l = (__bf16)(v - (float)h);
+    const int64_t idx = split_index(row, k, m_pad);
+    Ph[idx] = h;
+    Pl[idx] = l;
+}
+
+// Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) into an LDS panel.
+__device__ __forceinline__ void sb_load(uint4 (&r)[2], const __bf16* __restrict__ src,
+                                        int64_t m_pad, int64_t row0, int64_t c) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = t + 256 * h;  // 0 .. 511
+        const int q = p >> 7, row = p & 127;
+        const __bf16* g = src + ((c * 4 + q) * m_pad + row0 + row) * 8;
+        r[h] = *reinterpret_cast<const uint4*>(g);
+    }
+}
+
+__device__ __forceinline__ void sb_store(__bf16* __restrict__ panel, const uint4 (&r)[2]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = t + 256 * h;
+        reinterpret_cast<uint4*>(panel)[p] = r[h];  // (q * 128 + row) * 8 elements
+    }
+}
+
+__device__ __forceinline__ bf16x8_t sb_frag(const __bf16* __restrict__ panel, int row, int g) {
+    return *reinterpret_cast<const bf16x8_t*>(panel + (g * SB_T + row) * 8);
+}
+
+__global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __restrict__ Ph,
+                                                             const __bf16* __restrict__ Pl,
+                                                             int64_t m, int64_t m_pad,
+                                                             int64_t kb_pad, float alpha,
+                                                             float* __restrict__ C, int64_t ldc,
+                                                             const int32_t* __restrict__ info) {
+    // buffers x {row hi, row lo, col hi, col lo}
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2][4][SB_PANEL];
+    if (info && *info) return;
+    const int64_t bid = blockIdx.x;
+    int64_t bi = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while (bi * (bi + 1) / 2 > bid) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= bid) ++bi;
+    const int64_t bj = bid - bi * (bi + 1) / 2;  // bi >= bj
+    const int64_t rows0 = bi * SB_T, cols0 = bj * SB_T;
+
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int r = lane & 15, g = lane >> 4;
+    // Wave w: first operand (accumulator rows) = output columns cols0 + 64 (w & 1),
+    // second operand = output rows rows0 + 64 (w >> 1).
+    const int fr = 64 * (w & 1), sr = 64 * (w >> 1);
+
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t nchunks = kb_pad / SB_KC;
+    uint4 rh[2], rl[2], ch[2], cl[2];
+    sb_load(rh, Ph, m_pad, rows0, 0);
+    sb_load(rl, Pl, m_pad, rows0, 0);
+    sb_load(ch, Ph, m_pad, cols0, 0);
+    sb_load(cl, Pl, m_pad, cols0, 0);
+    sb_store(lds[0][0], rh);
+    sb_store(lds[0][1], rl);
+    sb_store(lds[0][2], ch);
+    sb_store(lds[0][3], cl);
+    __syncthreads();
+    int st = 0;
+    for (int64_t c = 0; c < nchunks; ++c) {
+        const bool more = c + 1 < nchunks;
+        if (more) {
+            sb_load(rh, Ph, m_pad, rows0, c + 1);
+            sb_load(rl, Pl, m_pad, rows0, c + 1);
+            sb_load(ch, Ph, m_pad, cols0, c + 1);
+            sb_load(cl, Pl, m_pad, cols0, c + 1);
+        }
+        bf16x8_t fh[4], fl[4], sh[4], sl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            fh[q] = sb_frag(lds[st][2], fr + 16 * q + r, g);
+            fl[q] = sb_frag(lds[st][3], fr + 16 * q + r, g);
+            sh[q] = sb_frag(lds[st][0], sr + 16 * q + r, g);
+            sl[q] = sb_frag(lds[st][1], sr + 16 * q + r, g);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[a], sh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], sl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], sh[b], acc[a][b], 0, 0, 0);
+            }
+        if (more) {
+            sb_store(lds[st ^ 1][0], rh);
+            sb_store(lds[st ^ 1][1], rl);
+            sb_store(lds[st ^ 1][2], ch);
+            sb_store(lds[st ^ 1][3], cl);
+        }
+        __syncthreads();
+        st ^= 1;
+    }
+
+    const int64_t col0 = cols0 + fr;
+    const int64_t row0 = rows0 + sr;
+    if (row0 >= m || col0 >= m || col0 > row0 + 63) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + r;
+            if (i >= m) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + 16 * a + 4 * g + e;  // f32 accumulator rows: 4 g + e
+                if (j >= m || j > i) continue;
+                float* cp = C + i + j * ldc;
+                *cp = *cp + alpha * acc[a][b][e];
+            }
+        }
+    }
+}
+
+size_t syrk_bf16_workspace(int64_t n, int64_t kb) {
+    const int64_t m_pad = round_up(n > 0 ? n : 1, SB_T);
+    const int64_t kb_pad = round_up(kb, SB_KC);
+    return (size_t)(2 * m_pad * kb_pad) * sizeof(__bf16);
+}
+
+// C (m x m, lower) += alpha * P P^T with P (m x kb) column-major FP32; ws holds
+// syrk_bf16_workspace(m, kb) bytes.
+int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float alpha, float* C,
+                int64_t ldc, const int32_t* info, void* ws, hipStream_t s) {
+    if (m <= 0 || kb <= 0) return QMLH_OK;
+    const int64_t m_pad = round_up(m, SB_T);
+    const int64_t kb_pad = round_up(kb, SB_KC);
+    __bf16* Ph = static_cast<__bf16*>(ws);
+    __bf16* Pl = Ph + m_pad * kb_pad;
+    hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)kb_pad),
+                       dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl);
+    QMLH_CHECK_LAUNCH("split_panel launch");
+    const int64_t tm = m_pad / SB_T;
+    TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
+    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3((unsigned)(tm * (tm + 1) / 2)), dim3(256), 0, s,
+                       Ph, Pl, m, m_pad, kb_pad, alpha, C, ldc, info);
+    QMLH_CHECK_LAUNCH("syrk_bf16x3 launch");
+    return QMLH_OK;
+}
+
+}  // namespace qmlh

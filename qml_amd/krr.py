@@ -52,7 +52,8 @@ class KRR:
     """
 
     def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9,
-                 precision: str = "fp64", tol: float = 1e-10, max_iter: int = 200):
+                 precision: str = "fp64", tol: float = 1e-10, max_iter: int = 200,
+                 factor: str | None = None):
         if kernel not in KERNELS:
             raise ValueError(f"kernel must be one of {sorted(KERNELS)}")
         if precision not in ("fp64", "mixed"):
@@ -63,6 +64,13 @@ class KRR:
         self.precision = precision
         self.tol = float(tol)
         self.max_iter = int(max_iter)
+        # Mixed-precision factor: "bf16x3" (split-BF16 trailing updates) needs the deflated
+        # Gaussian remainder to stay accurate relative to lambda; "fp32" everywhere else.
+        if factor is None:
+            factor = "bf16x3" if kernel == "gaussian" else "fp32"
+        if factor not in ("fp32", "bf16x3"):
+            raise ValueError("factor must be 'fp32' or 'bf16x3'")
+        self.factor = factor
         self.Xd = None
         self.norms = None
         self.alpha = None
@@ -133,6 +141,7 @@ class KRR:
         # Low-precision factor; on breakdown retry with a diagonal shift (the CG on the FP64
         # matrix still converges to the unshifted solution, only more slowly).
         shifts = [0.0] + [max(self.lam, 1e-12) * 10.0 ** k for k in range(1, 9)]
+        bws = None
         for attempt, shift in enumerate(shifts):
             if self.kernel == "gaussian":
                 _lib.check(L.qmlh_gaussian_deflated(
@@ -141,7 +150,14 @@ class KRR:
                     _lib.ptr(u), h))
             else:
                 _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), n, shift, h))
-            _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
+            if self.factor == "bf16x3":
+                if bws is None:
+                    nb = L.qmlh_spotrf_bf16x3_workspace(n)
+                    bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
+                _lib.check(L.qmlh_spotrf_bf16x3(_lib.ptr(B), n, n, _lib.ptr(info), _lib.ptr(bws),
+                                                bws.numel() * 8, h))
+            else:
+                _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
             rep.info = int(info.item())
             rep.shift = shift
             if rep.info == 0:
@@ -167,7 +183,7 @@ class KRR:
                                     ctypes.byref(iters), ctypes.byref(relres), h))
         rep.iterations, rep.relres = iters.value, relres.value
         rep.seconds["fit"] = time.perf_counter() - t0
-        del K, B, ws, V, C
+        del K, B, ws, V, C, bws
         if norms is None and self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
         self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d

@@ -49,6 +49,32 @@ def test_spotrf_fp32_factor_vs_lapack(n):
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
+@pytest.mark.parametrize("n", [257, 700, 1500])
+def test_spotrf_bf16x3_factor_accuracy(n):
+    # Split-BF16 trailing updates: each update ~2^-16 relative, so the factor agrees with
+    # LAPACK's to ~1e-5 normwise on a well-conditioned matrix.
+    from scipy.linalg import lapack
+
+    from qml_amd import _lib
+
+    L = _lib.lib()
+    rng = np.random.default_rng(n + 11)
+    M = rng.standard_normal((n, n))
+    A = (M @ M.T / n + np.eye(n)).astype(np.float32)
+    Ad = torch.from_numpy(A.copy()).cuda()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    nb = L.qmlh_spotrf_bf16x3_workspace(n)
+    ws = torch.empty(nb // 8 + 1, dtype=torch.float64, device="cuda")
+    _lib.check(L.qmlh_spotrf_bf16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
+                                    ws.numel() * 8, None))
+    assert int(info.item()) == 0
+    Lg = np.tril(Ad.cpu().numpy().T.astype(np.float64))
+    Lref = np.tril(lapack.dpotrf(A.astype(np.float64), lower=1)[0])
+    err = np.linalg.norm(Lg - Lref) / np.linalg.norm(Lref)
+    assert 1e-9 < err <= 5e-5, err  # lower bound: the BF16 path really ran
+    assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
+
+
 def test_spotrf_info_matches_lapack():
     from scipy.linalg import lapack
 
@@ -167,9 +193,10 @@ def test_woodbury_preconditioner_is_exact_for_exact_factor():
         np.linalg.solve(A, y))
 
 
-@pytest.mark.parametrize("kernel,sigma,lam,n", [("gaussian", 4000.0, 1e-6, 3000),
-                                                ("laplacian", 724.0, 1e-8, 2000)])
-def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n):
+@pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "bf16x3"),
+                                                       ("gaussian", 4000.0, 1e-6, 3000, "fp32"),
+                                                       ("laplacian", 724.0, 1e-8, 2000, "fp32")])
+def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n, factor):
     from qml_amd.krr import KRR
 
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=n, n_test=300)
@@ -179,7 +206,7 @@ def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n):
     m64 = KRR(kernel, sigma, lam, "fp64")
     m64.fit(X, tr.labels)
     p64 = m64.predict(Xs)
-    mx = KRR(kernel, sigma, lam, "mixed", tol=1e-11)
+    mx = KRR(kernel, sigma, lam, "mixed", tol=1e-11, factor=factor)
     rep = mx.fit(X, tr.labels)
     pmx = mx.predict(Xs)
     _, pref = oracle.krr_fit_predict(X, tr.labels, Xs, kernel, sigma, lam)
