@@ -3,13 +3,21 @@
 
 Workload: 50,000 seeded synthetic training molecules (up to 29 atoms, Coulomb matrix size 29,
 435 FP64 features), Gaussian kernel sigma = 4000, lambda = 1e-6. One step is one full fit with
-the raw molecules (Z, R) already resident in HBM: Coulomb matrices on the GPU, FP64 squared
-norms, the symmetric 50k x 50k Gaussian kernel (FP64 MFMA), K + lambda I, the blocked Cholesky
-and the two triangular solves. After the timed steps, prediction requests of 1-512 molecules
-(log-uniform) are served one at a time (host molecules in, predictions out) and their latency
-percentiles are reported.
+the raw molecules (Z, R) already resident in HBM, timed for two solvers:
 
-Prints one JSON line. `value` is the fit time in seconds (lower is better).
+  fp64  -- Coulomb matrices on the GPU, squared norms, the symmetric 50k x 50k Gaussian kernel
+           (FP64 MFMA), K + lambda I, blocked FP64 Cholesky, two triangular solves;
+  mixed -- the same kernel matrix in FP64 for residuals plus its deflated remainder in FP32,
+           a low-precision Cholesky of the remainder, a Woodbury correction for the low-rank
+           part, FP64 preconditioned CG to |y - K alpha| <= 1e-10 |y|.
+
+The mixed fit's error is its prediction difference from the FP64 fit on 2048 held-out
+molecules, over the label standard deviation. Then prediction requests of 1-512 molecules
+(log-uniform) are served one at a time (host molecules in, predictions out), and a Poisson
+request stream runs through the continuous-batching server for each fitted model; latency
+percentiles are reported for both.
+
+Prints one JSON line. `value` is the mixed fit time in seconds (lower is better).
 """
 
 from __future__ import annotations
@@ -39,6 +47,9 @@ def parse():
                    help="training molecules in the CPU-baseline sample (0 = skip)")
     p.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     p.add_argument("--tol", type=float, default=1e-10)
+    p.add_argument("--serve-rate", type=float, default=500.0,
+                   help="Poisson arrival rate (requests/s) of the serving-load phase")
+    p.add_argument("--serve-seconds", type=float, default=4.0)
     return p.parse_args()
 
 
@@ -92,7 +103,7 @@ def main():
     import numpy as np
     import torch
 
-    from qml_amd import _lib, synthetic
+    from qml_amd import _lib, serve, synthetic
     from qml_amd.device import ragged_to_device
     from qml_amd.krr import KRR
     from qml_amd.representations import cm_length, generate_coulomb_matrices_device
@@ -176,6 +187,26 @@ def main():
     lat = np.array(lat)
     log(f"fit {fit_s:.3f}s; predict p50 {np.percentile(lat, 50):.2f} ms")
 
+    # Serving load: Poisson arrivals into the continuous-batching server, per model.
+    serve_out = {}
+    for tag, mdl in (("mixed", modelmx), ("fp64", model64)):
+        if mdl is None or args.serve_seconds <= 0:
+            continue
+        with serve.KRRServer(mdl, size, max_molecules=2048, max_inflight=3) as srv:
+            reqs = serve.poisson_load(srv, te, args.serve_rate, args.serve_seconds,
+                                      np.random.default_rng(9), max_mol=512, size=size)
+            for r in reqs:
+                r.wait(120)
+            sl = np.array([r.latency for r in reqs]) * 1e3
+            span = max(r.t_done for r in reqs) - min(r.t_arrival for r in reqs)
+            serve_out[tag] = {
+                "p50_ms": float(np.percentile(sl, 50)), "p99_ms": float(np.percentile(sl, 99)),
+                "requests": len(reqs), "batches": srv.batches,
+                "molecules_per_s": float(sum(r.nmol for r in reqs) / span),
+                "rate_req_per_s": args.serve_rate}
+        log(f"serve {tag}: p50 {serve_out[tag]['p50_ms']:.2f} ms "
+            f"p99 {serve_out[tag]['p99_ms']:.2f} ms over {len(reqs)} requests")
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"cpu baseline sample N={args.cpu_sample} ...")
@@ -205,6 +236,7 @@ def main():
         "predict": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                     "requests": int(len(lat)), "molecules_per_s": float(molecules / (lat.sum() / 1e3)),
                     "mode": "sequential requests, host molecules in, predictions out"},
+        "serve": serve_out or None,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),

@@ -136,5 +136,15 @@ def stream_handle(stream=None) -> int:
     return s.cuda_stream
 
 
+def host_int(t, stream=None) -> int:
+    """Read a one-element device tensor after waiting for ``stream`` with the GIL released
+    (a bare ``.item()`` holds the GIL for the whole wait, stalling other host threads)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    s.synchronize()
+    return int(t.item())
+
+
 def ptr(t) -> int:
     return t.data_ptr() if t is not None else 0

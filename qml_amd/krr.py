@@ -103,7 +103,7 @@ class KRR:
         info = potrf_device(K, n, stream)
         alpha = torch.as_tensor(y, dtype=torch.float64).to(Xd.device).clone()
         potrs_device(K, n, alpha, stream)
-        rep.info = int(info.item())
+        rep.info = _lib.host_int(info, stream)
         rep.seconds["fit"] = time.perf_counter() - t0
         del K
         if rep.info != 0:
@@ -158,7 +158,7 @@ class KRR:
                                                 bws.numel() * 8, h))
             else:
                 _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
-            rep.info = int(info.item())
+            rep.info = _lib.host_int(info, stream)
             rep.shift = shift
             if rep.info == 0:
                 break
@@ -169,7 +169,7 @@ class KRR:
         if r:
             _lib.check(L.qmlh_woodbury_setup(_lib.ptr(B), n, n, _lib.ptr(V), n, r, _lib.ptr(C), r,
                                              _lib.ptr(info), h))
-            if int(info.item()) != 0:
+            if _lib.host_int(info, stream) != 0:
                 raise CholeskyError("Woodbury capacitance matrix is not positive definite")
         alpha = torch.zeros(n, dtype=torch.float64, device=dev)
         yd = torch.as_tensor(y, dtype=torch.float64).to(dev).contiguous()
