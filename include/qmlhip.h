@@ -186,6 +186,54 @@ int qmlh_krr_predict(int32_t kind, const double *Xq, int64_t m, int64_t ldq,
                      double *y, void *ws, size_t ws_bytes, qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
+ * Batched KRR fits: many independent problems of uneven size -- a learning curve over
+ * training-set sizes, a (sigma, lambda) grid -- fitted together. Each problem is the plain
+ * FP64 fit the reference's users write per problem (qml/kernels.py:33-87 laplacian_kernel /
+ * gaussian_kernel on rows of a representation array, K[diag] += lambda, qml/math.py:61-100
+ * cho_solve); the reference has no batched entry point, this replaces their loop.
+ *
+ * Problem p trains on rows x_row0 .. x_row0 + n - 1 of the shared row-major pool X (x_rows
+ * rows, ldx >= d), reads its labels y[y_offset .. y_offset + n) and writes alpha[alpha_offset ..
+ * alpha_offset + n) (alpha ranges must not overlap); info[p] follows LAPACK (0, or the first
+ * failing column + 1). norms (squared row norms of X, qmlh_row_sqnorms) are needed when any
+ * problem is Gaussian. nb (64, 128 or 256) is the Cholesky block size.
+ *
+ * qmlh_krr_batched_create plans the launches and uploads the job tables into ws (a blocking
+ * copy; ws 256-byte aligned, qmlh_krr_batched_workspace bytes; it also holds the factors).
+ * qmlh_krr_batched_run enqueues the whole fit (kernel launches only, no host synchronisation,
+ * no allocation); qmlh_krr_batched_capture records that sequence into a hipGraph on a
+ * non-default stream, and qmlh_krr_batched_launch replays it as one graph launch. Every run
+ * recomputes everything from X and y. qmlh_krr_batched_factor gives the element offset into
+ * ws and the leading dimension of problem p's lower Cholesky factor.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct qmlh_problem {
+    int64_t n;
+    int64_t x_row0;
+    int64_t y_offset;
+    int64_t alpha_offset;
+    double sigma;
+    double lambda;
+    int32_t kind; /* QMLH_PAIR_GAUSSIAN or QMLH_PAIR_LAPLACIAN */
+    int32_t reserved;
+} qmlh_problem;
+
+typedef struct qmlh_batched_plan qmlh_batched_plan;
+
+size_t qmlh_krr_batched_workspace(const qmlh_problem *probs, int32_t nprob, int32_t nb);
+int qmlh_krr_batched_create(const qmlh_problem *probs, int32_t nprob, int32_t nb,
+                            const double *X, int64_t x_rows, int64_t ldx, int64_t d,
+                            const double *norms, const double *y, int64_t y_len, double *alpha,
+                            int64_t alpha_len, int32_t *info, void *ws, size_t ws_bytes,
+                            qmlh_batched_plan **plan);
+int qmlh_krr_batched_run(qmlh_batched_plan *plan, qmlh_stream_t stream);
+int qmlh_krr_batched_capture(qmlh_batched_plan *plan, qmlh_stream_t stream);
+int qmlh_krr_batched_launch(qmlh_batched_plan *plan, qmlh_stream_t stream);
+int64_t qmlh_krr_batched_launches(const qmlh_batched_plan *plan);
+int qmlh_krr_batched_factor(const qmlh_batched_plan *plan, int32_t p, int64_t *offset,
+                            int64_t *ld);
+void qmlh_krr_batched_destroy(qmlh_batched_plan *plan);
+
+/* ---------------------------------------------------------------------------------------- *
  * Measurement hooks (no counterpart in the reference; used by bench.py). When enabled, each
  * heavy launch is bracketed by HIP events on its stream and its algorithmic work is added to
  * its class: flops for the MFMA classes and the Cholesky, pair-features for the VALU pairwise

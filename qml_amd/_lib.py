@@ -35,6 +35,15 @@ _i32 = ctypes.c_int32
 _f64 = ctypes.c_double
 _sz = ctypes.c_size_t
 
+class Problem(ctypes.Structure):
+    """struct qmlh_problem."""
+
+    _fields_ = [("n", ctypes.c_int64), ("x_row0", ctypes.c_int64), ("y_offset", ctypes.c_int64),
+                ("alpha_offset", ctypes.c_int64), ("sigma", ctypes.c_double),
+                ("lambda_", ctypes.c_double), ("kind", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); mirrors include/qmlhip.h
 _SIGNATURES = {
     "qmlh_version": (ctypes.c_int, []),
@@ -65,6 +74,17 @@ _SIGNATURES = {
     "qmlh_predict_workspace": (_sz, [_i64, _i64]),
     "qmlh_krr_predict": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp,
                                         _i64, _f64, _vp, _vp, _sz, _vp]),
+    "qmlh_krr_batched_workspace": (_sz, [_vp, _i32, _i32]),
+    "qmlh_krr_batched_create": (ctypes.c_int, [_vp, _i32, _i32, _vp, _i64, _i64, _i64, _vp, _vp,
+                                               _i64, _vp, _i64, _vp, _vp, _sz,
+                                               ctypes.POINTER(_vp)]),
+    "qmlh_krr_batched_run": (ctypes.c_int, [_vp, _vp]),
+    "qmlh_krr_batched_capture": (ctypes.c_int, [_vp, _vp]),
+    "qmlh_krr_batched_launch": (ctypes.c_int, [_vp, _vp]),
+    "qmlh_krr_batched_launches": (_i64, [_vp]),
+    "qmlh_krr_batched_factor": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i64),
+                                               ctypes.POINTER(_i64)]),
+    "qmlh_krr_batched_destroy": (None, [_vp]),
     "qmlh_timer_enable": (ctypes.c_int, [_i32]),
     "qmlh_timer_read": (ctypes.c_int, [_i32, ctypes.POINTER(_f64), ctypes.POINTER(_i64),
                                        ctypes.POINTER(_f64)]),

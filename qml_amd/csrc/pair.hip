@@ -1,62 +1,13 @@
 // Pairwise kernels and distances on the FP64 VALU: Laplacian / Manhattan (sum |a-b|) and
 // Gaussian / L2 (sum (a-b)^2) with direct differences, as qml/fkernels.f90:340-400 and
-// qml/fdistance.f90:23-80 compute them.
-//
-// Tiling: a 256-lane workgroup owns a 128 x 128 output tile; each lane owns an 8 x 8 register
-// tile (rows 2*tx + 32*r + {0,1}, columns 2*ty + 32*c + {0,1}, tx = t % 16, ty = t / 16).
-// Features stream through LDS in chunks of 16, stored k-major so a lane fetches its two
-// adjacent rows (or columns) for one feature with a single 16-byte LDS read. Each lane keeps
-// the feature sum of every entry in feature order k = 0, 1, ..., d-1, so an entry equals the
-// reference's serial sum. Per pair-feature the L1 form costs two FP64 VALU ops (v_add_f64 for
-// a - b, v_add_f64 with the |.| source modifier), the squared form one sub and one FMA.
-#include "common.h"
+// qml/fdistance.f90:23-80 compute them. The 128 x 128 tile (pair_tile.h) keeps each entry's
+// feature sum in feature order, so an entry equals the reference's serial sum. Per
+// pair-feature the L1 form costs two FP64 VALU ops (v_add_f64 for a - b, v_add_f64 with the
+// |.| source modifier), the squared form one sub and one FMA.
+#include "pair_tile.h"
 #include "timer.h"
 
 namespace qmlh {
-
-static constexpr int PT_BM = 128;
-static constexpr int PT_BN = 128;
-static constexpr int PT_BK = 16;
-static constexpr int PT_LDS = PT_BM + 2;  // padded k-major row (doubles), keeps 16 B alignment
-static constexpr int PT_THREADS = 256;
-
-enum { OP_L1 = 0, OP_SQ = 1 };
-enum { EPI_IDENTITY = 0, EPI_SQRT = 1, EPI_EXP = 2 };
-
-template <int OP>
-__device__ __forceinline__ double pair_term(double acc, double a, double b) {
-    if (OP == OP_L1) {
-        return acc + fabs(a - b);
-    } else {
-        double t = a - b;
-        return fma(t, t, acc);
-    }
-}
-
-template <int EPI>
-__device__ __forceinline__ double pair_epilogue(double acc, double scale) {
-    if (EPI == EPI_EXP) return exp(scale * acc);
-    if (EPI == EPI_SQRT) return sqrt(acc);
-    return acc;
-}
-
-// Load a 128-row x 16-feature chunk of X (row-major, ld) into LDS as s[k][row].
-// Rows >= n and features >= d read as zero; zero features add exactly zero to either sum.
-__device__ __forceinline__ void pair_load_chunk(double (*s)[PT_LDS], const double* __restrict__ X,
-                                                int64_t ld, int64_t n, int64_t d, int64_t row0,
-                                                int64_t k0, int t) {
-    const int k = t % PT_BK;
-    const int rbase = t / PT_BK;  // 0..15
-    const int64_t kk = k0 + k;
-#pragma unroll
-    for (int q = 0; q < PT_BM / 16; ++q) {
-        int r = rbase + 16 * q;
-        int64_t row = row0 + r;
-        double v = 0.0;
-        if (row < n && kk < d) v = X[row * ld + kk];
-        s[k][r] = v;
-    }
-}
 
 template <int OP, int EPI>
 __global__ __launch_bounds__(PT_THREADS, 2) void pair_tile_kernel(
@@ -81,40 +32,11 @@ __global__ __launch_bounds__(PT_THREADS, 2) void pair_tile_kernel(
     }
     const int64_t row0 = bi * PT_BM;
     const int64_t col0 = bj * PT_BN;
-
     const int t = threadIdx.x;
     const int tx = t % 16;
     const int ty = t / 16;
-
     double acc[8][8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
-
-    for (int64_t k0 = 0; k0 < d; k0 += PT_BK) {
-        pair_load_chunk(sA, A, lda, na, d, row0, k0, t);
-        pair_load_chunk(sB, B, ldb, nb, d, col0, k0, t);
-        __syncthreads();
-#pragma unroll 4
-        for (int k = 0; k < PT_BK; ++k) {
-            double a[8], b[8];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double2 va = *reinterpret_cast<const double2*>(&sA[k][2 * tx + 32 * r]);
-                a[2 * r] = va.x;
-                a[2 * r + 1] = va.y;
-                const double2 vb = *reinterpret_cast<const double2*>(&sB[k][2 * ty + 32 * r]);
-                b[2 * r] = vb.x;
-                b[2 * r + 1] = vb.y;
-            }
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[r][c] = pair_term<OP>(acc[r][c], a[r], b[c]);
-        }
-        __syncthreads();
-    }
+    pair_tile_accumulate<OP>(acc, sA, sB, A, lda, na, B, ldb, nb, d, row0, col0);
 
     const bool mirror = symmetric && (bi != bj);
 #pragma unroll

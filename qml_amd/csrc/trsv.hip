@@ -56,26 +56,22 @@ __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L,
     }
 }
 
+// Solve block b of one system: wait for the blocks it depends on (flags[j] of the earlier
+// blocks in solve order), accumulate their contributions, solve the diagonal block, publish.
+// `err` is the shared error word. All 256 lanes call this.
 template <typename T, bool FWD>
-__global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
-                                                            int64_t n, const double* __restrict__ rhs,
-                                                            double* __restrict__ out,
-                                                            int32_t* __restrict__ sync) {
+__device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl, int64_t n,
+                                           const double* __restrict__ rhs,
+                                           double* __restrict__ out, int32_t* __restrict__ flags,
+                                           int32_t* __restrict__ err, int64_t b) {
     __shared__ double sL[TS_B][TS_B + 1];
     __shared__ double sInv[TS_B];
     __shared__ double red[TS_B][TS_B + 1];
     __shared__ double sv[TS_B];
-    __shared__ int s_block, s_fail;
+    __shared__ int s_fail;
     const int t = threadIdx.x;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
-    int32_t* flags = sync + 2;
-    if (t == 0) {
-        const int ticket = atomicAdd(&sync[0], 1);
-        s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
-        s_fail = 0;
-    }
-    __syncthreads();
-    const int64_t b = s_block;
+    if (t == 0) s_fail = 0;
     const int64_t r0 = b * TS_B;
     const int w = (int)(n - r0 < TS_B ? n - r0 : TS_B);
     const int lane = t & 63, grp = t >> 6;  // 4 groups of 16 columns
@@ -85,6 +81,7 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
         const int i = idx % TS_B, jj = idx / TS_B;
         if (i < w && jj < w && i >= jj) sL[i][jj] = (double)L[(r0 + i) + (r0 + jj) * ldl];
     }
+    __syncthreads();  // s_fail initialised before any lane may set it
 
     // Forward: thread (row lane, columns 16 grp ..) keeps one partial sum.
     // Backward: thread (row lane of block j, columns 16 grp .. of block b) keeps 16.
@@ -99,7 +96,7 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
     for (int64_t q = 0; q < nj; ++q) {
         const int64_t j = FWD ? q : nblocks - 1 - q;
         if (q + 1 < nj) load_tile16<T, FWD>(ln, L, ldl, n, r0, w, FWD ? q + 1 : j - 1, lane, grp);
-        if (ok) ok = wait_flag(&flags[j], &sync[1]);
+        if (ok) ok = wait_flag(&flags[j], err);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int64_t c0 = j * TS_B;
         const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
@@ -167,6 +164,21 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
+                                                            int64_t n, const double* __restrict__ rhs,
+                                                            double* __restrict__ out,
+                                                            int32_t* __restrict__ sync) {
+    __shared__ int s_block;
+    const int64_t nblocks = (n + TS_B - 1) / TS_B;
+    if (threadIdx.x == 0) {
+        const int ticket = atomicAdd(&sync[0], 1);
+        s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
+    }
+    __syncthreads();
+    trsv_block<T, FWD>(L, ldl, n, rhs, out, sync + 2, &sync[1], s_block);
 }
 
 size_t trsv_sync_ints(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B + 2); }
