@@ -31,7 +31,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_MFMA_TFLOPS = 78.6
+# MI355X published dense MFMA peaks, TFLOP/s.
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+SYRK_KERNEL = {
+    "fp64": ("gemm_update_kernel<double, lower> (Cholesky trailing update, FP64 MFMA)", "fp64"),
+    "fp32": ("gemm_update_kernel<float, lower> (FP32 MFMA trailing update)", "fp32"),
+    "bf16x3": ("syrk_bf16x3_kernel (trailing update as 3 BF16 MFMA products, FP32 accumulate)",
+               "bf16"),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_krr_serve_50k.json")
 
 
 def parse():
@@ -50,6 +58,10 @@ def parse():
     p.add_argument("--serve-rate", type=float, default=500.0,
                    help="Poisson arrival rate (requests/s) of the serving-load phase")
     p.add_argument("--serve-seconds", type=float, default=4.0)
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="run a device copy of this many MiB first (HBM counter calibration)")
+    p.add_argument("--sweep", type=int, default=64,
+                   help="problems in the batched learning-curve sweep (0 = skip)")
     return p.parse_args()
 
 
@@ -94,6 +106,93 @@ def cpu_baseline(n_full, size, sigma, lam, n_sample):
     }
 
 
+def roofline(timers, steps, factor):
+    """Roofline of the Cholesky trailing update (the dominant kernel of a fit): algorithmic
+    flops m (m + 1) kb per launch over the launches' HIP-event time, against the published
+    MFMA peak of the arithmetic the kernel runs. traffic: HBM bytes per launch from the
+    committed counter summary, when one exists for this kernel."""
+    name, prec = SYRK_KERNEL[factor]
+    ms, launches, flops = timers["syrk"]
+    achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f).get(factor)
+        if t:
+            traffic = t["hbm_bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
+    return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[prec],
+            "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic,
+            "algorithmic_flops_per_launch": flops / max(launches, 1),
+            "launches_per_step": launches / steps, "ms_per_step": ms / steps}
+
+
+def run_sweep(count, log):
+    """64 Laplacian KRR fits, n log-uniform in [500, 8000], (sigma, lambda) grid, size-23
+    Coulomb matrices: one graph launch vs the same fits solved one after another."""
+    import numpy as np
+    import torch
+
+    from qml_amd import synthetic
+    from qml_amd.representations import cm_length, generate_coulomb_matrices
+    from qml_amd.sweep import BatchedFit, learning_curve_grid, sequential_fit
+
+    probs = learning_curve_grid(count=count)
+    nmax = max(p.n for p in probs)
+    pool, _, size = synthetic.workload("krr_sweep_64", n_train=nmax, n_test=0)
+    Xd = generate_coulomb_matrices(*pool.ragged(), size=size)
+    d = cm_length(size)
+    fit = BatchedFit(probs, Xd, d, pool.labels)  # nb from the autotuner
+    s = fit.stream
+    fit.run()
+    reps = 5
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fit.run(sync=False)
+    e1.record(s)
+    e1.synchronize()
+    graph_ms = e0.elapsed_time(e1) / reps
+    e0.record(s)
+    fit.run_direct(sync=False)
+    e1.record(s)
+    e1.synchronize()
+    direct_ms = e0.elapsed_time(e1)
+    info = fit.info()
+    alphas = [a.clone() for a in fit.alphas()]
+    with torch.cuda.stream(s):
+        sequential_fit(probs, Xd, d, pool.labels, stream=s)  # warm
+        s.synchronize()
+        e0.record(s)
+        seq = sequential_fit(probs, Xd, d, pool.labels, stream=s)
+        e1.record(s)
+        e1.synchronize()
+    seq_ms = e0.elapsed_time(e1)
+    # Problems whose FP64 Cholesky breaks down (info > 0: numerically singular K + lambda I,
+    # e.g. sigma = 2000 with lambda = 1e-10 at large n) are reported, not compared.
+    ok = [i for i in range(len(probs)) if info[i] == 0 and bool(torch.isfinite(seq[i]).all())]
+    diff = max(float((alphas[i] - seq[i]).abs().max() / seq[i].abs().max()) for i in ok)
+    flops = sum(p.n ** 3 / 3.0 for p in probs)
+    pairs = sum(p.n * (p.n + 1) / 2.0 * d for p in probs)
+    log(f"sweep: graph {graph_ms:.2f} ms, direct {direct_ms:.2f} ms, sequential {seq_ms:.2f} ms "
+        f"(nb={fit.nb}, {fit.launches} launches)")
+    out = {"workload": f"{count} Laplacian KRR fits, n log-uniform [500, 8000], size-23 CM "
+                       "(d=276), sigma x lambda grid, shared pool in HBM",
+           "problems": count, "n_min": min(p.n for p in probs), "n_max": nmax,
+           "nb": fit.nb, "launches_in_graph": fit.launches,
+           "graph_ms": graph_ms, "direct_launches_ms": direct_ms, "sequential_ms": seq_ms,
+           "speedup_vs_sequential": seq_ms / graph_ms,
+           "cholesky_flops": flops, "build_pair_features": pairs,
+           "effective_cholesky_tflops": flops / (graph_ms * 1e-3) / 1e12,
+           "failed_problems": [{"n": probs[i].n, "sigma": probs[i].sigma, "lambda": probs[i].lam,
+                                "info": int(info[i])} for i in np.flatnonzero(info)],
+           "max_alpha_rel_diff_vs_sequential": diff,
+           "compared_problems": len(ok)}
+    fit.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -110,6 +209,15 @@ def main():
 
     torch.cuda.set_device(0)
     _lib.lib()
+    if args.copy_mib > 0:
+        nel = args.copy_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_mib} MiB device copies")
+        del src, dst
     N = args.n_train
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)
     d = cm_length(size)
@@ -161,9 +269,9 @@ def main():
     fit_s = fitmx_s if fitmx_s is not None else fit64_s
     model = modelmx if modelmx is not None else model64
 
-    # Dominant kernel: the FP64 MFMA trailing update of the Cholesky.
-    syrk_ms, syrk_n, syrk_flops = timers["syrk"]
-    achieved = syrk_flops / (syrk_ms * 1e-3) / 1e12 if syrk_ms > 0 else 0.0
+    # Dominant kernel of each fit: the Cholesky trailing update.
+    roof64 = roofline(timers, args.steps, "fp64")
+    roofmx = roofline(timers_mx, args.steps, modelmx.factor) if timers_mx else None
     phases = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
                   "rate": (v[2] / (v[0] * 1e-3) if v[0] > 0 else 0.0)}
               for k, v in timers.items() if v[1] > 0}
@@ -184,7 +292,7 @@ def main():
         tb = time.perf_counter()
         lat.append((tb - ta) * 1e3)
         molecules += k
-    lat = np.array(lat)
+    lat = np.array(lat) if lat else np.full(1, np.nan)
     log(f"fit {fit_s:.3f}s; predict p50 {np.percentile(lat, 50):.2f} ms")
 
     # Serving load: Poisson arrivals into the continuous-batching server, per model.
@@ -206,6 +314,8 @@ def main():
                 "rate_req_per_s": args.serve_rate}
         log(f"serve {tag}: p50 {serve_out[tag]['p50_ms']:.2f} ms "
             f"p99 {serve_out[tag]['p99_ms']:.2f} ms over {len(reqs)} requests")
+
+    sweep_out = run_sweep(args.sweep, log) if args.sweep > 0 else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -234,16 +344,16 @@ def main():
                   "speedup_mixed_vs_fp64": (fit64_s / fitmx_s) if fitmx_s else None},
         "mixed_error": mixed_err if fitmx_s is not None else None,
         "predict": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                    "requests": int(len(lat)), "molecules_per_s": float(molecules / (lat.sum() / 1e3)),
+                    "requests": int(args.requests),
+                    "molecules_per_s": float(molecules / (lat.sum() / 1e3)) if molecules else None,
                     "mode": "sequential requests, host molecules in, predictions out"},
         "serve": serve_out or None,
+        "sweep": sweep_out,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),
-        "roofline": {"kernel": "gemm_update_kernel<lower> (Cholesky trailing update, FP64 MFMA)",
-                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS,
-                     "traffic": None, "launches": syrk_n / args.steps},
+        "roofline": roofmx or roof64,
+        "roofline_fp64": roof64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

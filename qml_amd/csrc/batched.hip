@@ -252,30 +252,17 @@ __global__ __launch_bounds__(256, 2) void bupdate_kernel(const BUpdateJob* __res
     if (bj * BT > bi * BT + BT - 1) return;  // tile entirely above the diagonal
     Operand<double, LAYOUT_COL> opA{jb.A, jb.ld, jb.M, jb.Kd};
     Operand<double, LAYOUT_COL> opB{jb.A, jb.ld, jb.N, jb.Kd};
-    WaveTile<double> t;
-    wave_tile_zero(t);
-    wg_tile_mma_lds(t, lds, opB, opA, bj * BT, bi * BT, jb.Kd);
-
     const int w = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
     const int64_t col0 = bj * BT + 64 * (w & 1);
     const int64_t row0 = bi * BT + 64 * (w >> 1);
-    if (row0 >= jb.M || col0 >= jb.N || col0 > row0 + 63) return;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t i = row0 + 16 * b + (lane & 15);
-            if (i >= jb.M) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + tile_row<double>(a, e);
-                if (j >= jb.N || j > i) continue;
-                double* c = jb.C + i + j * jb.ld;
-                *c = *c - t.acc[a][b][e];
-            }
-        }
-    }
+    const bool live = row0 < jb.M && col0 < jb.N && col0 <= row0 + 63;
+    const bool interior = live && row0 + 64 <= jb.M && col0 + 64 <= jb.N && col0 + 63 <= row0;
+    WaveTile<double> t;
+    wave_tile_load_c<double, true>(t, jb.C, jb.ld, row0, col0, jb.M, jb.N, live, interior);
+    wg_tile_mma_lds<double, LAYOUT_COL, LAYOUT_COL, true>(t, lds, opB, opA, bj * BT, bi * BT,
+                                                          jb.Kd);
+    if (!live) return;
+    wave_tile_store_c<double, true>(t, jb.C, jb.ld, row0, col0, jb.M, jb.N, interior);
 }
 
 // ---- solve ---------------------------------------------------------------------------------

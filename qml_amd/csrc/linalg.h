@@ -7,15 +7,16 @@
 namespace qmlh {
 
 // Lower Cholesky in place (column-major), storage T in {double, float}; LAPACK info rules.
-// For float, a non-null bf16_ws (syrk_bf16_workspace(n, 256) bytes) selects split-BF16
+// For float, a non-null bf16_ws (qmlh_spotrf_bf16x3_workspace(n) bytes) selects split-BF16
 // trailing updates instead of FP32 MFMA ones.
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
                void* bf16_ws = nullptr);
 
 size_t syrk_bf16_workspace(int64_t n, int64_t kb);
-int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float alpha, float* C,
-                int64_t ldc, const int32_t* info, void* ws, hipStream_t s);
+// C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three BF16 products.
+int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
+                const int32_t* info, void* ws, hipStream_t s);
 
 // Synchronisation words the single-launch triangular solve needs for order n.
 size_t trsv_sync_ints(int64_t n);

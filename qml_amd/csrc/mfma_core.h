@@ -181,10 +181,11 @@ __device__ __forceinline__ void panel_store(T* __restrict__ panel, const PanelRe
     }
 }
 
-// t += F(f0 .. f0+127, :) * S(s0 .. s0+127, :)^T restricted to this wave's 64 x 64 block:
+// t += F(f0 .. f0+127, :) * S(s0 .. s0+127, :)^T restricted to this wave's 64 x 64 block
+// (t -= ... with SUB: the first operand's fragments are negated, which is exact):
 // wave w owns first-operand rows f0 + 64 (w & 1) and second-operand rows s0 + 64 (w >> 1).
 // `lds` holds WG_LDS_ELEMS elements. All 256 lanes must call this (it has barriers).
-template <typename T, int LF, int LS>
+template <typename T, int LF, int LS, bool SUB = false>
 __device__ __forceinline__ void wg_tile_mma_lds(WaveTile<T>& t, T* __restrict__ lds,
                                                 const Operand<T, LF>& F,
                                                 const Operand<T, LS>& S, int64_t f0, int64_t s0,
@@ -221,6 +222,10 @@ __device__ __forceinline__ void wg_tile_mma_lds(WaveTile<T>& t, T* __restrict__ 
             for (int q = 0; q < 4; ++q) {
                 fa[q] = *reinterpret_cast<const P2*>(Fp + lds_index(fr + 16 * q + r, 4 * g + 2 * h));
                 fb[q] = *reinterpret_cast<const P2*>(Sp + lds_index(sr + 16 * q + r, 4 * g + 2 * h));
+                if (SUB) {
+                    fa[q].x = -fa[q].x;
+                    fa[q].y = -fa[q].y;
+                }
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
@@ -239,6 +244,71 @@ __device__ __forceinline__ void wg_tile_mma_lds(WaveTile<T>& t, T* __restrict__ 
         __syncthreads();
         stage ^= 1;
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Accumulators <-> a column-major C block: element e of block (a, b) of this lane is
+// C(row0 + 16 b + (lane & 15), col0 + tile_row(a, e)) (first operand = output columns).
+// `live`: the wave's block has entries to update; `interior`: every entry of its 64 x 64 block
+// is inside the M x N matrix (and on or below the diagonal when LOWER), so no masks are needed
+// and all loads are issued back to back. Masked entries load as zero and are not stored.
+// ------------------------------------------------------------------------------------------
+template <typename T, bool LOWER>
+__device__ __forceinline__ void wave_tile_load_c(WaveTile<T>& t, const T* __restrict__ C,
+                                                 int64_t ldc, int64_t row0, int64_t col0,
+                                                 int64_t M, int64_t N, bool live, bool interior) {
+    const int lane = threadIdx.x & 63;
+    if (interior) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    t.acc[a][b][e] = C[(row0 + 16 * b + (lane & 15)) + (col0 + tile_row<T>(a, e)) * ldc];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = row0 + 16 * b + (lane & 15);
+                const int64_t j = col0 + tile_row<T>(a, e);
+                const bool ok = live && i < M && j < N && (!LOWER || j <= i);
+                t.acc[a][b][e] = ok ? C[i + j * ldc] : T(0);
+            }
+}
+
+template <typename T, bool LOWER>
+__device__ __forceinline__ void wave_tile_store_c(const WaveTile<T>& t, T* __restrict__ C,
+                                                  int64_t ldc, int64_t row0, int64_t col0,
+                                                  int64_t M, int64_t N, bool interior) {
+    const int lane = threadIdx.x & 63;
+    if (interior) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    C[(row0 + 16 * b + (lane & 15)) + (col0 + tile_row<T>(a, e)) * ldc] = t.acc[a][b][e];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + (lane & 15);
+            if (i >= M) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + tile_row<T>(a, e);
+                if (j >= N || (LOWER && j > i)) continue;
+                C[i + j * ldc] = t.acc[a][b][e];
+            }
+        }
 }
 
 // ------------------------------------------------------------------------------------------

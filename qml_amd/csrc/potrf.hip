@@ -24,19 +24,27 @@
 namespace qmlh {
 
 static constexpr int PF_NB = 256;
+// Block size when the trailing updates run as split-BF16 products: the update of C costs a
+// fixed read and write of the FP32 trailing matrix per step, so a deeper update pays off.
+#ifndef QMLH_PF_NB_BF16
+#define QMLH_PF_NB_BF16 512
+#endif
+static constexpr int PF_NB_BF16 = QMLH_PF_NB_BF16;
 static constexpr int PF_IB = 64;
 static constexpr int UP_TILE = WG_T;
 
 // ------------------------------------------------------------------------------------------
-// C(i, j) += alpha * sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL:
-// B(j, k) at B[j + k ldb]) or row-major (LB = ROW: B(j, k) at B[j ldb + k]). LOWER: only tiles
-// with bi >= bj run and only i >= j is written (a SYRK when A == B).
+// C(i, j) -= sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL: B(j, k) at
+// B[j + k ldb]) or row-major (LB = ROW: B(j, k) at B[j ldb + k]). LOWER: only tiles with
+// bi >= bj run and only i >= j is written (a SYRK when A == B). The accumulators start from C
+// (loaded while the first operand chunks are in flight) and the products are subtracted, so
+// C is read once and written once.
 // ------------------------------------------------------------------------------------------
 template <typename T, bool LOWER, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
-    int64_t N, int64_t Kd, T alpha, T* __restrict__ C, int64_t ldc,
-    const int32_t* __restrict__ info, int64_t tiles_m) {
+    int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, const int32_t* __restrict__ info,
+    int64_t tiles_m) {
     __shared__ __attribute__((aligned(16))) T lds[WG_LDS_ELEMS];
     if (info && *info) return;
     int64_t bi, bj;
@@ -55,50 +63,39 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     // lanes of one accumulator element run down 16 consecutive rows of column-major C.
     Operand<T, LAYOUT_COL> opA{A, lda, M, Kd};
     Operand<T, LB> opB{B, ldb, N, Kd};
-    WaveTile<T> t;
-    wave_tile_zero(t);
-    wg_tile_mma_lds(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
-
     const int w = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
     const int64_t col0 = bj * UP_TILE + 64 * (w & 1);
     const int64_t row0 = bi * UP_TILE + 64 * (w >> 1);
-    if (row0 >= M || col0 >= N) return;
-    if (LOWER && col0 > row0 + 63) return;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t i = row0 + 16 * b + (lane & 15);
-            if (i >= M) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + tile_row<T>(a, e);
-                if (j >= N || (LOWER && j > i)) continue;
-                T* c = C + i + j * ldc;
-                *c = *c + alpha * t.acc[a][b][e];
-            }
-        }
-    }
+    const bool live = row0 < M && col0 < N && !(LOWER && col0 > row0 + 63);
+    const bool interior = live && row0 + 64 <= M && col0 + 64 <= N && (!LOWER || col0 + 63 <= row0);
+    WaveTile<T> t;
+    wave_tile_load_c<T, LOWER>(t, C, ldc, row0, col0, M, N, live, interior);
+    wg_tile_mma_lds<T, LB, LAYOUT_COL, true>(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
+    if (!live) return;
+    wave_tile_store_c<T, LOWER>(t, C, ldc, row0, col0, M, N, interior);
 }
 
+// C -= A B^T (lower: the lower triangle of C -= A A^T-style SYRK with B = A). timer_class: the
+// measurement class the launch is counted in; by default trailing (lower) updates are SYRK and
+// the rest GEMM.
 template <typename T, int LB = LAYOUT_COL>
 static int launch_update(bool lower, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
-                         int64_t N, int64_t Kd, double alpha, T* C, int64_t ldc,
-                         const int32_t* info, hipStream_t s) {
+                         int64_t N, int64_t Kd, T* C, int64_t ldc, const int32_t* info,
+                         hipStream_t s, int timer_class = -1) {
     if (M <= 0 || N <= 0 || Kd <= 0) return QMLH_OK;
     int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
-    TimerScope ts(lower ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM,
+    if (timer_class < 0) timer_class = lower ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
+    TimerScope ts(timer_class,
                   lower ? (double)M * (double)(M + 1) * (double)Kd
                         : 2.0 * (double)M * (double)N * (double)Kd,
                   s);
     if (lower) {
         int64_t nt = tm * (tm + 1) / 2;
         hipLaunchKernelGGL((gemm_update_kernel<T, true, LB>), dim3((unsigned)nt), dim3(256), 0, s,
-                           A, lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
+                           A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
     } else {
         hipLaunchKernelGGL((gemm_update_kernel<T, false, LB>), dim3((unsigned)(tm * tn)),
-                           dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, (T)alpha, C, ldc, info, tm);
+                           dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
@@ -196,8 +193,7 @@ static int trsm_panel(T* B, int64_t ldb, int64_t m, const T* L, int64_t ldl, int
     int64_t h = round_up(w / 2, PF_IB);
     QMLH_TRY(trsm_panel(B, ldb, m, L, ldl, h, info, s));
     // B2 -= X1 L21^T
-    QMLH_TRY(launch_update<T>(false, B, ldb, L + h, ldl, m, w - h, h, -1.0, B + h * ldb, ldb,
-                              info, s));
+    QMLH_TRY(launch_update<T>(false, B, ldb, L + h, ldl, m, w - h, h, B + h * ldb, ldb, info, s));
     return trsm_panel(B + h * ldb, ldb, m, L + h + h * ldl, ldl, w - h, info, s);
 }
 
@@ -214,8 +210,8 @@ static int factor_diag(T* A, int64_t lda, int64_t kb, int64_t col_base, int32_t*
         int64_t rest = kb - kk - w;
         if (rest > 0) {
             QMLH_TRY(launch_trsm_base(Akk + w, lda, rest, (const T*)Akk, lda, w, info, s));
-            QMLH_TRY(launch_update<T>(true, Akk + w, lda, Akk + w, lda, rest, rest, w, -1.0,
-                                      Akk + w + w * lda, lda, info, s));
+            QMLH_TRY(launch_update<T>(true, Akk + w, lda, Akk + w, lda, rest, rest, w,
+                                      Akk + w + w * lda, lda, info, s, QMLH_TIMER_GEMM));
         }
     }
     return QMLH_OK;
@@ -233,21 +229,23 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     if (n == 0) return QMLH_OK;
     if (!A) return set_error(QMLH_ERR_ARG, "potrf: A is null");
     TimerScope ts(QMLH_TIMER_POTRF, (double)n * (double)n * (double)n / 3.0, s);
-    for (int64_t k = 0; k < n; k += PF_NB) {
-        int64_t kb = n - k < PF_NB ? n - k : PF_NB;
+    const bool bf16 = std::is_same<T, float>::value && bf16_ws;
+    const int64_t nb = bf16 ? PF_NB_BF16 : PF_NB;
+    for (int64_t k = 0; k < n; k += nb) {
+        int64_t kb = n - k < nb ? n - k : nb;
         T* Akk = A + k + k * lda;
         QMLH_TRY(factor_diag(Akk, lda, kb, k, info, s));
         int64_t m = n - k - kb;
         if (m > 0) {
             T* P = Akk + kb;
             QMLH_TRY(trsm_panel(P, lda, m, (const T*)Akk, lda, kb, info, s));
-            if (std::is_same<T, float>::value && bf16_ws) {
-                QMLH_TRY(syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb, -1.0f,
+            if (bf16) {
+                QMLH_TRY(syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb,
                                      reinterpret_cast<float*>(P + kb * lda), lda, info, bf16_ws,
                                      s));
             } else {
-                QMLH_TRY(launch_update<T>(true, P, lda, P, lda, m, m, kb, -1.0, P + kb * lda,
-                                          lda, info, s));
+                QMLH_TRY(launch_update<T>(true, P, lda, P, lda, m, m, kb, P + kb * lda, lda,
+                                          info, s));
             }
         }
     }
@@ -455,8 +453,7 @@ int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64
         const int64_t rest = n - k - w;
         // V[k+w:, :] -= L[k+w:, k:k+w] * V[k:k+w, :]; B(j, c) = V[(k + c) + j ldv] (row layout).
         QMLH_TRY((launch_update<float, LAYOUT_ROW>(false, L + (k + w) + k * ldl, ldl, V + k, ldv,
-                                                   rest, nrhs, w, -1.0, V + (k + w), ldv, nullptr,
-                                                   s)));
+                                                   rest, nrhs, w, V + (k + w), ldv, nullptr, s)));
     }
     return QMLH_OK;
 }
@@ -489,7 +486,7 @@ extern "C" int qmlh_spotrf(float* A, int64_t n, int64_t lda, int32_t* info,
 }
 
 extern "C" size_t qmlh_spotrf_bf16x3_workspace(int64_t n) {
-    return n > 0 ? qmlh::syrk_bf16_workspace(n, qmlh::PF_NB) : 0;
+    return n > 0 ? qmlh::syrk_bf16_workspace(n, qmlh::PF_NB_BF16) : 0;
 }
 
 extern "C" int qmlh_spotrf_bf16x3(float* A, int64_t n, int64_t lda, int32_t* info, void* ws,

@@ -20,6 +20,9 @@
 namespace qmlh {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+// 16-byte staging registers. A native vector type: arrays of the HIP uint4 struct are copied
+// with memcpy and end up in scratch memory instead of registers.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 static constexpr int SB_T = 128;          // tile rows
 static constexpr int SB_KC = 32;          // features per chunk
@@ -47,7 +50,7 @@ __global__ void split_panel_kernel(const float* __restrict__ P, int64_t ldp, int
 }
 
 // Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) into an LDS panel.
-__device__ __forceinline__ void sb_load(uint4 (&r)[2], const __bf16* __restrict__ src,
+__device__ __forceinline__ void sb_load(u32x4_t (&r)[2], const __bf16* __restrict__ src,
                                         int64_t m_pad, int64_t row0, int64_t c) {
     const int t = threadIdx.x;
 #pragma unroll
@@ -55,16 +58,16 @@ __device__ __forceinline__ void sb_load(uint4 (&r)[2], const __bf16* __restrict_
         const int p = t + 256 * h;  // 0 .. 511
         const int q = p >> 7, row = p & 127;
         const __bf16* g = src + ((c * 4 + q) * m_pad + row0 + row) * 8;
-        r[h] = *reinterpret_cast<const uint4*>(g);
+        r[h] = *reinterpret_cast<const u32x4_t*>(g);
     }
 }
 
-__device__ __forceinline__ void sb_store(__bf16* __restrict__ panel, const uint4 (&r)[2]) {
+__device__ __forceinline__ void sb_store(__bf16* __restrict__ panel, const u32x4_t (&r)[2]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int p = t + 256 * h;
-        reinterpret_cast<uint4*>(panel)[p] = r[h];  // (q * 128 + row) * 8 elements
+        reinterpret_cast<u32x4_t*>(panel)[p] = r[h];  // (q * 128 + row) * 8 elements
     }
 }
 
@@ -72,10 +75,20 @@ __device__ __forceinline__ bf16x8_t sb_frag(const __bf16* __restrict__ panel, in
     return *reinterpret_cast<const bf16x8_t*>(panel + (g * SB_T + row) * 8);
 }
 
+// Flip the sign of 8 BF16 values (exact).
+__device__ __forceinline__ bf16x8_t sb_neg(bf16x8_t v) {
+    const u32x4_t u = __builtin_bit_cast(u32x4_t, v) ^ 0x80008000u;
+    return __builtin_bit_cast(bf16x8_t, u);
+}
+
+// C(i, j) -= sum_k P(i, k) P(j, k) for i >= j. The accumulators start from C -- its loads are
+// issued together with the first panel chunk, so their latency hides behind the prologue --
+// and the products are subtracted by negating the second operand's fragments; the epilogue
+// only stores.
 __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __restrict__ Ph,
                                                              const __bf16* __restrict__ Pl,
                                                              int64_t m, int64_t m_pad,
-                                                             int64_t kb_pad, float alpha,
+                                                             int64_t kb_pad,
                                                              float* __restrict__ C, int64_t ldc,
                                                              const int32_t* __restrict__ info) {
     // buffers x {row hi, row lo, col hi, col lo}
@@ -94,16 +107,38 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
     // Wave w: first operand (accumulator rows) = output columns cols0 + 64 (w & 1),
     // second operand = output rows rows0 + 64 (w >> 1).
     const int fr = 64 * (w & 1), sr = 64 * (w >> 1);
+    const int64_t col0 = cols0 + fr;
+    const int64_t row0 = rows0 + sr;
+    const bool live = row0 < m && col0 < m && col0 <= row0 + 63;
 
+    // interior: the wave's 64 x 64 block lies inside the matrix and on or below the diagonal
+    // in every entry, so its loads and stores need no masks.
+    const bool interior = live && row0 + 63 < m && col0 + 63 <= row0;
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 acc[4][4];
+    if (interior) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    acc[a][b][e] = C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t i = row0 + 16 * b + r;
+                    const int64_t j = col0 + 16 * a + 4 * g + e;  // accumulator rows 4 g + e
+                    acc[a][b][e] = (live && i < m && j <= i) ? C[i + j * ldc] : 0.0f;
+                }
+    }
 
     const int64_t nchunks = kb_pad / SB_KC;
-    uint4 rh[2], rl[2], ch[2], cl[2];
+    u32x4_t rh[2], rl[2], ch[2], cl[2];
     sb_load(rh, Ph, m_pad, rows0, 0);
     sb_load(rl, Pl, m_pad, rows0, 0);
     sb_load(ch, Ph, m_pad, cols0, 0);
@@ -127,8 +162,8 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
         for (int q = 0; q < 4; ++q) {
             fh[q] = sb_frag(lds[st][2], fr + 16 * q + r, g);
             fl[q] = sb_frag(lds[st][3], fr + 16 * q + r, g);
-            sh[q] = sb_frag(lds[st][0], sr + 16 * q + r, g);
-            sl[q] = sb_frag(lds[st][1], sr + 16 * q + r, g);
+            sh[q] = sb_neg(sb_frag(lds[st][0], sr + 16 * q + r, g));
+            sl[q] = sb_neg(sb_frag(lds[st][1], sr + 16 * q + r, g));
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -148,9 +183,17 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
         st ^= 1;
     }
 
-    const int64_t col0 = cols0 + fr;
-    const int64_t row0 = rows0 + sr;
-    if (row0 >= m || col0 >= m || col0 > row0 + 63) return;
+    if (!live) return;
+    if (interior) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] = acc[a][b][e];
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -159,10 +202,9 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
             if (i >= m) continue;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + 16 * a + 4 * g + e;  // f32 accumulator rows: 4 g + e
-                if (j >= m || j > i) continue;
-                float* cp = C + i + j * ldc;
-                *cp = *cp + alpha * acc[a][b][e];
+                const int64_t j = col0 + 16 * a + 4 * g + e;
+                if (j > i) continue;
+                C[i + j * ldc] = acc[a][b][e];
             }
         }
     }
@@ -174,10 +216,10 @@ size_t syrk_bf16_workspace(int64_t n, int64_t kb) {
     return (size_t)(2 * m_pad * kb_pad) * sizeof(__bf16);
 }
 
-// C (m x m, lower) += alpha * P P^T with P (m x kb) column-major FP32; ws holds
+// C (m x m, lower) -= P P^T with P (m x kb) column-major FP32; ws holds
 // syrk_bf16_workspace(m, kb) bytes.
-int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float alpha, float* C,
-                int64_t ldc, const int32_t* info, void* ws, hipStream_t s) {
+int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
+                const int32_t* info, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
@@ -189,7 +231,7 @@ int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float alpha,
     const int64_t tm = m_pad / SB_T;
     TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
     hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3((unsigned)(tm * (tm + 1) / 2)), dim3(256), 0, s,
-                       Ph, Pl, m, m_pad, kb_pad, alpha, C, ldc, info);
+                       Ph, Pl, m, m_pad, kb_pad, C, ldc, info);
     QMLH_CHECK_LAUNCH("syrk_bf16x3 launch");
     return QMLH_OK;
 }
