@@ -213,10 +213,10 @@ def main():
         nel = args.copy_mib * (1 << 20) // 8
         src = torch.ones(nel, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        for _ in range(3):  # an elementwise kernel: reads and writes copy_mib MiB each
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {args.copy_mib} MiB device copies")
+        log(f"copy calibration: 3 x {args.copy_mib} MiB elementwise read + write")
         del src, dst
     N = args.n_train
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)

@@ -29,13 +29,14 @@
 #include <new>
 #include <vector>
 
+#include "chol_panel.h"
 #include "mfma_core.h"
 #include "pair_tile.h"
 
 namespace qmlh {
 
 static constexpr int BT = WG_T;   // update / build tile (128)
-static constexpr int BW = 64;     // sub-panel width
+static constexpr int BW = CP_B;   // sub-panel width
 static constexpr int BROWS = 256; // panel rows per workgroup
 static constexpr int BS_B = 64;   // solve block
 static constexpr unsigned BS_SPIN_LIMIT = 1u << 24;
@@ -171,68 +172,15 @@ __global__ __launch_bounds__(PT_THREADS, 2) void bbuild_laplace_kernel(
 }
 
 // ---- factor --------------------------------------------------------------------------------
-// Every workgroup of a job factors the job's w x w diagonal block in LDS (FP64, the same
-// operations in the same order, so all agree); workgroup 0 of the job writes it back and sets
-// info on a non-positive pivot (LAPACK: info = global column + 1, the Schur value left on the
-// diagonal); then each workgroup solves its 256 panel rows X := B L^-T in FP64 registers.
+// One fused launch per 64-wide sub-panel step over every active problem: each workgroup factors
+// its job's diagonal block in one wavefront's registers (FP64) and solves its 256 panel rows
+// (chol_panel.h); workgroup 0 of a job writes the block back and sets that problem's info.
 __global__ __launch_bounds__(256) void bpanel_kernel(const BPanelJob* __restrict__ jobs,
                                                      int njobs) {
-    __shared__ double s[BW][BW + 1];
-    __shared__ double sInv[BW];
     const int jid = find_job(jobs, njobs, (int64_t)blockIdx.x, &BPanelJob::block_begin);
     const BPanelJob jb = jobs[jid];
-    if (*jb.info) return;
-    const int64_t local = (int64_t)blockIdx.x - jb.block_begin;
-    const int t = threadIdx.x;
-    const int w = jb.w;
-    for (int idx = t; idx < w * w; idx += 256) {
-        const int i = idx % w, j = idx / w;
-        if (i >= j) s[i][j] = jb.A[i + j * jb.ld];
-    }
-    __syncthreads();
-    int fail = 0;
-    for (int c = 0; c < w; ++c) {
-        const double ajj = s[c][c];
-        if (!(ajj > 0.0)) {
-            fail = c + 1;
-            break;
-        }
-        const double l = sqrt(ajj);
-        __syncthreads();
-        if (t == 0) s[c][c] = l;
-        for (int i = c + 1 + t; i < w; i += 256) s[i][c] = s[i][c] / l;
-        __syncthreads();
-        const int m = w - c - 1;
-        for (int idx = t; idx < m * m; idx += 256) {
-            const int i = c + 1 + idx % m, j = c + 1 + idx / m;
-            if (i >= j) s[i][j] = s[i][j] - s[i][c] * s[j][c];
-        }
-        __syncthreads();
-    }
-    if (local == 0) {
-        for (int idx = t; idx < w * w; idx += 256) {
-            const int i = idx % w, j = idx / w;
-            if (i >= j) jb.A[i + j * jb.ld] = s[i][j];
-        }
-        if (fail && t == 0) *jb.info = (int32_t)(jb.col_base + fail);
-    }
-    if (fail) return;
-    if (t < w) sInv[t] = 1.0 / s[t][t];
-    __syncthreads();
-    const int64_t row = local * BROWS + t;
-    if (row >= jb.m) return;
-    double* B = jb.A + w;
-    double x[BW];
-#pragma unroll
-    for (int c = 0; c < BW; ++c) {
-        if (c < w) {
-            double v = B[row + c * jb.ld];
-#pragma unroll
-            for (int l = 0; l < c; ++l) v = v - x[l] * s[c][l];
-            x[c] = v * sInv[c];
-            B[row + c * jb.ld] = x[c];
-        }
-    }
+    panel64_body<double>(jb.A, jb.ld, jb.w, jb.m, jb.col_base, jb.info,
+                         (int64_t)blockIdx.x - jb.block_begin);
 }
 
 __global__ __launch_bounds__(256, 2) void bupdate_kernel(const BUpdateJob* __restrict__ jobs,

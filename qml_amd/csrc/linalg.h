@@ -21,25 +21,31 @@ int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, in
 // Synchronisation words the single-launch triangular solve needs for order n.
 size_t trsv_sync_ints(int64_t n);
 
-// Single-launch triangular solve (forward z = L^-1 rhs, or backward x = L^-T rhs).
+// Single-launch triangular solve (forward z = L^-1 rhs, or backward x = L^-T rhs). dinv:
+// optional FP64 inverses of the 64 x 64 diagonal blocks (trsv_diag_inverse).
 template <typename T>
 int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
-                  int32_t* sync, bool forward, hipStream_t s);
+                  int32_t* sync, bool forward, hipStream_t s, const double* dinv = nullptr);
+
+// Doubles trsv_diag_inverse writes for order n, and the inverses themselves.
+size_t trsv_dinv_doubles(int64_t n);
+template <typename T>
+int trsv_diag_inverse(const T* L, int64_t n, int64_t ldl, double* dinv, hipStream_t s);
 
 // x := (L L^T)^-1 x; tmp is n doubles of scratch, sync trsv_sync_ints(n) ints.
 template <typename T>
 int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32_t* sync,
                hipStream_t s);
 
-// z := L^-1 r (r may be clobbered).
+// z := L^-1 r (r may be clobbered); dinv as for trsv_syncfree (used for large n).
 template <typename T>
 int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
-                   hipStream_t s);
+                   hipStream_t s, const double* dinv = nullptr);
 
 // x := L^-T z (z may be clobbered).
 template <typename T>
 int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
-                   hipStream_t s);
+                   hipStream_t s, const double* dinv = nullptr);
 
 // V := L^-1 V for an (n, nrhs) column-major right-hand side (storage float), L lower float.
 int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,

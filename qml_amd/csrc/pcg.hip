@@ -169,7 +169,8 @@ struct PcgCtx {
     hipStream_t s;
     int64_t n;
     double* partial;
-    int32_t* sync;  // triangular-solve synchronisation words
+    int32_t* sync;        // triangular-solve synchronisation words
+    const double* dinv;   // FP64 inverses of the factor's 64 x 64 diagonal blocks
 };
 
 static inline unsigned grid1(int64_t n) { return (unsigned)ceil_div(n, 256); }
@@ -202,7 +203,7 @@ struct Woodbury {
 static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury& wb,
                    const double* r, double* z, double* q, double* tmp) {
     QMLH_TRY(copy(c, r, tmp));
-    QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, c.sync, c.s));
+    QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, c.sync, c.s, c.dinv));
     if (wb.r > 0) {
         hipLaunchKernelGGL(yt_q_kernel, dim3((unsigned)wb.r), dim3(256), 0, c.s, wb.Y, wb.ldy, c.n,
                            q, wb.t);
@@ -212,7 +213,7 @@ static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury&
                            c.n, wb.r, wb.t, q);
         QMLH_CHECK_LAUNCH("pcg q - Y t launch");
     }
-    return trsv_lower_bwd<float>(L, c.n, ldl, q, z, c.sync, c.s);
+    return trsv_lower_bwd<float>(L, c.n, ldl, q, z, c.sync, c.s, c.dinv);
 }
 
 }  // namespace qmlh
@@ -265,6 +266,7 @@ extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
     if (n <= 0) return 0;
     if (r < 0) r = 0;
     return (size_t)(5 * n + 2 * r + qmlh::RED_BLOCKS + 8) * sizeof(double) +
+           qmlh::trsv_dinv_doubles(n) * sizeof(double) +
            qmlh::trsv_sync_ints(n) * sizeof(int32_t);
 }
 
@@ -294,9 +296,14 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     double* tmp_r = t + r;
     double* partial = tmp_r + r;
     double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 6 rz_new
-    int32_t* sync = reinterpret_cast<int32_t*>(sc + 8);
-    PcgCtx c{s, n, partial, sync};
+    double* dinv = sc + 8;
+    int32_t* sync = reinterpret_cast<int32_t*>(dinv + trsv_dinv_doubles(n));
+    PcgCtx c{s, n, partial, sync, dinv};
     Woodbury wb{Y, ldy, r, C, ldc, t, tmp_r};
+    // The preconditioner's triangular solves use explicit inverses of the diagonal blocks of
+    // the low-precision factor (computed once here), so each 64-row block's diagonal solve is
+    // a parallel product instead of a sequential substitution.
+    QMLH_TRY(trsv_diag_inverse<float>(L, n, ldl, dinv, s));
 
     double yy_h = 0.0;
     QMLH_TRY(dot(c, y, y, sc + 2));

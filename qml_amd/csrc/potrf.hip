@@ -3,13 +3,13 @@
 // Replaces LAPACK dpotrf("U") / dpotrs("U", n, 1) as called at qml/fcho_solve.f90:35,46 (see
 // the header for why "U" on the Fortran view is the lower triangle here).
 //
-// Factorisation, outer block NB = 256, per step k:
-//   1. diagonal block (kb x kb): itself blocked by 64 -- potf2 in LDS (one workgroup, FP64
-//      arithmetic), a row-parallel triangular solve for the rows below, an MFMA update of the
-//      rest;
-//   2. panel A[k+kb:n, k:k+kb] := A[k+kb:n, k:k+kb] L_kk^-T: recursive halving down to
-//      64-wide row-parallel substitution (FP64 arithmetic), MFMA updates in between;
-//   3. trailing update A[k+kb:n, k+kb:n] -= P P^T on the lower tiles only (MFMA).
+// Factorisation, outer block nb (256; 512 with split-BF16 trailing updates), per step k:
+//   for each 64-wide sub-panel of the nb-wide block column (left to right):
+//     1. one fused launch (panel64): the 64 x 64 diagonal block is factored in the registers
+//        of one wavefront (FP64 arithmetic) in every workgroup, and each workgroup solves its
+//        256 rows of the whole column below against it;
+//     2. an MFMA update of the block column's remaining columns (all rows below, i >= j);
+//   3. the trailing update A[k+kb:n, k+kb:n] -= P P^T on the lower tiles only (MFMA).
 // The storage type T is double (FP64 MFMA updates) or float (FP32 MFMA updates, the
 // mixed-precision factor); diagonal blocks and substitutions always run in FP64.
 // A device int32 `info` follows LAPACK: the first non-positive (or NaN) pivot j sets
@@ -17,6 +17,7 @@
 // the factorisation returns without work.
 #include <type_traits>
 
+#include "chol_panel.h"
 #include "linalg.h"
 #include "mfma_core.h"
 #include "timer.h"
@@ -26,30 +27,32 @@ namespace qmlh {
 static constexpr int PF_NB = 256;
 // Block size when the trailing updates run as split-BF16 products: the update of C costs a
 // fixed read and write of the FP32 trailing matrix per step, so a deeper update pays off.
-#ifndef QMLH_PF_NB_BF16
-#define QMLH_PF_NB_BF16 512
-#endif
-static constexpr int PF_NB_BF16 = QMLH_PF_NB_BF16;
-static constexpr int PF_IB = 64;
+static constexpr int PF_NB_BF16 = 512;
+static constexpr int PF_IB = CP_B;
 static constexpr int UP_TILE = WG_T;
+
+// Update shapes: FULL -- every entry of the M x N block; SYRK -- the lower triangle of an
+// M x M block (lower tiles only); TRAP -- the entries i >= j of an M x N block whose (0, 0)
+// sits on the diagonal (tiles entirely above the diagonal do nothing).
+enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2 };
 
 // ------------------------------------------------------------------------------------------
 // C(i, j) -= sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL: B(j, k) at
-// B[j + k ldb]) or row-major (LB = ROW: B(j, k) at B[j ldb + k]). LOWER: only tiles with
-// bi >= bj run and only i >= j is written (a SYRK when A == B). The accumulators start from C
-// (loaded while the first operand chunks are in flight) and the products are subtracted, so
+// B[j + k ldb]) or row-major (LB = ROW: B(j, k) at B[j ldb + k]). The accumulators start from
+// C (loaded while the first operand chunks are in flight) and the products are subtracted, so
 // C is read once and written once.
 // ------------------------------------------------------------------------------------------
-template <typename T, bool LOWER, int LB = LAYOUT_COL>
+template <typename T, int MODE, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
     int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, const int32_t* __restrict__ info,
     int64_t tiles_m) {
     __shared__ __attribute__((aligned(16))) T lds[WG_LDS_ELEMS];
+    constexpr bool MASK = MODE != UPD_FULL;
     if (info && *info) return;
     int64_t bi, bj;
     const int64_t bid = blockIdx.x;
-    if (LOWER) {
+    if (MODE == UPD_SYRK) {
         int64_t i = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
         while (i * (i + 1) / 2 > bid) --i;
         while ((i + 1) * (i + 2) / 2 <= bid) ++i;
@@ -58,6 +61,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     } else {
         bi = bid % tiles_m;
         bj = bid / tiles_m;
+        if (MODE == UPD_TRAP && bj > bi) return;  // tile entirely above the diagonal
     }
     // First MFMA operand: B rows (output columns); second: A rows (output rows), so the 16
     // lanes of one accumulator element run down 16 consecutive rows of column-major C.
@@ -66,154 +70,60 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const int w = threadIdx.x >> 6;
     const int64_t col0 = bj * UP_TILE + 64 * (w & 1);
     const int64_t row0 = bi * UP_TILE + 64 * (w >> 1);
-    const bool live = row0 < M && col0 < N && !(LOWER && col0 > row0 + 63);
-    const bool interior = live && row0 + 64 <= M && col0 + 64 <= N && (!LOWER || col0 + 63 <= row0);
+    const bool live = row0 < M && col0 < N && !(MASK && col0 > row0 + 63);
+    const bool interior = live && row0 + 64 <= M && col0 + 64 <= N && (!MASK || col0 + 63 <= row0);
     WaveTile<T> t;
-    wave_tile_load_c<T, LOWER>(t, C, ldc, row0, col0, M, N, live, interior);
+    wave_tile_load_c<T, MASK>(t, C, ldc, row0, col0, M, N, live, interior);
     wg_tile_mma_lds<T, LB, LAYOUT_COL, true>(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
     if (!live) return;
-    wave_tile_store_c<T, LOWER>(t, C, ldc, row0, col0, M, N, interior);
+    wave_tile_store_c<T, MASK>(t, C, ldc, row0, col0, M, N, interior);
 }
 
-// C -= A B^T (lower: the lower triangle of C -= A A^T-style SYRK with B = A). timer_class: the
-// measurement class the launch is counted in; by default trailing (lower) updates are SYRK and
-// the rest GEMM.
+// C -= A B^T in the given shape. timer_class: the measurement class the launch is counted in
+// (by default SYRK for the trailing update shape and GEMM for the others).
 template <typename T, int LB = LAYOUT_COL>
-static int launch_update(bool lower, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
+static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
                          int64_t N, int64_t Kd, T* C, int64_t ldc, const int32_t* info,
                          hipStream_t s, int timer_class = -1) {
     if (M <= 0 || N <= 0 || Kd <= 0) return QMLH_OK;
-    int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
-    if (timer_class < 0) timer_class = lower ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
-    TimerScope ts(timer_class,
-                  lower ? (double)M * (double)(M + 1) * (double)Kd
-                        : 2.0 * (double)M * (double)N * (double)Kd,
-                  s);
-    if (lower) {
-        int64_t nt = tm * (tm + 1) / 2;
-        hipLaunchKernelGGL((gemm_update_kernel<T, true, LB>), dim3((unsigned)nt), dim3(256), 0, s,
-                           A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
-    } else {
-        hipLaunchKernelGGL((gemm_update_kernel<T, false, LB>), dim3((unsigned)(tm * tn)),
-                           dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+    const int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
+    if (timer_class < 0) timer_class = mode == UPD_SYRK ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
+    double flops = 2.0 * (double)M * (double)N * (double)Kd;
+    if (mode == UPD_SYRK) flops = (double)M * (double)(M + 1) * (double)Kd;
+    if (mode == UPD_TRAP) flops = ((double)M * (double)N - (double)N * (double)(N - 1) / 2.0) * 2.0 * (double)Kd;
+    TimerScope ts(timer_class, flops, s);
+    switch (mode) {
+        case UPD_SYRK:
+            hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+            break;
+        case UPD_TRAP:
+            hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP, LB>), dim3((unsigned)(tm * tn)),
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+            break;
+        default:
+            hipLaunchKernelGGL((gemm_update_kernel<T, UPD_FULL, LB>), dim3((unsigned)(tm * tn)),
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
 }
 
-// ------------------------------------------------------------------------------------------
-// Unblocked right-looking Cholesky of a w x w (w <= 64) diagonal block, FP64 in LDS.
-// ------------------------------------------------------------------------------------------
+// Fused 64-wide sub-panel: factor the w x w diagonal block at A and solve the m rows below it.
 template <typename T>
-__global__ __launch_bounds__(256) void potf2_kernel(T* __restrict__ A, int64_t lda, int w,
-                                                    int64_t col_base, int32_t* __restrict__ info) {
-    if (*info) return;
-    __shared__ double s[PF_IB][PF_IB + 1];
-    const int t = threadIdx.x;
-    for (int idx = t; idx < w * w; idx += blockDim.x) {
-        int i = idx % w, j = idx / w;
-        if (i >= j) s[i][j] = (double)A[i + j * lda];
-    }
-    __syncthreads();
-    int fail = 0;
-    for (int c = 0; c < w; ++c) {
-        const double ajj = s[c][c];
-        if (!(ajj > 0.0)) {  // non-positive or NaN pivot: LAPACK stops here
-            fail = c + 1;
-            break;
-        }
-        const double l = sqrt(ajj);
-        __syncthreads();
-        if (t == 0) s[c][c] = l;
-        for (int i = c + 1 + t; i < w; i += blockDim.x) s[i][c] = s[i][c] / l;
-        __syncthreads();
-        const int m = w - c - 1;  // trailing order
-        for (int idx = t; idx < m * m; idx += blockDim.x) {
-            int i = c + 1 + idx % m, j = c + 1 + idx / m;
-            if (i >= j) s[i][j] = s[i][j] - s[i][c] * s[j][c];
-        }
-        __syncthreads();
-    }
-    for (int idx = t; idx < w * w; idx += blockDim.x) {
-        int i = idx % w, j = idx / w;
-        if (i >= j) A[i + j * lda] = (T)s[i][j];
-    }
-    if (fail && t == 0) *info = (int32_t)(col_base + fail);
-}
-
-// ------------------------------------------------------------------------------------------
-// Row-parallel triangular solve X := B L^-T for a panel with w <= 64 columns: every lane
-// owns one row and runs the substitution in FP64 registers; L sits in LDS.
-// ------------------------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_base_kernel(T* __restrict__ B, int64_t ldb,
-                                                        int64_t m, const T* __restrict__ L,
-                                                        int64_t ldl, int w,
-                                                        const int32_t* __restrict__ info) {
-    if (info && *info) return;
-    __shared__ double sL[PF_IB][PF_IB + 1];
-    __shared__ double sInv[PF_IB];
-    for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
-        int i = idx % w, j = idx / w;
-        if (i >= j) sL[i][j] = (double)L[i + j * ldl];
-    }
-    __syncthreads();
-    if (threadIdx.x < w) sInv[threadIdx.x] = 1.0 / sL[threadIdx.x][threadIdx.x];
-    __syncthreads();
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    double x[PF_IB];
-#pragma unroll
-    for (int c = 0; c < PF_IB; ++c) {
-        if (c < w) {
-            double s = (double)B[row + c * ldb];
-#pragma unroll
-            for (int l = 0; l < c; ++l) s = s - x[l] * sL[c][l];
-            x[c] = s * sInv[c];
-            B[row + c * ldb] = (T)x[c];
-        }
-    }
+__global__ __launch_bounds__(256) void panel64_kernel(T* __restrict__ A, int64_t lda, int w,
+                                                      int64_t m, int64_t col_base,
+                                                      int32_t* __restrict__ info) {
+    panel64_body<T>(A, lda, w, m, col_base, info, blockIdx.x);
 }
 
 template <typename T>
-static int launch_trsm_base(T* B, int64_t ldb, int64_t m, const T* L, int64_t ldl, int w,
-                            const int32_t* info, hipStream_t s) {
-    if (m <= 0) return QMLH_OK;
-    hipLaunchKernelGGL(trsm_base_kernel<T>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, B,
-                       ldb, m, L, ldl, w, info);
-    QMLH_CHECK_LAUNCH("trsm_base launch");
-    return QMLH_OK;
-}
-
-// X := B L^-T, B (m x w) and L (w x w, lower) column-major, by recursive halving.
-template <typename T>
-static int trsm_panel(T* B, int64_t ldb, int64_t m, const T* L, int64_t ldl, int64_t w,
-                      const int32_t* info, hipStream_t s) {
-    if (w <= PF_IB) return launch_trsm_base(B, ldb, m, L, ldl, (int)w, info, s);
-    int64_t h = round_up(w / 2, PF_IB);
-    QMLH_TRY(trsm_panel(B, ldb, m, L, ldl, h, info, s));
-    // B2 -= X1 L21^T
-    QMLH_TRY(launch_update<T>(false, B, ldb, L + h, ldl, m, w - h, h, B + h * ldb, ldb, info, s));
-    return trsm_panel(B + h * ldb, ldb, m, L + h + h * ldl, ldl, w - h, info, s);
-}
-
-// Factor a kb x kb (kb <= NB) diagonal block: 64-wide potf2 / trsm / syrk steps.
-template <typename T>
-static int factor_diag(T* A, int64_t lda, int64_t kb, int64_t col_base, int32_t* info,
-                       hipStream_t s) {
-    for (int64_t kk = 0; kk < kb; kk += PF_IB) {
-        int w = (int)(kb - kk < PF_IB ? kb - kk : PF_IB);
-        T* Akk = A + kk + kk * lda;
-        hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Akk, lda, w, col_base + kk,
-                           info);
-        QMLH_CHECK_LAUNCH("potf2 launch");
-        int64_t rest = kb - kk - w;
-        if (rest > 0) {
-            QMLH_TRY(launch_trsm_base(Akk + w, lda, rest, (const T*)Akk, lda, w, info, s));
-            QMLH_TRY(launch_update<T>(true, Akk + w, lda, Akk + w, lda, rest, rest, w,
-                                      Akk + w + w * lda, lda, info, s, QMLH_TIMER_GEMM));
-        }
-    }
+static int launch_panel64(T* A, int64_t lda, int w, int64_t m, int64_t col_base, int32_t* info,
+                          hipStream_t s) {
+    const int64_t blocks = m > 0 ? ceil_div(m, 256) : 1;
+    hipLaunchKernelGGL(panel64_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, A, lda, w, m,
+                       col_base, info);
+    QMLH_CHECK_LAUNCH("panel64 launch");
     return QMLH_OK;
 }
 
@@ -232,19 +142,26 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     const bool bf16 = std::is_same<T, float>::value && bf16_ws;
     const int64_t nb = bf16 ? PF_NB_BF16 : PF_NB;
     for (int64_t k = 0; k < n; k += nb) {
-        int64_t kb = n - k < nb ? n - k : nb;
-        T* Akk = A + k + k * lda;
-        QMLH_TRY(factor_diag(Akk, lda, kb, k, info, s));
-        int64_t m = n - k - kb;
+        const int64_t kb = n - k < nb ? n - k : nb;
+        for (int64_t jj = 0; jj < kb; jj += PF_IB) {
+            const int64_t c = k + jj;
+            const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
+            T* Acc = A + c + c * lda;
+            QMLH_TRY(launch_panel64<T>(Acc, lda, w, n - c - w, c, info, s));
+            const int64_t ncols = kb - jj - w;  // rest of this block column
+            if (ncols > 0)
+                QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, n - c - w, ncols,
+                                          w, Acc + w + w * lda, lda, info, s));
+        }
+        const int64_t m = n - k - kb;
         if (m > 0) {
-            T* P = Akk + kb;
-            QMLH_TRY(trsm_panel(P, lda, m, (const T*)Akk, lda, kb, info, s));
+            T* P = A + (k + kb) + k * lda;
             if (bf16) {
                 QMLH_TRY(syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb,
                                      reinterpret_cast<float*>(P + kb * lda), lda, info, bf16_ws,
                                      s));
             } else {
-                QMLH_TRY(launch_update<T>(true, P, lda, P, lda, m, m, kb, P + kb * lda, lda,
+                QMLH_TRY(launch_update<T>(UPD_SYRK, P, lda, P, lda, m, m, kb, P + kb * lda, lda,
                                           info, s));
             }
         }
@@ -363,8 +280,9 @@ static constexpr int64_t TRSV_SYNCFREE_MIN = 1024;
 
 template <typename T>
 int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
-                   hipStream_t s) {
-    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, r, z, sync, true, s);
+                   hipStream_t s, const double* dinv) {
+    if (n >= TRSV_SYNCFREE_MIN && sync)
+        return trsv_syncfree<T>(L, n, lda, r, z, sync, true, s, dinv);
     for (int64_t k = 0; k < n; k += PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
         const int64_t rest = n - k - w;
@@ -382,8 +300,9 @@ int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int
 
 template <typename T>
 int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
-                   hipStream_t s) {
-    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, z, x, sync, false, s);
+                   hipStream_t s, const double* dinv) {
+    if (n >= TRSV_SYNCFREE_MIN && sync)
+        return trsv_syncfree<T>(L, n, lda, z, x, sync, false, s, dinv);
     const int64_t last = ((n - 1) / PF_IB) * PF_IB;
     for (int64_t k = last; k >= 0; k -= PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
@@ -452,7 +371,7 @@ int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64
         QMLH_CHECK_LAUNCH("trsm multi diag launch");
         const int64_t rest = n - k - w;
         // V[k+w:, :] -= L[k+w:, k:k+w] * V[k:k+w, :]; B(j, c) = V[(k + c) + j ldv] (row layout).
-        QMLH_TRY((launch_update<float, LAYOUT_ROW>(false, L + (k + w) + k * ldl, ldl, V + k, ldv,
+        QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (k + w) + k * ldl, ldl, V + k, ldv,
                                                    rest, nrhs, w, V + (k + w), ldv, nullptr, s)));
     }
     return QMLH_OK;
@@ -464,14 +383,16 @@ template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
                                hipStream_t);
-template int trsv_lower_fwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
-                                   hipStream_t);
-template int trsv_lower_bwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
-                                   hipStream_t);
+template int trsv_lower_fwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*, hipStream_t,
+                                   const double*);
+template int trsv_lower_bwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*, hipStream_t,
+                                   const double*);
 template int trsv_lower_fwd<double>(const double*, int64_t, int64_t, double*, double*,
-                                    int32_t*, hipStream_t);
+                                    int32_t*, hipStream_t,
+                                   const double*);
 template int trsv_lower_bwd<double>(const double*, int64_t, int64_t, double*, double*,
-                                    int32_t*, hipStream_t);
+                                    int32_t*, hipStream_t,
+                                   const double*);
 
 }  // namespace qmlh
 

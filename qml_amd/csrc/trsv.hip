@@ -58,12 +58,15 @@ __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L,
 
 // Solve block b of one system: wait for the blocks it depends on (flags[j] of the earlier
 // blocks in solve order), accumulate their contributions, solve the diagonal block, publish.
-// `err` is the shared error word. All 256 lanes call this.
+// `err` is the shared error word. With dinv (the FP64 inverses of the 64 x 64 diagonal blocks,
+// trsv_diag_inverse) the diagonal solve is a matrix-vector product spread over all lanes
+// instead of a 64-step substitution on the critical path. All 256 lanes call this.
 template <typename T, bool FWD>
 __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl, int64_t n,
                                            const double* __restrict__ rhs,
                                            double* __restrict__ out, int32_t* __restrict__ flags,
-                                           int32_t* __restrict__ err, int64_t b) {
+                                           int32_t* __restrict__ err, int64_t b,
+                                           const double* __restrict__ dinv) {
     __shared__ double sL[TS_B][TS_B + 1];
     __shared__ double sInv[TS_B];
     __shared__ double red[TS_B][TS_B + 1];
@@ -77,9 +80,19 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     const int lane = t & 63, grp = t >> 6;  // 4 groups of 16 columns
 
     // The diagonal block does not depend on other blocks: stage it first.
-    for (int idx = t; idx < TS_B * TS_B; idx += 256) {
-        const int i = idx % TS_B, jj = idx / TS_B;
-        if (i < w && jj < w && i >= jj) sL[i][jj] = (double)L[(r0 + i) + (r0 + jj) * ldl];
+    if (dinv) {
+        // inverse block: FWD x = D s, BWD x = D^T s; lane (i, part) will use row/column i,
+        // entries 16 part .. 16 part + 15
+        const double* D = dinv + b * (TS_B * TS_B);
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int i = idx % TS_B, jj = idx / TS_B;
+            sL[i][jj] = FWD ? D[i + jj * TS_B] : D[jj + i * TS_B];
+        }
+    } else {
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int i = idx % TS_B, jj = idx / TS_B;
+            if (i < w && jj < w && i >= jj) sL[i][jj] = (double)L[(r0 + i) + (r0 + jj) * ldl];
+        }
     }
     __syncthreads();  // s_fail initialised before any lane may set it
 
@@ -136,10 +149,22 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
             for (int i = 0; i < TS_B; ++i) s += red[i][t];
         }
         sv[t] = (t < w ? rhs[r0 + t] : 0.0) - s;
-        if (t < w) sInv[t] = 1.0 / sL[t][t];
+        if (t < w && !dinv) sInv[t] = 1.0 / sL[t][t];
     }
     __syncthreads();
-    if (t < 64) {
+    if (dinv) {
+        // x_i = sum_c M(i, c) sv_c: four partial sums per row, added in a fixed order.
+        double p = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) p = fma(sL[lane][16 * grp + c], sv[16 * grp + c], p);
+        red[lane][grp] = p;
+        __syncthreads();
+        if (t < 64 && t < w) {
+            const double xi = ((red[t][0] + red[t][1]) + red[t][2]) + red[t][3];
+            __hip_atomic_store(&out[r0 + t], s_fail ? __builtin_nan("") : xi, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (t < 64) {
         const int i = t;
         double xi = sv[i];
         if (FWD) {
@@ -166,11 +191,41 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     }
 }
 
+// dinv block b (64 x 64, column-major, zero above the diagonal and outside the block) :=
+// L_bb^-1 in FP64. Lane c computes column c by forward substitution in registers.
+template <typename T>
+__global__ __launch_bounds__(64) void trsv_diag_inverse_kernel(const T* __restrict__ L,
+                                                               int64_t ldl, int64_t n,
+                                                               double* __restrict__ dinv) {
+    __shared__ double sL[TS_B][TS_B + 1];
+    const int64_t b = blockIdx.x;
+    const int64_t r0 = b * TS_B;
+    const int w = (int)(n - r0 < TS_B ? n - r0 : TS_B);
+    const int c = threadIdx.x;
+    for (int idx = c; idx < TS_B * TS_B; idx += 64) {
+        const int i = idx % TS_B, j = idx / TS_B;
+        sL[i][j] = (i < w && j < w && i >= j) ? (double)L[(r0 + i) + (r0 + j) * ldl] : 0.0;
+    }
+    __syncthreads();
+    double x[TS_B];
+#pragma unroll
+    for (int i = 0; i < TS_B; ++i) {
+        double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = fma(-sL[i][k], x[k], v);
+        x[i] = (i < w && c < w && i >= c) ? v / sL[i][i] : 0.0;
+    }
+    double* D = dinv + b * (TS_B * TS_B);
+#pragma unroll
+    for (int i = 0; i < TS_B; ++i) D[i + c * TS_B] = x[i];
+}
+
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
                                                             int64_t n, const double* __restrict__ rhs,
                                                             double* __restrict__ out,
-                                                            int32_t* __restrict__ sync) {
+                                                            int32_t* __restrict__ sync,
+                                                            const double* __restrict__ dinv) {
     __shared__ int s_block;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
     if (threadIdx.x == 0) {
@@ -178,14 +233,25 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
         s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
     }
     __syncthreads();
-    trsv_block<T, FWD>(L, ldl, n, rhs, out, sync + 2, &sync[1], s_block);
+    trsv_block<T, FWD>(L, ldl, n, rhs, out, sync + 2, &sync[1], s_block, dinv);
 }
 
 size_t trsv_sync_ints(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B + 2); }
 
+size_t trsv_dinv_doubles(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B) * TS_B * TS_B; }
+
+template <typename T>
+int trsv_diag_inverse(const T* L, int64_t n, int64_t ldl, double* dinv, hipStream_t s) {
+    if (n <= 0) return QMLH_OK;
+    hipLaunchKernelGGL(trsv_diag_inverse_kernel<T>, dim3((unsigned)((n + TS_B - 1) / TS_B)),
+                       dim3(64), 0, s, L, ldl, n, dinv);
+    QMLH_CHECK_LAUNCH("trsv diagonal inverse launch");
+    return QMLH_OK;
+}
+
 template <typename T>
 int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
-                  int32_t* sync, bool forward, hipStream_t s) {
+                  int32_t* sync, bool forward, hipStream_t s, const double* dinv) {
     if (n <= 0) return QMLH_OK;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
     const int64_t cnt = nblocks + 2;
@@ -193,17 +259,19 @@ int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double*
                        sync, cnt);
     if (forward)
         hipLaunchKernelGGL((trsv_syncfree_kernel<T, true>), dim3((unsigned)nblocks), dim3(256), 0,
-                           s, L, ldl, n, rhs, out, sync);
+                           s, L, ldl, n, rhs, out, sync, dinv);
     else
         hipLaunchKernelGGL((trsv_syncfree_kernel<T, false>), dim3((unsigned)nblocks), dim3(256),
-                           0, s, L, ldl, n, rhs, out, sync);
+                           0, s, L, ldl, n, rhs, out, sync, dinv);
     QMLH_CHECK_LAUNCH("trsv launch");
     return QMLH_OK;
 }
 
 template int trsv_syncfree<double>(const double*, int64_t, int64_t, const double*, double*,
-                                   int32_t*, bool, hipStream_t);
+                                   int32_t*, bool, hipStream_t, const double*);
 template int trsv_syncfree<float>(const float*, int64_t, int64_t, const double*, double*,
-                                  int32_t*, bool, hipStream_t);
+                                  int32_t*, bool, hipStream_t, const double*);
+template int trsv_diag_inverse<double>(const double*, int64_t, int64_t, double*, hipStream_t);
+template int trsv_diag_inverse<float>(const float*, int64_t, int64_t, double*, hipStream_t);
 
 }  // namespace qmlh
