@@ -10,10 +10,11 @@
 //            K_p(i, j), i >= j, plus lambda_p on the diagonal (FP64 MFMA cross term for the
 //            Gaussian, FP64 VALU L1 tile for the Laplacian);
 //   factor   right-looking blocked Cholesky, all problems in lock step over column blocks of
-//            width nb (64, 128 or 256; tuned). Per 64-wide sub-panel one fused launch factors
-//            the diagonal block in LDS (FP64) in every workgroup and solves that workgroup's
-//            256 panel rows against it; an MFMA launch updates the rest of the nb-wide panel;
-//            per block column one MFMA launch updates the lower trailing tiles;
+//            width nb (64, 128 or 256; tuned). Per 64-wide sub-panel one launch factors the
+//            diagonal blocks (one workgroup per problem, FP64 in one wavefront's registers),
+//            one solves the panel rows below them (256 rows per workgroup), and an MFMA launch
+//            updates the rest of the nb-wide panel; per block column one MFMA launch updates
+//            the lower trailing tiles;
 //   solve    forward and backward substitutions of all problems in two single launches:
 //            workgroups take 64-row blocks from an atomic ticket in level order (block l of
 //            every problem that has one, then block l + 1, ...), wait (bounded) for the blocks
@@ -172,15 +173,19 @@ __global__ __launch_bounds__(PT_THREADS, 2) void bbuild_laplace_kernel(
 }
 
 // ---- factor --------------------------------------------------------------------------------
-// One fused launch per 64-wide sub-panel step over every active problem: each workgroup factors
-// its job's diagonal block in one wavefront's registers (FP64) and solves its 256 panel rows
-// (chol_panel.h); workgroup 0 of a job writes the block back and sets that problem's info.
-__global__ __launch_bounds__(256) void bpanel_kernel(const BPanelJob* __restrict__ jobs,
-                                                     int njobs) {
+// Per 64-wide sub-panel step over every active problem: one launch factors each job's diagonal
+// block (one workgroup per job, chol_panel.h) and sets that problem's info; the next solves the
+// panel rows below (256 rows per workgroup) against the finished factor.
+__global__ __launch_bounds__(256) void bpotf2_kernel(const BPanelJob* __restrict__ jobs) {
+    const BPanelJob jb = jobs[blockIdx.x];
+    potf2_body<double>(jb.A, jb.ld, jb.w, jb.col_base, jb.info);
+}
+
+__global__ __launch_bounds__(256) void btrsm_kernel(const BPanelJob* __restrict__ jobs,
+                                                    int njobs) {
     const int jid = find_job(jobs, njobs, (int64_t)blockIdx.x, &BPanelJob::block_begin);
     const BPanelJob jb = jobs[jid];
-    panel64_body<double>(jb.A, jb.ld, jb.w, jb.m, jb.col_base, jb.info,
-                         (int64_t)blockIdx.x - jb.block_begin);
+    trsm_body<double>(jb.A, jb.ld, jb.w, jb.m, jb.info, (int64_t)blockIdx.x - jb.block_begin);
 }
 
 __global__ __launch_bounds__(256, 2) void bupdate_kernel(const BUpdateJob* __restrict__ jobs,
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(256) void bsolve_kernel(const BSolveProb* __restric
 }
 
 // ---- planner -------------------------------------------------------------------------------
-enum { BL_CLEAR, BL_BUILD_GAUSS, BL_BUILD_LAPLACE, BL_PANEL, BL_UPDATE, BL_SOLVE_FWD,
+enum { BL_CLEAR, BL_BUILD_GAUSS, BL_BUILD_LAPLACE, BL_POTF2, BL_TRSM, BL_UPDATE, BL_SOLVE_FWD,
        BL_SOLVE_BWD };
 
 struct BLaunch {
@@ -488,7 +493,7 @@ static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Ba
                 p.info = infop(h);
                 p.w = w;
                 p.pad = 0;
-                blocks += std::max<int64_t>(1, ceil_div(p.m, (int64_t)BROWS));
+                blocks += ceil_div(p.m, (int64_t)BROWS);
                 pj.push_back(p);
                 const int64_t ncols = kb - jj - w;  // rest of this nb-wide panel
                 if (ncols > 0) {
@@ -508,8 +513,12 @@ static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Ba
                     uj.push_back(u);
                 }
             }
-            if (!pj.empty())
-                L.launches.push_back({BL_PANEL, blocks, push_table(L.tables, pj), (int)pj.size()});
+            if (!pj.empty()) {
+                const size_t off = push_table(L.tables, pj);
+                L.launches.push_back({BL_POTF2, (int64_t)pj.size(), off, (int)pj.size()});
+                // jobs with m = 0 own no row blocks: find_job skips them (same first block)
+                if (blocks > 0) L.launches.push_back({BL_TRSM, blocks, off, (int)pj.size()});
+            }
             if (!uj.empty())
                 L.launches.push_back({BL_UPDATE, tiles, push_table(L.tables, uj), (int)uj.size()});
         }
@@ -712,8 +721,12 @@ extern "C" int qmlh_krr_batched_run(qmlh_batched_plan* plan, qmlh_stream_t strea
                                    reinterpret_cast<const BBuildJob*>(tab + l.table_off), l.njobs,
                                    plan->X, plan->ldx, plan->d);
                 break;
-            case BL_PANEL:
-                hipLaunchKernelGGL(bpanel_kernel, grid, block, 0, s,
+            case BL_POTF2:
+                hipLaunchKernelGGL(bpotf2_kernel, grid, block, 0, s,
+                                   reinterpret_cast<const BPanelJob*>(tab + l.table_off));
+                break;
+            case BL_TRSM:
+                hipLaunchKernelGGL(btrsm_kernel, grid, block, 0, s,
                                    reinterpret_cast<const BPanelJob*>(tab + l.table_off), l.njobs);
                 break;
             case BL_UPDATE:

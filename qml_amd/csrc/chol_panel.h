@@ -1,6 +1,6 @@
 // Cholesky panel building blocks shared by the single-matrix factorisation (potrf.hip) and the
-// batched one (batched.hip): the unblocked factor of a <= 64 x 64 diagonal block held in the
-// registers of one wavefront, and the row-parallel solve of the panel rows below it.
+// batched one (batched.hip): the factor of a <= 64 x 64 diagonal block in LDS, and the
+// row-parallel solve of the panel rows below it.
 #pragma once
 
 #include "common.h"
@@ -9,76 +9,128 @@ namespace qmlh {
 
 static constexpr int CP_B = 64;  // diagonal block / sub-panel width
 
-// Right-looking unblocked Cholesky of the w x w (w <= 64) lower block in s (FP64), by wave 0
-// of the workgroup only: lane i keeps row i in registers; column c is scaled by 1 / L(c, c)
-// and its entries are broadcast with lane shuffles for the rank-1 update of the rows below,
-// so no LDS round trips or barriers sit on the 64-step critical path. On exit s holds L in
-// its lower triangle. Returns 0, or c + 1 for the first non-positive (or NaN) pivot c, with
-// the Schur value left on that diagonal entry (as LAPACK leaves it). The result is the same
-// in every lane (the pivot is read from one lane).
-__device__ __forceinline__ int potf2_wave(double (*s)[CP_B + 1], int w) {
-    const int lane = threadIdx.x & 63;
-    double a[CP_B];
+static constexpr int CP_S = 16;  // step of the blocked diagonal factor
+
+// Cholesky of the w x w (w <= 64) lower block in s (FP64, LDS), all 256 lanes, in four
+// 16-wide steps: the 16 x 16 diagonal block is factored in the registers of lanes 0..15 of
+// wave 0 (pivot and column entries broadcast by lane shuffles), the rows below it are solved
+// against it (one lane per row), and the trailing lower block gets the rank-16 update (lanes
+// spread over its entries). On exit s holds L in its lower triangle. Returns 0, or c + 1 for
+// the first non-positive (or NaN) pivot c, with the Schur value left on that diagonal entry
+// (as LAPACK leaves it); the value is the same in every lane.
+__device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
+    __shared__ int s_fail;
+    const int t = threadIdx.x;
+    if (t == 0) s_fail = 0;
+    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < CP_B; ++j) a[j] = (lane < w && j <= lane) ? s[lane][j] : 0.0;
-    int fail = 0;
+    for (int kb = 0; kb < CP_B; kb += CP_S) {
+        if (kb >= w) break;
+        const int bw = w - kb < CP_S ? w - kb : CP_S;
+        // 1. factor the diagonal 16 x 16 block
+        if (t < 64) {
+            const int lane = t;
+            double a[CP_S];
 #pragma unroll
-    for (int c = 0; c < CP_B; ++c) {
-        if (c < w && fail == 0) {
-            const double piv = __shfl(a[c], c, 64);
-            if (!(piv > 0.0)) {
-                fail = c + 1;
-            } else {
-                const double l = sqrt(piv);
-                const double rl = 1.0 / l;
-                a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
+            for (int j = 0; j < CP_S; ++j)
+                a[j] = (lane < bw && j <= lane) ? s[kb + lane][kb + j] : 0.0;
+            int f = 0;
 #pragma unroll
-                for (int j = c + 1; j < CP_B; ++j) {
-                    if (j < w) {
-                        const double lj = __shfl(a[c], j, 64);  // L(j, c)
-                        // Rows i < j receive upper-triangle values that are never used.
-                        a[j] = fma(-a[c], lj, a[j]);
+            for (int c = 0; c < CP_S; ++c) {
+                if (c < bw && f == 0) {
+                    const double piv = __shfl(a[c], c, 64);
+                    if (!(piv > 0.0)) {
+                        f = c + 1;
+                    } else {
+                        const double l = sqrt(piv);
+                        const double rl = 1.0 / l;
+                        a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
+#pragma unroll
+                        for (int j = c + 1; j < CP_S; ++j) {
+                            const double lj = __shfl(a[c], j, 64);  // L(kb + j, kb + c)
+                            a[j] = fma(-a[c], lj, a[j]);  // rows above j: unused upper values
+                        }
                     }
                 }
             }
-        }
-    }
+            if (lane < bw) {
 #pragma unroll
-    for (int j = 0; j < CP_B; ++j)
-        if (lane < w && j <= lane) s[lane][j] = a[j];
-    return fail;
+                for (int j = 0; j < CP_S; ++j)
+                    if (j <= lane) s[kb + lane][kb + j] = a[j];
+            }
+            if (lane == 0 && f) s_fail = kb + f;
+        }
+        __syncthreads();
+        if (s_fail) return s_fail;
+        const int r0 = kb + bw, rem = w - r0;  // rows below the block
+        // 2. rows below: X := B L^-T (one lane per row)
+        if (t < rem) {
+            const int i = r0 + t;
+            double x[CP_S];
+#pragma unroll
+            for (int c = 0; c < CP_S; ++c) x[c] = c < bw ? s[i][kb + c] : 0.0;
+#pragma unroll
+            for (int c = 0; c < CP_S; ++c) {
+                if (c < bw) {
+                    x[c] = x[c] / s[kb + c][kb + c];
+#pragma unroll
+                    for (int j = c + 1; j < CP_S; ++j) x[j] = fma(-x[c], s[kb + j][kb + c], x[j]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CP_S; ++c)
+                if (c < bw) s[i][kb + c] = x[c];
+        }
+        __syncthreads();
+        // 3. trailing lower block: s[i][j] -= sum_c s[i][kb + c] s[j][kb + c], r0 <= j <= i < w
+        constexpr int RM = CP_B - CP_S;  // largest trailing order (48); entries beyond rem skip
+        for (int idx = t; idx < RM * RM; idx += 256) {
+            const int io = idx % RM, jo = idx / RM;
+            if (io < rem && jo <= io) {
+                const int i = r0 + io, j = r0 + jo;
+                double v = s[i][j];
+#pragma unroll
+                for (int c = 0; c < CP_S; ++c)
+                    if (c < bw) v = fma(-s[i][kb + c], s[j][kb + c], v);
+                s[i][j] = v;
+            }
+        }
+        __syncthreads();
+    }
+    return 0;
 }
 
-// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L (w x w) in s and the
-// reciprocal pivots in inv; FP64 arithmetic, storage T.
+// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L^T in sT (sT[c][j] =
+// L(j, c)) and the reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded
+// whole first; then, column by column, x_c is scaled and subtracted from the later entries
+// (independent FMAs; per entry the same operations in the same order as a row substitution);
+// then the row is stored.
 template <typename T>
 __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t row, int w,
-                                         const double (*s)[CP_B + 1],
+                                         const double (*sT)[CP_B + 2],
                                          const double* __restrict__ inv) {
     double x[CP_B];
 #pragma unroll
+    for (int c = 0; c < CP_B; ++c) x[c] = c < w ? (double)B[row + c * ldb] : 0.0;
+#pragma unroll
     for (int c = 0; c < CP_B; ++c) {
         if (c < w) {
-            double v = (double)B[row + c * ldb];
+            x[c] = x[c] * inv[c];
 #pragma unroll
-            for (int l = 0; l < c; ++l) v = v - x[l] * s[c][l];
-            x[c] = v * inv[c];
-            B[row + c * ldb] = (T)x[c];
+            for (int j = c + 1; j < CP_B; ++j) x[j] = fma(-x[c], sT[c][j], x[j]);
         }
     }
+#pragma unroll
+    for (int c = 0; c < CP_B; ++c)
+        if (c < w) B[row + c * ldb] = (T)x[c];
 }
 
-// One launch per 64-wide sub-panel: every workgroup loads the w x w diagonal block at A,
-// factors it in wave 0 (all workgroups compute the same factor), workgroup 0 writes it back
-// and sets info = col_base + c + 1 on a failed pivot, then each workgroup solves its 256 rows
-// of the m rows below the block. `info` already set: nothing is done.
+// Diagonal block of a sub-panel, one workgroup: factor the w x w block at A in wave 0, write
+// L back, set info = col_base + c + 1 on a failed pivot. `info` already set: nothing is done.
 template <typename T>
-__device__ __forceinline__ void panel64_body(T* __restrict__ A, int64_t lda, int w, int64_t m,
-                                             int64_t col_base, int32_t* __restrict__ info,
-                                             int64_t local) {
+__device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
+                                           int64_t col_base, int32_t* __restrict__ info) {
     __shared__ double s[CP_B][CP_B + 1];
-    __shared__ double inv[CP_B];
-    __shared__ int s_fail;
     if (*info) return;
     const int t = threadIdx.x;
     for (int idx = t; idx < w * w; idx += 256) {
@@ -86,24 +138,33 @@ __device__ __forceinline__ void panel64_body(T* __restrict__ A, int64_t lda, int
         if (i >= j) s[i][j] = (double)A[i + j * lda];
     }
     __syncthreads();
-    if (t < 64) {
-        const int f = potf2_wave(s, w);
-        if (t == 0) s_fail = f;
+    const int fail = potf2_block64(s, w);
+    for (int idx = t; idx < w * w; idx += 256) {
+        const int i = idx % w, j = idx / w;
+        if (i >= j) A[i + j * lda] = (T)s[i][j];
+    }
+    if (t == 0 && fail) *info = (int32_t)(col_base + fail);
+}
+
+// Rows of a sub-panel, workgroup `local` of the launch: rows 256 local .. of the m rows below
+// the factored w x w block L at A become B L^-T. Runs after potf2_body in stream order, so L
+// is final when it is read. `info` set: nothing is done.
+template <typename T>
+__device__ __forceinline__ void trsm_body(T* __restrict__ A, int64_t lda, int w, int64_t m,
+                                          const int32_t* __restrict__ info, int64_t local) {
+    __shared__ __attribute__((aligned(16))) double sT[CP_B][CP_B + 2];  // sT[c][j] = L(j, c)
+    __shared__ double inv[CP_B];
+    if (*info) return;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < CP_B * CP_B; idx += 256) {
+        const int j = idx % CP_B, c = idx / CP_B;  // column-major: consecutive lanes, rows j
+        sT[c][j] = (j < w && c < w && j >= c) ? (double)A[j + c * lda] : 0.0;
     }
     __syncthreads();
-    const int fail = s_fail;
-    if (local == 0) {
-        for (int idx = t; idx < w * w; idx += 256) {
-            const int i = idx % w, j = idx / w;
-            if (i >= j) A[i + j * lda] = (T)s[i][j];
-        }
-        if (fail && t == 0) *info = (int32_t)(col_base + fail);
-    }
-    if (fail) return;
-    if (t < w) inv[t] = 1.0 / s[t][t];
+    if (t < w) inv[t] = 1.0 / sT[t][t];
     __syncthreads();
     const int64_t row = local * 256 + t;
-    if (row < m) trsm_row<T>(A + w, lda, row, w, s, inv);
+    if (row < m) trsm_row<T>(A + w, lda, row, w, sT, inv);
 }
 
 }  // namespace qmlh

@@ -5,9 +5,9 @@
 //
 // Factorisation, outer block nb (256; 512 with split-BF16 trailing updates), per step k:
 //   for each 64-wide sub-panel of the nb-wide block column (left to right):
-//     1. one fused launch (panel64): the 64 x 64 diagonal block is factored in the registers
-//        of one wavefront (FP64 arithmetic) in every workgroup, and each workgroup solves its
-//        256 rows of the whole column below against it;
+//     1. the 64 x 64 diagonal block is factored in the registers of one wavefront (FP64
+//        arithmetic), then every row of the column below is solved against it (256 rows per
+//        workgroup);
 //     2. an MFMA update of the block column's remaining columns (all rows below, i >= j);
 //   3. the trailing update A[k+kb:n, k+kb:n] -= P P^T on the lower tiles only (MFMA).
 // The storage type T is double (FP64 MFMA updates) or float (FP32 MFMA updates, the
@@ -109,21 +109,30 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
     return QMLH_OK;
 }
 
-// Fused 64-wide sub-panel: factor the w x w diagonal block at A and solve the m rows below it.
+// 64-wide sub-panel: the diagonal block (one workgroup), then the rows below it.
 template <typename T>
-__global__ __launch_bounds__(256) void panel64_kernel(T* __restrict__ A, int64_t lda, int w,
-                                                      int64_t m, int64_t col_base,
-                                                      int32_t* __restrict__ info) {
-    panel64_body<T>(A, lda, w, m, col_base, info, blockIdx.x);
+__global__ __launch_bounds__(256) void potf2_kernel(T* __restrict__ A, int64_t lda, int w,
+                                                    int64_t col_base, int32_t* __restrict__ info) {
+    potf2_body<T>(A, lda, w, col_base, info);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ A, int64_t lda, int w,
+                                                        int64_t m,
+                                                        const int32_t* __restrict__ info) {
+    trsm_body<T>(A, lda, w, m, info, blockIdx.x);
 }
 
 template <typename T>
 static int launch_panel64(T* A, int64_t lda, int w, int64_t m, int64_t col_base, int32_t* info,
                           hipStream_t s) {
-    const int64_t blocks = m > 0 ? ceil_div(m, 256) : 1;
-    hipLaunchKernelGGL(panel64_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, A, lda, w, m,
-                       col_base, info);
-    QMLH_CHECK_LAUNCH("panel64 launch");
+    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, A, lda, w, col_base, info);
+    QMLH_CHECK_LAUNCH("potf2 launch");
+    if (m > 0) {
+        hipLaunchKernelGGL(trsm_rows_kernel<T>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0,
+                           s, A, lda, w, m, info);
+        QMLH_CHECK_LAUNCH("panel rows launch");
+    }
     return QMLH_OK;
 }
 
@@ -350,15 +359,19 @@ __global__ __launch_bounds__(256) void trsm_multi_diag_kernel(const float* __res
     float* col = V + k0 + j * ldv;
     double x[PF_IB];
 #pragma unroll
+    for (int c = 0; c < PF_IB; ++c) x[c] = c < w ? (double)col[c] : 0.0;  // loads first
+#pragma unroll
     for (int c = 0; c < PF_IB; ++c) {
         if (c < w) {
-            double s = (double)col[c];
+            double s = x[c];
 #pragma unroll
             for (int l = 0; l < c; ++l) s = s - sL[c][l] * x[l];
             x[c] = s * sInv[c];
-            col[c] = (float)x[c];
         }
     }
+#pragma unroll
+    for (int c = 0; c < PF_IB; ++c)
+        if (c < w) col[c] = (float)x[c];
 }
 
 int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
