@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--serve-seconds", type=float, default=4.0)
     p.add_argument("--copy-mib", type=int, default=0,
                    help="run a device copy of this many MiB first (HBM counter calibration)")
+    p.add_argument("--kernels", type=int, default=1,
+                   help="time the kernel-matrix builds at the large shapes (0 = skip)")
     p.add_argument("--sweep", type=int, default=64,
                    help="problems in the batched learning-curve sweep (0 = skip)")
     return p.parse_args()
@@ -126,6 +128,57 @@ def roofline(timers, steps, factor):
             "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic,
             "algorithmic_flops_per_launch": flops / max(launches, 1),
             "launches_per_step": launches / steps, "ms_per_step": ms / steps}
+
+
+def run_kernels(log):
+    """Kernel-matrix builds at the large-configuration shapes: the Laplacian (FP64 VALU) on a
+    row block of 8192 queries against 100,000 training rows (size 29, d = 435, sigma = 724),
+    and the symmetric Gaussian (FP64 MFMA expanded form) at N = 20,000 (sigma = 4000)."""
+    import torch
+
+    from qml_amd import _lib, synthetic
+    from qml_amd.kernels import gaussian_mfma_device, pairwise_device, row_sqnorms_device
+    from qml_amd.representations import cm_length, generate_coulomb_matrices
+
+    out = {}
+    big, _, size = synthetic.workload("krr_large_100k", n_train=100000, n_test=0)
+    d = cm_length(size)
+    X = generate_coulomb_matrices(*big.ragged(), size=size)
+    rows = 8192
+    K = torch.empty((100000, rows), dtype=torch.float64, device=X.device)  # (rows, 100k) col-major
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    pairwise_device(_lib.PAIR_LAPLACIAN, X[:rows], rows, X, 100000, d, 724.0, False, K)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        pairwise_device(_lib.PAIR_LAPLACIAN, X[:rows], rows, X, 100000, d, 724.0, False, K)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    pf = rows * 100000.0 * d
+    out["laplacian_rowblock"] = {
+        "shape": f"{rows} x 100000, d={d}", "ms": ms, "pair_features_per_s": pf / (ms * 1e-3),
+        "frac_of_fp64_valu": pf / (ms * 1e-3) / 19.65e12,
+        "hbm_write_GBps": rows * 100000.0 * 8 / (ms * 1e-3) / 1e9}
+    del K
+    n = 20000
+    X2 = X[:n]
+    nr = row_sqnorms_device(X2, n, d)
+    K2 = torch.empty((n, n), dtype=torch.float64, device=X.device)
+    gaussian_mfma_device(X2, nr, n, X2, nr, n, d, 4000.0, True, K2)
+    e0.record()
+    for _ in range(reps):
+        gaussian_mfma_device(X2, nr, n, X2, nr, n, d, 4000.0, True, K2)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = n * (n + 1) / 2.0 * d * 2.0  # symmetric: the upper tiles' cross terms
+    out["gaussian_mfma_20k"] = {"shape": f"{n} x {n} symmetric, d={d}", "ms": ms,
+                                "tflops": flops / (ms * 1e-3) / 1e12,
+                                "frac_of_fp64_mfma": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
+    log(f"kernels: laplacian {out['laplacian_rowblock']['frac_of_fp64_valu']:.2f} of FP64 VALU, "
+        f"gaussian {out['gaussian_mfma_20k']['tflops']:.1f} TF/s")
+    return out
 
 
 def run_sweep(count, log):
@@ -316,6 +369,7 @@ def main():
             f"p99 {serve_out[tag]['p99_ms']:.2f} ms over {len(reqs)} requests")
 
     sweep_out = run_sweep(args.sweep, log) if args.sweep > 0 else None
+    kernels_out = run_kernels(log) if args.kernels else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -349,6 +403,7 @@ def main():
                     "mode": "sequential requests, host molecules in, predictions out"},
         "serve": serve_out or None,
         "sweep": sweep_out,
+        "kernels": kernels_out,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),
