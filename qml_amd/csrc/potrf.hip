@@ -3,7 +3,9 @@
 // Replaces LAPACK dpotrf("U") / dpotrs("U", n, 1) as called at qml/fcho_solve.f90:35,46 (see
 // the header for why "U" on the Fortran view is the lower triangle here).
 //
-// Factorisation, outer block nb (256; 512 with split-BF16 trailing updates), per step k:
+// Factorisation, outer block nb (256; 512 with split-BF16 trailing updates), per step k
+// (from order 4096 on with look-ahead: the next block column is factored on a second stream
+// while the rest of the trailing update runs, see potrf_lookahead):
 //   for each 64-wide sub-panel of the nb-wide block column (left to right):
 //     1. the 64 x 64 diagonal block is factored in the registers of one wavefront (FP64
 //        arithmetic), then every row of the column below is solved against it (256 rows per
@@ -138,6 +140,76 @@ static int launch_panel64(T* A, int64_t lda, int w, int64_t m, int64_t col_base,
 
 __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 
+// Factor the block column A[k:n, k:k+kb] (its trailing updates from earlier columns applied):
+// per 64-wide sub-panel the diagonal factor, the rows below, and the update of the rest of the
+// block column.
+template <typename T>
+static int factor_block_column(T* A, int64_t lda, int64_t n, int64_t k, int64_t kb, int32_t* info,
+                               hipStream_t s) {
+    for (int64_t jj = 0; jj < kb; jj += PF_IB) {
+        const int64_t c = k + jj;
+        const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
+        T* Acc = A + c + c * lda;
+        QMLH_TRY(launch_panel64<T>(Acc, lda, w, n - c - w, c, info, s));
+        const int64_t ncols = kb - jj - w;  // rest of this block column
+        if (ncols > 0)
+            QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, n - c - w, ncols, w,
+                                      Acc + w + w * lda, lda, info, s));
+    }
+    return QMLH_OK;
+}
+
+// Lower triangle of C (m x m) -= P P^T, P (m x kb) the panel below the block column.
+template <typename T>
+static int trailing_syrk(T* P, int64_t lda, int64_t m, int64_t kb, T* C, int32_t* info,
+                         hipStream_t s, void* bf16_ws) {
+    if (m <= 0) return QMLH_OK;
+    if (std::is_same<T, float>::value && bf16_ws)
+        return syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb,
+                           reinterpret_cast<float*>(C), lda, info, bf16_ws, s);
+    return launch_update<T>(UPD_SYRK, P, lda, P, lda, m, m, kb, C, lda, info, s);
+}
+
+// Matrices at least this large are factored with look-ahead.
+static constexpr int64_t PF_LOOKAHEAD_MIN = 4096;
+
+// Look-ahead schedule. Step k's trailing update is split into the next block column's part
+// (an MFMA update on the caller's stream) and the rest; the next block column is factored on a
+// second stream while the rest of the update runs, so the latency-bound panel work overlaps the
+// bandwidth/MFMA-bound trailing update. Events order the two streams; the caller's stream waits
+// for the last panel, so on return all work is ordered on `s`.
+template <typename T>
+static int potrf_lookahead(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
+                           void* bf16_ws, int64_t nb) {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_p = nullptr;
+    int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
+    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming), "potrf: event");
+    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_p, hipEventDisableTiming), "potrf: event");
+    if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n, 0, n < nb ? n : nb, info, s);
+    for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
+        const int64_t kb = n - k < nb ? n - k : nb;
+        const int64_t m = n - k - kb;
+        if (m <= 0) break;
+        const int64_t kb2 = m < nb ? m : nb;  // next block column
+        T* P = A + (k + kb) + k * lda;
+        T* C = P + kb * lda;  // A(k + kb, k + kb)
+        rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m, kb2, kb, C, lda, info, s);
+        if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
+        if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
+        if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n, k + kb, kb2, info, s2);
+        if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
+        if (rc == QMLH_OK && m > kb2)
+            rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s, bf16_ws);
+        if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
+    }
+    // Destroying them here is safe: pending work completes first, and `s` already waits for it.
+    if (ev_a) (void)hipEventDestroy(ev_a);
+    if (ev_p) (void)hipEventDestroy(ev_p);
+    if (s2) (void)hipStreamDestroy(s2);
+    return rc;
+}
+
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* bf16_ws) {
     if (n < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0");
@@ -150,30 +222,13 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     TimerScope ts(QMLH_TIMER_POTRF, (double)n * (double)n * (double)n / 3.0, s);
     const bool bf16 = std::is_same<T, float>::value && bf16_ws;
     const int64_t nb = bf16 ? PF_NB_BF16 : PF_NB;
+    if (n >= PF_LOOKAHEAD_MIN) return potrf_lookahead<T>(A, n, lda, info, s, bf16 ? bf16_ws : nullptr, nb);
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
-        for (int64_t jj = 0; jj < kb; jj += PF_IB) {
-            const int64_t c = k + jj;
-            const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
-            T* Acc = A + c + c * lda;
-            QMLH_TRY(launch_panel64<T>(Acc, lda, w, n - c - w, c, info, s));
-            const int64_t ncols = kb - jj - w;  // rest of this block column
-            if (ncols > 0)
-                QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, n - c - w, ncols,
-                                          w, Acc + w + w * lda, lda, info, s));
-        }
+        QMLH_TRY(factor_block_column<T>(A, lda, n, k, kb, info, s));
         const int64_t m = n - k - kb;
-        if (m > 0) {
-            T* P = A + (k + kb) + k * lda;
-            if (bf16) {
-                QMLH_TRY(syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb,
-                                     reinterpret_cast<float*>(P + kb * lda), lda, info, bf16_ws,
-                                     s));
-            } else {
-                QMLH_TRY(launch_update<T>(UPD_SYRK, P, lda, P, lda, m, m, kb, P + kb * lda, lda,
-                                          info, s));
-            }
-        }
+        T* P = A + (k + kb) + k * lda;
+        QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, bf16 ? bf16_ws : nullptr));
     }
     return QMLH_OK;
 }

@@ -28,7 +28,7 @@ def _gpu():
     _lib.lib()
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 200, 257, 777, 1500])
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 257, 777, 1500, 4600])
 def test_spotrf_fp32_factor_vs_lapack(n):
     from scipy.linalg import lapack
 
@@ -49,7 +49,7 @@ def test_spotrf_fp32_factor_vs_lapack(n):
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
-@pytest.mark.parametrize("n", [257, 700, 1500])
+@pytest.mark.parametrize("n", [257, 700, 1500, 5000])
 def test_spotrf_bf16x3_factor_accuracy(n):
     # Split-BF16 trailing updates: each update ~2^-16 relative, so the factor agrees with
     # LAPACK's to ~1e-5 normwise on a well-conditioned matrix.
@@ -75,18 +75,18 @@ def test_spotrf_bf16x3_factor_accuracy(n):
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
-def test_spotrf_info_matches_lapack():
+@pytest.mark.parametrize("n,bad", [(300, 130), (4400, 1000), (4400, 4390)])
+def test_spotrf_info_matches_lapack(n, bad):
     from scipy.linalg import lapack
 
     from qml_amd import _lib
 
-    n = 300
     A = np.eye(n, dtype=np.float32)
-    A[130, 130] = -1.0
+    A[bad, bad] = -1.0
     Ad = torch.from_numpy(A.copy()).cuda()
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
     _lib.check(_lib.lib().qmlh_spotrf(Ad.data_ptr(), n, n, info.data_ptr(), None))
-    assert int(info.item()) == lapack.dpotrf(A.astype(np.float64), lower=1)[1] == 131
+    assert int(info.item()) == lapack.dpotrf(A.astype(np.float64), lower=1)[1] == bad + 1
 
 
 def test_gaussian_deflated_build_identities():

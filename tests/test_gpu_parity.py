@@ -212,7 +212,8 @@ def test_gaussian_mfma_expanded_form_vs_direct():
 # --------------------------------------------------------------------------------------------
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 255, 256, 257, 513, 1100])
+# 4097 and 5000 run the look-ahead schedule (second stream for the next block column).
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 255, 256, 257, 513, 1100, 4097, 5000])
 def test_potrf_potrs_vs_lapack(n):
     from scipy.linalg import lapack
 
@@ -268,13 +269,13 @@ def test_potrs_single_launch_solves(n):
     assert np.linalg.norm(A @ outs[0] - y) <= 1e-13 * np.linalg.norm(A) * np.linalg.norm(outs[0])
 
 
-@pytest.mark.parametrize("bad", [0, 5, 64, 255, 256, 300])
-def test_potrf_info_matches_lapack(bad):
+@pytest.mark.parametrize("n,bad", [(400, 0), (400, 5), (400, 64), (400, 255), (400, 256),
+                                   (400, 300), (4500, 0), (4500, 300), (4500, 2100), (4500, 4499)])
+def test_potrf_info_matches_lapack(n, bad):
     from scipy.linalg import lapack
 
     from qml_amd.math import potrf_device
 
-    n = 400
     rng = np.random.default_rng(bad)
     M = rng.standard_normal((n, n))
     A = M @ M.T / n + np.eye(n)
