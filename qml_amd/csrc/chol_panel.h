@@ -11,6 +11,28 @@ static constexpr int CP_B = 64;  // diagonal block / sub-panel width
 
 static constexpr int CP_S = 16;  // step of the blocked diagonal factor
 
+// LDS (FP64) <- the lower triangle of the w x w block at A (column-major, lda), zero elsewhere;
+// TRANS stores element (i, j) at s[j][i]. All 256 lanes take part; each issues its 16 loads
+// back to back before any LDS store (a loop with a runtime bound would wait on every load).
+template <bool TRANS, int LDS_LD, typename T>
+__device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __restrict__ A,
+                                              int64_t lda, int w) {
+    const int t = threadIdx.x;
+    double v[CP_B * CP_B / 256];
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;  // consecutive lanes: consecutive rows
+        v[q] = (i < w && j < w && i >= j) ? (double)A[i + (int64_t)j * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        if (TRANS) s[j][i] = v[q]; else s[i][j] = v[q];
+    }
+}
+
 // Cholesky of the w x w (w <= 64) lower block in s (FP64, LDS), all 256 lanes, in four
 // 16-wide steps: the 16 x 16 diagonal block is factored in the registers of lanes 0..15 of
 // wave 0 (pivot and column entries broadcast by lane shuffles), the rows below it are solved
@@ -133,15 +155,14 @@ __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w
     __shared__ double s[CP_B][CP_B + 1];
     if (*info) return;
     const int t = threadIdx.x;
-    for (int idx = t; idx < w * w; idx += 256) {
-        const int i = idx % w, j = idx / w;
-        if (i >= j) s[i][j] = (double)A[i + j * lda];
-    }
+    stage_lower64<false, CP_B + 1>(s, A, lda, w);
     __syncthreads();
     const int fail = potf2_block64(s, w);
-    for (int idx = t; idx < w * w; idx += 256) {
-        const int i = idx % w, j = idx / w;
-        if (i >= j) A[i + j * lda] = (T)s[i][j];
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        if (i < w && j < w && i >= j) A[i + (int64_t)j * lda] = (T)s[i][j];
     }
     if (t == 0 && fail) *info = (int32_t)(col_base + fail);
 }
@@ -156,10 +177,7 @@ __device__ __forceinline__ void trsm_body(T* __restrict__ A, int64_t lda, int w,
     __shared__ double inv[CP_B];
     if (*info) return;
     const int t = threadIdx.x;
-    for (int idx = t; idx < CP_B * CP_B; idx += 256) {
-        const int j = idx % CP_B, c = idx / CP_B;  // column-major: consecutive lanes, rows j
-        sT[c][j] = (j < w && c < w && j >= c) ? (double)A[j + c * lda] : 0.0;
-    }
+    stage_lower64<true, CP_B + 2>(sT, A, lda, w);
     __syncthreads();
     if (t < w) inv[t] = 1.0 / sT[t][t];
     __syncthreads();

@@ -125,42 +125,41 @@ __global__ __launch_bounds__(256, 2) void gaussian_deflated_kernel(
     Operand<double, LAYOUT_ROW> op{X, ldx, n, d};
     WaveTile<double> t;
     wave_tile_zero(t);
-    wg_tile_mma_lds(t, lds, op, op, bj * GK_TILE, bi * GK_TILE, d);
+    // First operand: block bi (the column index i of the lower-triangle entry), second: block
+    // bj (its row index j), so the 16 lanes of an accumulator element run down 16 consecutive
+    // rows j of the column-major outputs.
+    wg_tile_mma_lds(t, lds, op, op, bi * GK_TILE, bj * GK_TILE, d);
 
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t col0 = bj * GK_TILE + 64 * (w & 1);
-    const int64_t row0 = bi * GK_TILE + 64 * (w >> 1);
-    if (row0 >= n || col0 >= n) return;
+    const int64_t i0 = bi * GK_TILE + 64 * (w & 1);
+    const int64_t j0 = bj * GK_TILE + 64 * (w >> 1);
+    if (i0 >= n || j0 >= n || i0 > j0 + 63) return;
     const double inv_s2 = -2.0 * scale;  // 1 / sigma^2
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int b = 0; b < 4; ++b) {
+        const int64_t j = j0 + 16 * b + (lane & 15);  // row of the lower triangle
+        if (j >= n) continue;
+        const double nj = nrm[j], uj = u[j];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t i = row0 + 16 * b + (lane & 15);  // i <= j: row index of the upper half
-            if (i >= n) continue;
-            const double ni = nrm[i], ui = u[i];
+        for (int a = 0; a < 4; ++a) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t j = col0 + tile_row<double>(a, e);
-                if (j >= n || i > j) continue;
+                const int64_t i = i0 + tile_row<double>(a, e);  // column, i <= j
+                if (i > j) continue;
                 const double c = t.acc[a][b][e];
                 double kv, bv;
                 const double xv = c * inv_s2;
                 if (i == j) {
                     kv = 1.0 + lambda;
-                    bv = ui * ui * (expm1(xv) - xv) + (lambda + shift);
+                    bv = uj * uj * (expm1(xv) - xv) + (lambda + shift);
                 } else {
-                    double d2 = ni + nrm[j] - 2.0 * c;
+                    double d2 = nrm[i] + nj - 2.0 * c;
                     d2 = d2 > 0.0 ? d2 : 0.0;
                     kv = exp(scale * d2);
-                    bv = ui * u[j] * (expm1(xv) - xv);
+                    bv = u[i] * uj * (expm1(xv) - xv);
                 }
-                if (K) {
-                    K[i + j * ldk] = kv;
-                    K[j + i * ldk] = kv;
-                }
-                // B32 keeps the lower triangle (row >= column): element (j, i).
+                if (K) K[j + i * ldk] = kv;
                 B32[j + i * ldb] = (float)bv;
             }
         }

@@ -52,17 +52,111 @@ __global__ __launch_bounds__(256) void dot_final_kernel(const double* __restrict
 
 // y_j = sum_i A(i, j) x_i (column dots; A symmetric so this is A x). One wavefront per
 // column, lanes stride down the column, fixed-order butterfly.
-__global__ __launch_bounds__(256) void symv_cols_kernel(const double* __restrict__ A, int64_t lda,
-                                                        int64_t n, const double* __restrict__ x,
-                                                        double* __restrict__ y) {
-    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (j >= n) return;
-    const double* col = A + j * lda;
+// ------------------------------------------------------------------------------------------
+// y = A x for symmetric A reading only its lower triangle (column-major), deterministic.
+// Workgroup (strip J, chunk) takes the 64 columns 64 J .. and up to SY_CHUNK row blocks of 64
+// from block J + SY_CHUNK chunk down. Each 64 x 64 tile is staged in LDS once and used twice:
+// row sums sum_c A(r, c) x_c (c <= r on the diagonal tile) go to part_r[J][row] -- one
+// partial per (strip, row) --, column sums sum_r A(r, c) x_r (r > c on the diagonal tile)
+// accumulate over the chunk into part_c[J][chunk][c]. symv_reduce_kernel adds, for every i,
+// the row partials of strips 0 .. i/64 and the column partials of strip i/64, in a fixed order.
+// ------------------------------------------------------------------------------------------
+static constexpr int SY_B = 64;
+static constexpr int SY_CHUNK = 8;
+
+__host__ __device__ inline int64_t symv_blocks(int64_t n) { return (n + SY_B - 1) / SY_B; }
+__host__ __device__ inline int64_t symv_chunks(int64_t n) {
+    return (symv_blocks(n) + SY_CHUNK - 1) / SY_CHUNK;
+}
+// doubles of workspace: row partials + column partials
+static inline size_t symv_ws_doubles(int64_t n) {
+    return (size_t)(symv_blocks(n) * n + symv_blocks(n) * symv_chunks(n) * SY_B);
+}
+
+__global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restrict__ A, int64_t lda,
+                                                         int64_t n, const double* __restrict__ x,
+                                                         double* __restrict__ part_r,
+                                                         double* __restrict__ part_c) {
+    __shared__ double tile[SY_B][SY_B + 1];  // tile[r][c] = A(row0 + r, col0 + c)
+    __shared__ double xj[SY_B], xr[SY_B];
+    __shared__ double redr[4][SY_B];
+    __shared__ double redc[4][SY_B];
+    const int64_t J = blockIdx.y, chunk = blockIdx.x;
+    const int64_t nblk = symv_blocks(n);
+    const int64_t first = J + chunk * SY_CHUNK;
+    if (first >= nblk) return;
+    const int64_t last = first + SY_CHUNK < nblk ? first + SY_CHUNK : nblk;
+    const int t = threadIdx.x;
+    const int lane = t & 63, q = t >> 6;
+    const int64_t col0 = J * SY_B;
+    if (t < SY_B) xj[t] = col0 + t < n ? x[col0 + t] : 0.0;
+    double csum = 0.0;  // column lane, rows 16 q .. 16 q + 15 of every tile, in tile order
+    for (int64_t blk = first; blk < last; ++blk) {
+        const int64_t row0 = blk * SY_B;
+        const bool diag = blk == J;
+        double v[SY_B * SY_B / 256];
+#pragma unroll
+        for (int k = 0; k < SY_B * SY_B / 256; ++k) {
+            const int idx = t + 256 * k;
+            const int r = idx & (SY_B - 1), c = idx / SY_B;  // consecutive lanes: consecutive rows
+            const bool ok = row0 + r < n && col0 + c < n && (!diag || r >= c);
+            v[k] = ok ? A[(row0 + r) + (col0 + c) * lda] : 0.0;
+        }
+        __syncthreads();  // previous tile's readers are done
+#pragma unroll
+        for (int k = 0; k < SY_B * SY_B / 256; ++k) {
+            const int idx = t + 256 * k;
+            tile[idx & (SY_B - 1)][idx / SY_B] = v[k];
+        }
+        if (t < SY_B) xr[t] = row0 + t < n ? x[row0 + t] : 0.0;
+        __syncthreads();
+        // row part: lane = row, columns 16 q ..
+        double rs = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rs = fma(tile[lane][16 * q + c], xj[16 * q + c], rs);
+        redr[q][lane] = rs;
+        // column part: lane = column, rows 16 q .. (strictly below the diagonal on the diagonal tile)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rr = 16 * q + r;
+            const double a = (diag && rr <= lane) ? 0.0 : tile[rr][lane];
+            csum = fma(a, xr[rr], csum);
+        }
+        __syncthreads();
+        if (t < SY_B && row0 + t < n)
+            part_r[J * n + row0 + t] = ((redr[0][t] + redr[1][t]) + redr[2][t]) + redr[3][t];
+    }
+    redc[q][lane] = csum;
+    __syncthreads();
+    if (t < SY_B)
+        part_c[(J * symv_chunks(n) + chunk) * SY_B + t] =
+            ((redc[0][t] + redc[1][t]) + redc[2][t]) + redc[3][t];
+}
+
+__global__ void symv_reduce_kernel(const double* __restrict__ part_r,
+                                   const double* __restrict__ part_c, int64_t n,
+                                   double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t I = i / SY_B;
     double s = 0.0;
-    for (int64_t i = lane; i < n; i += 64) s = fma(col[i], x[i], s);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[j] = s;
+    for (int64_t J = 0; J <= I; ++J) s += part_r[J * n + i];
+    const int64_t nch = (symv_blocks(n) - I + SY_CHUNK - 1) / SY_CHUNK;  // chunks of strip I
+    const int64_t base = I * symv_chunks(n) * SY_B + (i - I * SY_B);
+    for (int64_t c = 0; c < nch; ++c) s += part_c[base + c * SY_B];
+    y[i] = s;
+}
+
+static int symv_lower(const double* A, int64_t lda, int64_t n, const double* x, double* y,
+                      double* ws, hipStream_t s) {
+    double* part_r = ws;
+    double* part_c = ws + symv_blocks(n) * n;
+    hipLaunchKernelGGL(symv_lower_kernel, dim3((unsigned)symv_chunks(n), (unsigned)symv_blocks(n)),
+                       dim3(256), 0, s, A, lda, n, x, part_r, part_c);
+    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                       part_r, part_c, n, y);
+    QMLH_CHECK_LAUNCH("pcg symv launch");
+    return QMLH_OK;
 }
 
 // sc layout: 0 rz, 1 pq, 2 rr, 3 (unused), 4 u.z, 5 den = 1 + u.Bu, 6 rz_new
@@ -267,6 +361,7 @@ extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
     if (r < 0) r = 0;
     return (size_t)(5 * n + 2 * r + qmlh::RED_BLOCKS + 8) * sizeof(double) +
            qmlh::trsv_dinv_doubles(n) * sizeof(double) +
+           qmlh::symv_ws_doubles(n) * sizeof(double) +
            qmlh::trsv_sync_ints(n) * sizeof(int32_t);
 }
 
@@ -297,7 +392,8 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     double* partial = tmp_r + r;
     double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 6 rz_new
     double* dinv = sc + 8;
-    int32_t* sync = reinterpret_cast<int32_t*>(dinv + trsv_dinv_doubles(n));
+    double* symws = dinv + trsv_dinv_doubles(n);
+    int32_t* sync = reinterpret_cast<int32_t*>(symws + symv_ws_doubles(n));
     PcgCtx c{s, n, partial, sync, dinv};
     Woodbury wb{Y, ldy, r, C, ldc, t, tmp_r};
     // The preconditioner's triangular solves use explicit inverses of the diagonal blocks of
@@ -323,9 +419,7 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     int it = 0;
     while (it < max_iter) {
         ++it;
-        hipLaunchKernelGGL(symv_cols_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, s, A,
-                           lda, n, p, q);
-        QMLH_CHECK_LAUNCH("pcg symv launch");
+        QMLH_TRY(symv_lower(A, lda, n, p, q, symws, s));
         QMLH_TRY(dot(c, p, q, sc + 1));
         hipLaunchKernelGGL(axpy2_kernel, dim3(grid1(n)), dim3(256), 0, s, x, rv, p, q, sc, n);
         QMLH_CHECK_LAUNCH("pcg axpy launch");

@@ -246,10 +246,7 @@ template <typename T>
 __device__ __forceinline__ void load_diag_block(double (*sL)[PF_IB + 1], double* sInv,
                                                 const T* __restrict__ L, int64_t ldl, int64_t k0,
                                                 int w) {
-    for (int idx = threadIdx.x; idx < w * PF_IB; idx += blockDim.x) {
-        const int i = idx % PF_IB, j = idx / PF_IB;
-        if (i < w && i >= j) sL[i][j] = (double)L[(k0 + i) + (k0 + j) * ldl];
-    }
+    stage_lower64<false, PF_IB + 1>(sL, L + k0 + k0 * ldl, ldl, w);
     __syncthreads();
     if (threadIdx.x < w) sInv[threadIdx.x] = 1.0 / sL[threadIdx.x][threadIdx.x];
     __syncthreads();

@@ -11,7 +11,7 @@
 //
 // The accumulation order is fixed (finished blocks in solve order, a fixed intra-thread
 // order, a fixed final reduction tree), so results are deterministic.
-#include "common.h"
+#include "chol_panel.h"
 
 namespace qmlh {
 
@@ -81,18 +81,20 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
 
     // The diagonal block does not depend on other blocks: stage it first.
     if (dinv) {
-        // inverse block: FWD x = D s, BWD x = D^T s; lane (i, part) will use row/column i,
-        // entries 16 part .. 16 part + 15
+        // inverse block: FWD x = D s, BWD x = D^T s (lane (i, part) later uses row i, entries
+        // 16 part .. 16 part + 15). Loads first, then the LDS stores.
         const double* D = dinv + b * (TS_B * TS_B);
-        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
-            const int i = idx % TS_B, jj = idx / TS_B;
-            sL[i][jj] = FWD ? D[i + jj * TS_B] : D[jj + i * TS_B];
+        double v[TS_B * TS_B / 256];
+#pragma unroll
+        for (int q = 0; q < TS_B * TS_B / 256; ++q) v[q] = D[t + 256 * q];
+#pragma unroll
+        for (int q = 0; q < TS_B * TS_B / 256; ++q) {
+            const int idx = t + 256 * q;
+            const int i = idx % TS_B, jj = idx / TS_B;  // D(i, jj)
+            if (FWD) sL[i][jj] = v[q]; else sL[jj][i] = v[q];
         }
     } else {
-        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
-            const int i = idx % TS_B, jj = idx / TS_B;
-            if (i < w && jj < w && i >= jj) sL[i][jj] = (double)L[(r0 + i) + (r0 + jj) * ldl];
-        }
+        stage_lower64<false, TS_B + 1>(sL, L + r0 + r0 * ldl, ldl, w);
     }
     __syncthreads();  // s_fail initialised before any lane may set it
 
