@@ -132,7 +132,7 @@ int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stre
  * qmlh_gaussian_deflated: from one FP64 MFMA pass over X (n, ldx), with norms from
  * qmlh_row_sqnorms, s = -0.5/sigma^2 and c = x_i.x_j, writes
  *   u_i = exp(s * norms_i)                                               (n FP64)
- *   K(i, j) = exp(s * max(n_i + n_j - 2c, 0)) + lambda [i==j]      (full, FP64; K may be NULL)
+ *   K(i, j) = exp(s * max(n_i + n_j - 2c, 0)) + lambda [i==j]     (lower, FP64; K may be NULL)
  *   B32(i, j) = u_i u_j (expm1(c/sigma^2) - c/sigma^2) + (lambda + shift) [i==j] (lower, FP32)
  *   V(i, :) = u_i [1, x_i / sigma]            (n x (d+1), column-major FP32; V may be NULL)
  * so that K + lambda I = V V^T + B32 - shift I exactly: the constant and linear terms of
@@ -140,9 +140,11 @@ int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stre
  * small entries that a low-precision factorisation resolves relative to lambda.
  *
  * qmlh_woodbury_setup: for the qmlh_spotrf factor L of B32 and V (n x r, r <= 1024),
- * overwrites V with Y = L^-1 V and writes C = chol(I + Y^T Y) (r x r, lower, FP64).
+ * overwrites V with Y = L^-1 V and writes C = chol(I + Y^T Y) (r x r, lower, FP64); ws holds
+ * qmlh_woodbury_workspace(n) bytes (the FP64 inverses of L's 64 x 64 diagonal blocks).
  *
- * qmlh_pcg_solve: FP64 preconditioned CG on A x = y (A symmetric, column-major, full), with
+ * qmlh_pcg_solve: FP64 preconditioned CG on A x = y (A symmetric, column-major; only its lower
+ * triangle is read), with
  * preconditioner M = L L^T + V V^T applied as M^-1 r = L^-T (q - Y C^-T C^-1 Y^T q),
  * q = L^-1 r (r = 0: M = L L^T). Iterates until |y - A x| <= tol |y| or max_iter and reports
  * the iterations and the final relative residual. Synchronises the stream once per iteration.
@@ -159,8 +161,10 @@ int qmlh_gaussian_deflated(const double *X, int64_t n, int64_t ldx, const double
                            int64_t d, double sigma, double lambda, double shift, double *K,
                            int64_t ldk, float *B32, int64_t ldb, float *V, int64_t ldv,
                            double *u, qmlh_stream_t stream);
+size_t qmlh_woodbury_workspace(int64_t n);
 int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_t ldv,
-                        int64_t r, double *C, int64_t ldc, int32_t *info, qmlh_stream_t stream);
+                        int64_t r, double *C, int64_t ldc, int32_t *info, void *ws,
+                        size_t ws_bytes, qmlh_stream_t stream);
 /* B(i, j) = (float)(A(i, j) + shift [i == j]) on the lower triangle (column-major). */
 int qmlh_lower_to_f32(const double *A, int64_t n, int64_t lda, float *B, int64_t ldb,
                       double shift, qmlh_stream_t stream);

@@ -63,8 +63,9 @@ _SIGNATURES = {
     "qmlh_spotrf_bf16x3": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "qmlh_gaussian_deflated": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _vp,
                                               _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "qmlh_woodbury_workspace": (_sz, [_i64]),
     "qmlh_woodbury_setup": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp,
-                                           _vp]),
+                                           _vp, _sz, _vp]),
     "qmlh_lower_to_f32": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _vp]),
     "qmlh_pcg_workspace": (_sz, [_i64, _i64]),
     "qmlh_pcg_solve": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _i64,

@@ -11,7 +11,7 @@
 // exactly symmetric with unit diagonal.
 //
 // The deflated build (mixed-precision KRR) writes, from the same cross term c = x_i.x_j:
-//   K64(i, j) = exp(scale * max(n_i + n_j - 2c, 0)) + lambda [i == j]              (full, FP64)
+//   K64(i, j) = exp(scale * max(n_i + n_j - 2c, 0)) + lambda [i == j]             (lower, FP64)
 //   B32(i, j) = u_i u_j (expm1(c / s2) - c / s2) + (lambda + shift) [i == j]     (lower, FP32)
 //   V(i, :)   = u_i [1, x_i / sigma]                                  (n x (d+1), FP32)
 // with s2 = sigma^2 and u_i = exp(scale * n_i). Since exp(c/s2) = 1 + c/s2 + (expm1(c/s2) -

@@ -337,9 +337,13 @@ extern "C" int qmlh_lower_to_f32(const double* A, int64_t n, int64_t lda, float*
     return QMLH_OK;
 }
 
+extern "C" size_t qmlh_woodbury_workspace(int64_t n) {
+    return n > 0 ? qmlh::trsv_dinv_doubles(n) * sizeof(double) : 0;
+}
+
 extern "C" int qmlh_woodbury_setup(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
-                                   int64_t r, double* C, int64_t ldc, int32_t* info,
-                                   qmlh_stream_t stream) {
+                                   int64_t r, double* C, int64_t ldc, int32_t* info, void* ws,
+                                   size_t ws_bytes, qmlh_stream_t stream) {
     using namespace qmlh;
     if (n < 0 || r < 0 || ldl < n || ldv < n || ldc < r)
         return set_error(QMLH_ERR_ARG, "woodbury_setup: bad shape");
@@ -347,9 +351,13 @@ extern "C" int qmlh_woodbury_setup(const float* L, int64_t n, int64_t ldl, float
         return set_error(QMLH_ERR_ARG, "woodbury_setup: rank %lld above %d", (long long)r,
                          WB_MAX_RANK);
     if (n == 0 || r == 0) return QMLH_OK;
-    if (!L || !V || !C || !info) return set_error(QMLH_ERR_ARG, "woodbury_setup: null pointer");
+    if (!L || !V || !C || !info || !ws) return set_error(QMLH_ERR_ARG, "woodbury_setup: null pointer");
+    if (ws_bytes < qmlh_woodbury_workspace(n))
+        return set_error(QMLH_ERR_WORKSPACE, "woodbury_setup: workspace too small");
     hipStream_t s = (hipStream_t)stream;
-    QMLH_TRY(trsm_lower_fwd_multi(L, n, ldl, V, ldv, r, s));
+    double* dinv = static_cast<double*>(ws);
+    QMLH_TRY(trsv_diag_inverse<float>(L, n, ldl, dinv, s));
+    QMLH_TRY(trsm_lower_fwd_multi(L, n, ldl, V, ldv, r, dinv, s));
     dim3 g((unsigned)ceil_div(r, 16), (unsigned)ceil_div(r, 16));
     hipLaunchKernelGGL(gram_kernel, g, dim3(256), 0, s, V, ldv, n, r, C, ldc);
     QMLH_CHECK_LAUNCH("woodbury gram launch");

@@ -395,49 +395,77 @@ int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32
 }
 
 // ------------------------------------------------------------------------------------------
-// Multiple right-hand sides: V := L^-1 V, 64-row steps. A step solves the 64 x nrhs block
-// (one lane per right-hand side, FP64 substitution against the diagonal block in LDS), then
-// updates the rows below with an FP32 MFMA product.
+// Multiple right-hand sides: V := L^-1 V (V FP32, n x nrhs column-major), right-looking in
+// 256-row blocks. Within a block, each 64-row step multiplies by the FP64 inverse of the 64 x 64
+// diagonal block (trsv_diag_inverse) and updates the rest of the block; then one FP32 MFMA
+// product updates every row below the 256-row block, so V streams through HBM once per 256
+// rows instead of once per 64.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void trsm_multi_diag_kernel(const float* __restrict__ L,
-                                                              int64_t ldl, int64_t k0, int w,
-                                                              float* __restrict__ V, int64_t ldv,
-                                                              int64_t nrhs) {
-    __shared__ double sL[PF_IB][PF_IB + 1];
-    __shared__ double sInv[PF_IB];
-    load_diag_block(sL, sInv, L, ldl, k0, w);
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nrhs) return;
-    float* col = V + k0 + j * ldv;
-    double x[PF_IB];
+// V[0:w, j0 .. j0+63] := D V[0:w, j0 ..] with D the 64 x 64 FP64 inverse (zero outside w).
+__global__ __launch_bounds__(256) void dinv_apply_kernel(const double* __restrict__ D,
+                                                         float* __restrict__ V, int64_t ldv,
+                                                         int w, int64_t nrhs) {
+    __shared__ double sD[PF_IB][PF_IB + 1];  // sD[i][c] = D(i, c)
+    __shared__ double sV[PF_IB][PF_IB + 1];  // sV[c][j] = V(c, j0 + j)
+    const int t = threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.x * PF_IB;
+    double dv[PF_IB * PF_IB / 256], vv[PF_IB * PF_IB / 256];
 #pragma unroll
-    for (int c = 0; c < PF_IB; ++c) x[c] = c < w ? (double)col[c] : 0.0;  // loads first
-#pragma unroll
-    for (int c = 0; c < PF_IB; ++c) {
-        if (c < w) {
-            double s = x[c];
-#pragma unroll
-            for (int l = 0; l < c; ++l) s = s - sL[c][l] * x[l];
-            x[c] = s * sInv[c];
-        }
+    for (int k = 0; k < PF_IB * PF_IB / 256; ++k) {
+        const int idx = t + 256 * k;
+        const int r = idx & (PF_IB - 1), c = idx / PF_IB;  // consecutive lanes: consecutive rows
+        dv[k] = D[idx];
+        vv[k] = (r < w && j0 + c < nrhs) ? (double)V[r + (j0 + c) * ldv] : 0.0;
     }
 #pragma unroll
-    for (int c = 0; c < PF_IB; ++c)
-        if (c < w) col[c] = (float)x[c];
+    for (int k = 0; k < PF_IB * PF_IB / 256; ++k) {
+        const int idx = t + 256 * k;
+        const int r = idx & (PF_IB - 1), c = idx / PF_IB;
+        sD[r][c] = dv[k];
+        sV[r][c] = vv[k];
+    }
+    __syncthreads();
+    const int i = t & (PF_IB - 1), q = t >> 6;  // row i, columns 16 q ..
+    double o[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) o[jj] = 0.0;
+    for (int c = 0; c < PF_IB; ++c) {
+        const double dic = sD[i][c];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) o[jj] = fma(dic, sV[c][16 * q + jj], o[jj]);
+    }
+    if (i < w) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int64_t j = j0 + 16 * q + jj;
+            if (j < nrhs) V[i + j * ldv] = (float)o[jj];
+        }
+    }
 }
 
 int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
-                         int64_t nrhs, hipStream_t s) {
+                         int64_t nrhs, const double* dinv, hipStream_t s) {
     if (n <= 0 || nrhs <= 0) return QMLH_OK;
-    for (int64_t k = 0; k < n; k += PF_IB) {
-        const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
-        hipLaunchKernelGGL(trsm_multi_diag_kernel, dim3((unsigned)ceil_div(nrhs, 256)), dim3(256),
-                           0, s, L, ldl, k, w, V, ldv, nrhs);
-        QMLH_CHECK_LAUNCH("trsm multi diag launch");
-        const int64_t rest = n - k - w;
-        // V[k+w:, :] -= L[k+w:, k:k+w] * V[k:k+w, :]; B(j, c) = V[(k + c) + j ldv] (row layout).
-        QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (k + w) + k * ldl, ldl, V + k, ldv,
-                                                   rest, nrhs, w, V + (k + w), ldv, nullptr, s)));
+    constexpr int64_t NBM = 256;
+    for (int64_t k = 0; k < n; k += NBM) {
+        const int64_t kb = n - k < NBM ? n - k : NBM;
+        for (int64_t jj = 0; jj < kb; jj += PF_IB) {
+            const int64_t c = k + jj;
+            const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
+            hipLaunchKernelGGL(dinv_apply_kernel, dim3((unsigned)ceil_div(nrhs, PF_IB)), dim3(256),
+                               0, s, dinv + (c / PF_IB) * (PF_IB * PF_IB), V + c, ldv, w, nrhs);
+            QMLH_CHECK_LAUNCH("trsm multi diagonal launch");
+            // rest of the 256-row block: V[c+w : k+kb] -= L[c+w : k+kb, c : c+w] V[c : c+w];
+            // B(j, kk) = V[(c + kk) + j ldv] (row layout).
+            const int64_t rest_in = kb - jj - w;
+            QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (c + w) + c * ldl, ldl, V + c,
+                                                       ldv, rest_in, nrhs, w, V + (c + w), ldv,
+                                                       nullptr, s)));
+        }
+        const int64_t rest = n - k - kb;
+        QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (k + kb) + k * ldl, ldl, V + k,
+                                                   ldv, rest, nrhs, kb, V + (k + kb), ldv, nullptr,
+                                                   s)));
     }
     return QMLH_OK;
 }

@@ -167,8 +167,11 @@ class KRR:
                 raise CholeskyError(f"FP32 factorisation failed at every shift: info={rep.info}")
             return rep
         if r:
+            wws = torch.empty(max(1, L.qmlh_woodbury_workspace(n) // 8), dtype=torch.float64,
+                              device=dev)
             _lib.check(L.qmlh_woodbury_setup(_lib.ptr(B), n, n, _lib.ptr(V), n, r, _lib.ptr(C), r,
-                                             _lib.ptr(info), h))
+                                             _lib.ptr(info), _lib.ptr(wws), wws.numel() * 8, h))
+            del wws
             if _lib.host_int(info, stream) != 0:
                 raise CholeskyError("Woodbury capacitance matrix is not positive definite")
         alpha = torch.zeros(n, dtype=torch.float64, device=dev)

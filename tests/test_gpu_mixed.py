@@ -163,11 +163,11 @@ def test_pcg_solve_random_spd(n):
     assert np.linalg.norm(x.cpu().numpy() - xs) <= 1e-11 * np.linalg.norm(xs)
 
 
-def test_woodbury_preconditioner_is_exact_for_exact_factor():
+@pytest.mark.parametrize("n,r", [(700, 9), (1333, 150)])
+def test_woodbury_preconditioner_is_exact_for_exact_factor(n, r):
     # With an exact factor of B, M = B + V V^T equals A, so PCG converges in one iteration.
     from qml_amd import _lib
 
-    n, r = 700, 9
     rng = np.random.default_rng(4)
     M = rng.standard_normal((n, n)) / np.sqrt(n)
     Bm = M @ M.T + 0.5 * np.eye(n)
@@ -180,9 +180,17 @@ def test_woodbury_preconditioner_is_exact_for_exact_factor():
     _lib.check(L.qmlh_spotrf(Bd.data_ptr(), n, n, info.data_ptr(), None))
     Vd = torch.from_numpy(np.ascontiguousarray(Vm.T).astype(np.float32)).cuda()  # column-major
     Cd = torch.empty((r, r), dtype=torch.float64, device="cuda")
+    wws = torch.empty(L.qmlh_woodbury_workspace(n) // 8 + 1, dtype=torch.float64, device="cuda")
     _lib.check(L.qmlh_woodbury_setup(Bd.data_ptr(), n, n, Vd.data_ptr(), n, r, Cd.data_ptr(), r,
-                                     info.data_ptr(), None))
+                                     info.data_ptr(), wws.data_ptr(), wws.numel() * 8, None))
     assert int(info.item()) == 0
+    # Y = L^-1 V against a host triangular solve with the same (FP32) factor
+    from scipy.linalg import solve_triangular
+
+    Lf = np.tril(Bd.cpu().numpy().T.astype(np.float64))
+    Yref = solve_triangular(Lf, Vm.astype(np.float32).astype(np.float64), lower=True)
+    Yg = Vd.cpu().numpy().T.astype(np.float64)
+    assert np.linalg.norm(Yg - Yref) <= 1e-5 * np.linalg.norm(Yref)
     Ad = torch.from_numpy(A.copy()).cuda()
     yd = torch.from_numpy(y).cuda()
     x = torch.empty(n, dtype=torch.float64, device="cuda")
