@@ -4,8 +4,13 @@
 // blocks are claimed in launch order and a block only ever waits for blocks already claimed
 // by running workgroups (no deadlock, whatever the dispatch order). It accumulates the
 // contributions of the finished blocks as they are published -- loading the L tile before
-// waiting on the block's flag, so the loads overlap the wait -- then solves its 64 x 64
-// diagonal block in FP64 and publishes the result (stores, agent-scope release, flag).
+// waiting, so the loads overlap the wait -- then solves its 64 x 64 diagonal block in FP64
+// and publishes the result.
+//
+// Publication needs no flag: before the solve the output vector is filled with a NaN bit
+// pattern that arithmetic never produces (TS_PENDING), each solved value is written with one
+// 64-bit atomic store, and a consumer polls the values it needs with 64-bit atomic loads until
+// none is pending. One memory round trip per dependency and no release fence on the producer.
 // Every wait is bounded: past the spin limit the workgroup records an error and stops
 // waiting, so the kernel always drains; the host turns the error into a failed status.
 //
@@ -16,28 +21,36 @@
 namespace qmlh {
 
 static constexpr int TS_B = 64;
-static constexpr unsigned TS_SPIN_LIMIT = 1u << 24;
+static constexpr unsigned TS_SPIN_LIMIT = 1u << 22;
 
-// sync: [0] ticket, [1] error, [2 .. 2 + nblocks) flags; cleared before each solve.
-__global__ void trsv_sync_clear_kernel(int32_t* sync, int64_t count) {
+// Quiet NaN with a payload no arithmetic produces (computed NaNs are the canonical
+// 0x7ff8000000000000; the timeout path below writes that one).
+static constexpr unsigned long long TS_PENDING = 0x7ffcdeadbeef0001ull;
+
+// sync: [0] ticket, [1] error (the remaining words are unused); out[0 .. n) := pending.
+__global__ void trsv_prepare_kernel(int32_t* __restrict__ sync, int64_t count,
+                                    double* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) sync[i] = 0;
+    if (i < n) out[i] = __builtin_bit_cast(double, TS_PENDING);
 }
 
-// Spin (bounded) until the block's flag is set. The flag and the published values are read
-// with agent-scope atomic loads, which go to the point of coherence, instead of an
-// agent-scope acquire fence: such a fence invalidates the whole L2 of the XCD and would throw
-// away the tiles every other workgroup has in flight.
-__device__ __forceinline__ bool wait_flag(int32_t* flag, int32_t* err) {
+// The published value at p, waiting (bounded) while it is pending. Read with an agent-scope
+// atomic load, which goes to the point of coherence, instead of an agent-scope acquire fence:
+// such a fence invalidates the whole L2 of the XCD and would throw away the tiles every other
+// workgroup has in flight. On timeout sets *err, clears ok and returns 0.
+__device__ __forceinline__ double wait_value(double* p, int32_t* err, bool& ok) {
     unsigned spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    while (true) {
+        const double v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__builtin_bit_cast(unsigned long long, v) != TS_PENDING) return v;
         if (++spins > TS_SPIN_LIMIT) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
+            ok = false;
+            return 0.0;
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    return true;
 }
 
 template <typename T, bool FWD>
@@ -56,15 +69,14 @@ __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L,
     }
 }
 
-// Solve block b of one system: wait for the blocks it depends on (flags[j] of the earlier
+// Solve block b of one system: wait for the values of the blocks it depends on (the earlier
 // blocks in solve order), accumulate their contributions, solve the diagonal block, publish.
 // `err` is the shared error word. With dinv (the FP64 inverses of the 64 x 64 diagonal blocks,
 // trsv_diag_inverse) the diagonal solve is a matrix-vector product spread over all lanes
 // instead of a 64-step substitution on the critical path. All 256 lanes call this.
 template <typename T, bool FWD>
 __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl, int64_t n,
-                                           const double* __restrict__ rhs,
-                                           double* __restrict__ out, int32_t* __restrict__ flags,
+                                           const double* __restrict__ rhs, double* out,
                                            int32_t* __restrict__ err, int64_t b,
                                            const double* __restrict__ dinv) {
     __shared__ double sL[TS_B][TS_B + 1];
@@ -111,23 +123,36 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     for (int64_t q = 0; q < nj; ++q) {
         const int64_t j = FWD ? q : nblocks - 1 - q;
         if (q + 1 < nj) load_tile16<T, FWD>(ln, L, ldl, n, r0, w, FWD ? q + 1 : j - 1, lane, grp);
-        if (ok) ok = wait_flag(&flags[j], err);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int64_t c0 = j * TS_B;
         const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
         if (FWD) {
+            // all 16 values in one round of loads; poll again only while one is pending
+            double zc[16];
+            unsigned spins = 0;
+            while (true) {
+                bool pending = false;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const int cc = 16 * grp + c;
-                const double zc = cc < wj ? __hip_atomic_load(&out[c0 + cc], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT)
-                                          : 0.0;
-                acc1 = fma((double)lv[c], zc, acc1);
+                for (int c = 0; c < 16; ++c) {
+                    const int cc = 16 * grp + c;
+                    zc[c] = (cc < wj && ok) ? __hip_atomic_load(&out[c0 + cc], __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT)
+                                            : 0.0;
+                    pending |= __builtin_bit_cast(unsigned long long, zc[c]) == TS_PENDING;
+                }
+                if (!pending) break;
+                if (++spins > TS_SPIN_LIMIT) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = false;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) zc[c] = 0.0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc1 = fma((double)lv[c], zc[c], acc1);
         } else {
-            const double xj = lane < wj ? __hip_atomic_load(&out[c0 + lane], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT)
-                                        : 0.0;
+            const double xj = (lane < wj && ok) ? wait_value(&out[c0 + lane], err, ok) : 0.0;
 #pragma unroll
             for (int c = 0; c < 16; ++c) acc16[c] = fma((double)lv[c], xj, acc16[c]);
         }
@@ -186,11 +211,6 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
             __hip_atomic_store(&out[r0 + i], s_fail ? __builtin_nan("") : xi, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    if (t == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // dinv block b (64 x 64, column-major, zero above the diagonal and outside the block) :=
@@ -225,7 +245,7 @@ __global__ __launch_bounds__(64) void trsv_diag_inverse_kernel(const T* __restri
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
                                                             int64_t n, const double* __restrict__ rhs,
-                                                            double* __restrict__ out,
+                                                            double* out,
                                                             int32_t* __restrict__ sync,
                                                             const double* __restrict__ dinv) {
     __shared__ int s_block;
@@ -235,7 +255,7 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
         s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
     }
     __syncthreads();
-    trsv_block<T, FWD>(L, ldl, n, rhs, out, sync + 2, &sync[1], s_block, dinv);
+    trsv_block<T, FWD>(L, ldl, n, rhs, out, &sync[1], s_block, dinv);
 }
 
 size_t trsv_sync_ints(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B + 2); }
@@ -256,9 +276,8 @@ int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double*
                   int32_t* sync, bool forward, hipStream_t s, const double* dinv) {
     if (n <= 0) return QMLH_OK;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
-    const int64_t cnt = nblocks + 2;
-    hipLaunchKernelGGL(trsv_sync_clear_kernel, dim3((unsigned)ceil_div(cnt, 256)), dim3(256), 0, s,
-                       sync, cnt);
+    hipLaunchKernelGGL(trsv_prepare_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, sync,
+                       (int64_t)2, out, n);
     if (forward)
         hipLaunchKernelGGL((trsv_syncfree_kernel<T, true>), dim3((unsigned)nblocks), dim3(256), 0,
                            s, L, ldl, n, rhs, out, sync, dinv);
