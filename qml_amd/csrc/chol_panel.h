@@ -11,6 +11,43 @@ static constexpr int CP_B = 64;  // diagonal block / sub-panel width
 
 static constexpr int CP_S = 16;  // step of the blocked diagonal factor
 
+// Lane `lane`'s value of v in every lane (two scalar reads; lane is a compile-time constant
+// where this is used, so no LDS round trip).
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Packed lower triangle of a 64 x 64 block in LDS, column-major: element (i, j), i >= j, at
+// pk(i, j); column j holds rows j .. 63. 2080 doubles (16.6 KB) instead of a padded square
+// (33 KB), so the panel kernels fit in the LDS the trailing-update workgroups leave free and run
+// beside them. Wherever the loops are unrolled the index is a compile-time constant (+ lane).
+static constexpr int CP_PACK = CP_B * (CP_B + 1) / 2;
+__host__ __device__ constexpr int pk(int i, int j) { return j * CP_B - (j * (j - 1)) / 2 + (i - j); }
+
+// Packed LDS <- the lower triangle of the w x w block at A (zero outside w); 256 lanes, loads
+// issued back to back.
+template <typename T>
+__device__ __forceinline__ void stage_packed64(double* __restrict__ sp, const T* __restrict__ A,
+                                               int64_t lda, int w) {
+    const int t = threadIdx.x;
+    double v[CP_B * CP_B / 256];
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        v[q] = (i < w && j < w && i >= j) ? (double)A[i + (int64_t)j * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        if (i >= j) sp[pk(i, j)] = v[q];
+    }
+}
+
 // LDS (FP64) <- the lower triangle of the w x w block at A (column-major, lda), zero elsewhere;
 // TRANS stores element (i, j) at s[j][i]. All 256 lanes take part; each issues its 16 loads
 // back to back before any LDS store (a loop with a runtime bound would wait on every load).
@@ -33,15 +70,16 @@ __device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __re
     }
 }
 
-// Cholesky of the w x w (w <= 64) lower block in s (FP64, LDS), all 256 lanes, in four
+// Cholesky of the w x w (w <= 64) lower block in packed LDS sp (FP64), all 256 lanes, in four
 // 16-wide steps: the 16 x 16 diagonal block is factored in the registers of lanes 0..15 of
 // wave 0 (pivot and column entries broadcast by lane shuffles), the rows below it are solved
 // against it (one lane per row), and the trailing lower block gets the rank-16 update (lanes
 // spread over its entries). On exit s holds L in its lower triangle. Returns 0, or c + 1 for
 // the first non-positive (or NaN) pivot c, with the Schur value left on that diagonal entry
 // (as LAPACK leaves it); the value is the same in every lane.
-__device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
+__device__ __forceinline__ int potf2_block64(double* __restrict__ sp, int w) {
     __shared__ int s_fail;
+    __shared__ double rinv[CP_S];
     const int t = threadIdx.x;
     if (t == 0) s_fail = 0;
     __syncthreads();
@@ -49,27 +87,29 @@ __device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
     for (int kb = 0; kb < CP_B; kb += CP_S) {
         if (kb >= w) break;
         const int bw = w - kb < CP_S ? w - kb : CP_S;
-        // 1. factor the diagonal 16 x 16 block
+        // 1. factor the diagonal 16 x 16 block (pivots and column entries broadcast with
+        //    scalar lane reads; the reciprocal pivots go to rinv for step 2)
         if (t < 64) {
             const int lane = t;
             double a[CP_S];
 #pragma unroll
             for (int j = 0; j < CP_S; ++j)
-                a[j] = (lane < bw && j <= lane) ? s[kb + lane][kb + j] : 0.0;
+                a[j] = (lane < bw && j <= lane) ? sp[pk(kb + lane, kb + j)] : 0.0;
             int f = 0;
 #pragma unroll
             for (int c = 0; c < CP_S; ++c) {
                 if (c < bw && f == 0) {
-                    const double piv = __shfl(a[c], c, 64);
+                    const double piv = readlane_f64(a[c], c);
                     if (!(piv > 0.0)) {
                         f = c + 1;
                     } else {
                         const double l = sqrt(piv);
                         const double rl = 1.0 / l;
                         a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
+                        if (lane == 0) rinv[c] = rl;
 #pragma unroll
                         for (int j = c + 1; j < CP_S; ++j) {
-                            const double lj = __shfl(a[c], j, 64);  // L(kb + j, kb + c)
+                            const double lj = readlane_f64(a[c], j);  // L(kb + j, kb + c)
                             a[j] = fma(-a[c], lj, a[j]);  // rows above j: unused upper values
                         }
                     }
@@ -78,7 +118,7 @@ __device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
             if (lane < bw) {
 #pragma unroll
                 for (int j = 0; j < CP_S; ++j)
-                    if (j <= lane) s[kb + lane][kb + j] = a[j];
+                    if (j <= lane) sp[pk(kb + lane, kb + j)] = a[j];
             }
             if (lane == 0 && f) s_fail = kb + f;
         }
@@ -90,31 +130,31 @@ __device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
             const int i = r0 + t;
             double x[CP_S];
 #pragma unroll
-            for (int c = 0; c < CP_S; ++c) x[c] = c < bw ? s[i][kb + c] : 0.0;
+            for (int c = 0; c < CP_S; ++c) x[c] = c < bw ? sp[pk(i, kb + c)] : 0.0;
 #pragma unroll
             for (int c = 0; c < CP_S; ++c) {
                 if (c < bw) {
-                    x[c] = x[c] / s[kb + c][kb + c];
+                    x[c] = x[c] * rinv[c];
 #pragma unroll
-                    for (int j = c + 1; j < CP_S; ++j) x[j] = fma(-x[c], s[kb + j][kb + c], x[j]);
+                    for (int j = c + 1; j < CP_S; ++j) x[j] = fma(-x[c], sp[pk(kb + j, kb + c)], x[j]);
                 }
             }
 #pragma unroll
             for (int c = 0; c < CP_S; ++c)
-                if (c < bw) s[i][kb + c] = x[c];
+                if (c < bw) sp[pk(i, kb + c)] = x[c];
         }
         __syncthreads();
-        // 3. trailing lower block: s[i][j] -= sum_c s[i][kb + c] s[j][kb + c], r0 <= j <= i < w
+        // 3. trailing lower block: L(i, j) -= sum_c L(i, kb + c) L(j, kb + c), r0 <= j <= i < w
         constexpr int RM = CP_B - CP_S;  // largest trailing order (48); entries beyond rem skip
         for (int idx = t; idx < RM * RM; idx += 256) {
             const int io = idx % RM, jo = idx / RM;
             if (io < rem && jo <= io) {
                 const int i = r0 + io, j = r0 + jo;
-                double v = s[i][j];
+                double v = sp[pk(i, j)];
 #pragma unroll
                 for (int c = 0; c < CP_S; ++c)
-                    if (c < bw) v = fma(-s[i][kb + c], s[j][kb + c], v);
-                s[i][j] = v;
+                    if (c < bw) v = fma(-sp[pk(i, kb + c)], sp[pk(j, kb + c)], v);
+                sp[pk(i, j)] = v;
             }
         }
         __syncthreads();
@@ -122,14 +162,13 @@ __device__ __forceinline__ int potf2_block64(double (*s)[CP_B + 1], int w) {
     return 0;
 }
 
-// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L^T in sT (sT[c][j] =
-// L(j, c)) and the reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded
-// whole first; then, column by column, x_c is scaled and subtracted from the later entries
-// (independent FMAs; per entry the same operations in the same order as a row substitution);
-// then the row is stored.
+// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L packed in sp and the
+// reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded whole first; then,
+// column by column, x_c is scaled and subtracted from the later entries (independent FMAs; per
+// entry the same operations in the same order as a row substitution); then the row is stored.
 template <typename T>
 __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t row, int w,
-                                         const double (*sT)[CP_B + 2],
+                                         const double* __restrict__ sp,
                                          const double* __restrict__ inv) {
     double x[CP_B];
 #pragma unroll
@@ -139,7 +178,7 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t
         if (c < w) {
             x[c] = x[c] * inv[c];
 #pragma unroll
-            for (int j = c + 1; j < CP_B; ++j) x[j] = fma(-x[c], sT[c][j], x[j]);
+            for (int j = c + 1; j < CP_B; ++j) x[j] = fma(-x[c], sp[pk(j, c)], x[j]);
         }
     }
 #pragma unroll
@@ -152,17 +191,17 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t
 template <typename T>
 __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
                                            int64_t col_base, int32_t* __restrict__ info) {
-    __shared__ double s[CP_B][CP_B + 1];
+    __shared__ double sp[CP_PACK];
     if (*info) return;
     const int t = threadIdx.x;
-    stage_lower64<false, CP_B + 1>(s, A, lda, w);
+    stage_packed64(sp, A, lda, w);
     __syncthreads();
-    const int fail = potf2_block64(s, w);
+    const int fail = potf2_block64(sp, w);
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
         const int i = idx & (CP_B - 1), j = idx / CP_B;
-        if (i < w && j < w && i >= j) A[i + (int64_t)j * lda] = (T)s[i][j];
+        if (i < w && j < w && i >= j) A[i + (int64_t)j * lda] = (T)sp[pk(i, j)];
     }
     if (t == 0 && fail) *info = (int32_t)(col_base + fail);
 }
@@ -173,16 +212,16 @@ __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w
 template <typename T>
 __device__ __forceinline__ void trsm_body(T* __restrict__ A, int64_t lda, int w, int64_t m,
                                           const int32_t* __restrict__ info, int64_t local) {
-    __shared__ __attribute__((aligned(16))) double sT[CP_B][CP_B + 2];  // sT[c][j] = L(j, c)
+    __shared__ double sp[CP_PACK];
     __shared__ double inv[CP_B];
     if (*info) return;
     const int t = threadIdx.x;
-    stage_lower64<true, CP_B + 2>(sT, A, lda, w);
+    stage_packed64(sp, A, lda, w);
     __syncthreads();
-    if (t < w) inv[t] = 1.0 / sT[t][t];
+    if (t < w) inv[t] = 1.0 / sp[pk(t, t)];
     __syncthreads();
     const int64_t row = local * 256 + t;
-    if (row < m) trsm_row<T>(A + w, lda, row, w, sT, inv);
+    if (row < m) trsm_row<T>(A + w, lda, row, w, sp, inv);
 }
 
 }  // namespace qmlh
