@@ -165,6 +165,24 @@ size_t qmlh_woodbury_workspace(int64_t n);
 int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_t ldv,
                         int64_t r, double *C, int64_t ldc, int32_t *info, void *ws,
                         size_t ws_bytes, qmlh_stream_t stream);
+/* qmlh_spotrf (bf16x3 = 0) or qmlh_spotrf_bf16x3 (bf16x3 != 0; ws as there, else ws may be
+ * NULL) of A that also overwrites V (n x r, column-major, ldv >= n) with L^-1 V in the same
+ * pass: V^T is placed in the r rows below the matrix (lda >= n + r; A[n:n+r, :] is
+ * overwritten), solved as extra panel rows while the trailing updates run (their own updates in
+ * FP32 MFMA), and copied back. The factor of A is the same, bit for bit, as without V. On a
+ * failed factorisation (info != 0) V is unspecified. Replaces qmlh_spotrf* + the solve of
+ * qmlh_woodbury_setup. */
+int qmlh_spotrf_lowrank(float *A, int64_t n, int64_t lda, float *V, int64_t ldv, int64_t r,
+                        int32_t bf16x3, int32_t *info, void *ws, size_t ws_bytes,
+                        qmlh_stream_t stream);
+/* C = chol(I + Y^T Y) (r x r, lower, FP64; r <= 1024) for Y (n x r, column-major FP32): the
+ * rest of qmlh_woodbury_setup once Y = L^-1 V is known. The Gram matrix is summed in FP64 over
+ * row slices in a fixed order (deterministic); ws holds
+ * qmlh_woodbury_capacitance_workspace(n, r) bytes. */
+size_t qmlh_woodbury_capacitance_workspace(int64_t n, int64_t r);
+int qmlh_woodbury_capacitance(const float *Y, int64_t n, int64_t ldy, int64_t r, double *C,
+                              int64_t ldc, int32_t *info, void *ws, size_t ws_bytes,
+                              qmlh_stream_t stream);
 /* B(i, j) = (float)(A(i, j) + shift [i == j]) on the lower triangle (column-major). */
 int qmlh_lower_to_f32(const double *A, int64_t n, int64_t lda, float *B, int64_t ldb,
                       double shift, qmlh_stream_t stream);

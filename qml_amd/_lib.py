@@ -64,6 +64,11 @@ _SIGNATURES = {
     "qmlh_gaussian_deflated": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _vp,
                                               _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "qmlh_woodbury_workspace": (_sz, [_i64]),
+    "qmlh_spotrf_lowrank": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _vp,
+                                           _sz, _vp]),
+    "qmlh_woodbury_capacitance_workspace": (_sz, [_i64, _i64]),
+    "qmlh_woodbury_capacitance": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp, _sz,
+                                                 _vp]),
     "qmlh_woodbury_setup": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp,
                                            _vp, _sz, _vp]),
     "qmlh_lower_to_f32": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _vp]),

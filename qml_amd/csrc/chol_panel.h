@@ -27,46 +27,52 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 static constexpr int CP_PACK = CP_B * (CP_B + 1) / 2;
 __host__ __device__ constexpr int pk(int i, int j) { return j * CP_B - (j * (j - 1)) / 2 + (i - j); }
 
+// Every lane loads unconditionally from an index clamped into the w x w block and selects
+// zero afterwards: a load (and its conversion) under a condition makes the compiler wait for
+// it inside the branch, which serialises a run of loads.
+__device__ __forceinline__ int cp_clamp(int i, int w) { return i < w ? i : w - 1; }
+
 // Packed LDS <- the lower triangle of the w x w block at A (zero outside w); 256 lanes, loads
 // issued back to back.
 template <typename T>
 __device__ __forceinline__ void stage_packed64(double* __restrict__ sp, const T* __restrict__ A,
                                                int64_t lda, int w) {
     const int t = threadIdx.x;
-    double v[CP_B * CP_B / 256];
+    T raw[CP_B * CP_B / 256];
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
         const int i = idx & (CP_B - 1), j = idx / CP_B;
-        v[q] = (i < w && j < w && i >= j) ? (double)A[i + (int64_t)j * lda] : 0.0;
+        raw[q] = A[cp_clamp(i, w) + (int64_t)cp_clamp(j, w) * lda];
     }
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
         const int i = idx & (CP_B - 1), j = idx / CP_B;
-        if (i >= j) sp[pk(i, j)] = v[q];
+        if (i >= j) sp[pk(i, j)] = (i < w && j < w) ? (double)raw[q] : 0.0;
     }
 }
 
 // LDS (FP64) <- the lower triangle of the w x w block at A (column-major, lda), zero elsewhere;
 // TRANS stores element (i, j) at s[j][i]. All 256 lanes take part; each issues its 16 loads
-// back to back before any LDS store (a loop with a runtime bound would wait on every load).
+// back to back before any LDS store.
 template <bool TRANS, int LDS_LD, typename T>
 __device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __restrict__ A,
                                               int64_t lda, int w) {
     const int t = threadIdx.x;
-    double v[CP_B * CP_B / 256];
+    T raw[CP_B * CP_B / 256];
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
         const int i = idx & (CP_B - 1), j = idx / CP_B;  // consecutive lanes: consecutive rows
-        v[q] = (i < w && j < w && i >= j) ? (double)A[i + (int64_t)j * lda] : 0.0;
+        raw[q] = A[cp_clamp(i, w) + (int64_t)cp_clamp(j, w) * lda];
     }
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
         const int i = idx & (CP_B - 1), j = idx / CP_B;
-        if (TRANS) s[j][i] = v[q]; else s[i][j] = v[q];
+        const double v = (i < w && j < w && i >= j) ? (double)raw[q] : 0.0;
+        if (TRANS) s[j][i] = v; else s[i][j] = v;
     }
 }
 
@@ -163,16 +169,32 @@ __device__ __forceinline__ int potf2_block64(double* __restrict__ sp, int w) {
 }
 
 // Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L packed in sp and the
-// reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded whole first; then,
-// column by column, x_c is scaled and subtracted from the later entries (independent FMAs; per
-// entry the same operations in the same order as a row substitution); then the row is stored.
+// reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded whole first (64 loads
+// in flight together); then, column by column, x_c is scaled and subtracted from the later
+// entries (independent FMAs; per entry the same operations in the same order as a row
+// substitution); then the row is stored. The branch on w per column keeps the compiler from
+// hoisting every column's LDS reads at once (which would spill). Addresses are a uniform
+// column pointer plus the lane's 32-bit byte offset (rows below 2^31 / 8).
 template <typename T>
-__device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t row, int w,
+__device__ __forceinline__ const T& at_byte(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ T& at_byte(T* base, uint32_t off) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+
+template <typename T>
+__device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_t row, int w,
                                          const double* __restrict__ sp,
                                          const double* __restrict__ inv) {
+    const uint32_t off = row * (uint32_t)sizeof(T);
+    T raw[CP_B];
+#pragma unroll
+    for (int c = 0; c < CP_B; ++c) raw[c] = at_byte<T>(B + cp_clamp(c, w) * ldb, off);
     double x[CP_B];
 #pragma unroll
-    for (int c = 0; c < CP_B; ++c) x[c] = c < w ? (double)B[row + c * ldb] : 0.0;
+    for (int c = 0; c < CP_B; ++c) x[c] = c < w ? (double)raw[c] : 0.0;
 #pragma unroll
     for (int c = 0; c < CP_B; ++c) {
         if (c < w) {
@@ -183,7 +205,7 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, int64_t
     }
 #pragma unroll
     for (int c = 0; c < CP_B; ++c)
-        if (c < w) B[row + c * ldb] = (T)x[c];
+        if (c < w) at_byte<T>(B + c * ldb, off) = (T)x[c];
 }
 
 // Diagonal block of a sub-panel, one workgroup: factor the w x w block at A in wave 0, write
@@ -218,10 +240,10 @@ __device__ __forceinline__ void trsm_body(T* __restrict__ A, int64_t lda, int w,
     const int t = threadIdx.x;
     stage_packed64(sp, A, lda, w);
     __syncthreads();
-    if (t < w) inv[t] = 1.0 / sp[pk(t, t)];
+    if (t < CP_B) inv[t] = t < w ? 1.0 / sp[pk(t, t)] : 0.0;
     __syncthreads();
     const int64_t row = local * 256 + t;
-    if (row < m) trsm_row<T>(A + w, lda, row, w, sp, inv);
+    if (row < m) trsm_row<T>(A + w, lda, (uint32_t)row, w, sp, inv);
 }
 
 }  // namespace qmlh

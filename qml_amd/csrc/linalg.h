@@ -8,10 +8,11 @@ namespace qmlh {
 
 // Lower Cholesky in place (column-major), storage T in {double, float}; LAPACK info rules.
 // For float, a non-null bf16_ws (qmlh_spotrf_bf16x3_workspace(n) bytes) selects split-BF16
-// trailing updates instead of FP32 MFMA ones.
+// trailing updates instead of FP32 MFMA ones. r > 0: the r rows below the matrix
+// (A[n:n+r, 0:n], lda >= n + r) are solved along, ending as W L^-T.
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
-               void* bf16_ws = nullptr);
+               void* bf16_ws = nullptr, int64_t r = 0);
 
 size_t syrk_bf16_workspace(int64_t n, int64_t kb);
 // C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three BF16 products.

@@ -50,8 +50,6 @@ __global__ __launch_bounds__(256) void dot_final_kernel(const double* __restrict
     if (threadIdx.x == 0) *out = tot;
 }
 
-// y_j = sum_i A(i, j) x_i (column dots; A symmetric so this is A x). One wavefront per
-// column, lanes stride down the column, fixed-order butterfly.
 // ------------------------------------------------------------------------------------------
 // y = A x for symmetric A reading only its lower triangle (column-major), deterministic.
 // Workgroup (strip J, chunk) takes the 64 columns 64 J .. and up to SY_CHUNK row blocks of 64
@@ -221,6 +219,86 @@ __global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ Y, 
     if (a < r && b < r) C[a + b * ldc] = acc + (a == b ? 1.0 : 0.0);
 }
 
+// C = I + Y^T Y in two deterministic passes. Pass 1: workgroup (block, slice) takes one
+// 32 x 32 block of the lower triangle (block index over the lower blocks) and one slice of the
+// rows of Y, stages 64-row chunks of the two column groups in LDS as FP64, and accumulates a
+// 2 x 2 micro tile per lane; partial[slice] holds the slice's block. Pass 2 adds the slices in
+// order, adds I, and writes both triangles from the lower one.
+static constexpr int GR_T = 32;
+static constexpr int GR_SLICES_MAX = 16;
+
+static int64_t gram_slices(int64_t n) {
+    const int64_t s = ceil_div(n, (int64_t)1024);
+    return s < 1 ? 1 : (s > GR_SLICES_MAX ? GR_SLICES_MAX : s);
+}
+
+__global__ __launch_bounds__(256) void gram_partial_kernel(const float* __restrict__ Y,
+                                                           int64_t ldy, int64_t n, int64_t r,
+                                                           int64_t slice_rows,
+                                                           double* __restrict__ partial) {
+    __shared__ double sa[64][GR_T + 1], sb[64][GR_T + 1];
+    int64_t bi = (int64_t)((sqrt(8.0 * (double)blockIdx.x + 1.0) - 1.0) * 0.5);
+    while (bi * (bi + 1) / 2 > (int64_t)blockIdx.x) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= (int64_t)blockIdx.x) ++bi;
+    const int64_t bj = blockIdx.x - bi * (bi + 1) / 2;
+    const int64_t a0 = bi * GR_T, b0 = bj * GR_T;
+    const int64_t lo = (int64_t)blockIdx.y * slice_rows;
+    const int64_t hi = lo + slice_rows < n ? lo + slice_rows : n;
+    const int t = threadIdx.x, ta = t & 15, tb = t >> 4;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int64_t i0 = lo; i0 < hi; i0 += 64) {
+        float ra[8], rb[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // element (ii, cc) of the 64 x 32 chunk, ii fastest
+            const int idx = t + 256 * q, ii = idx & 63, cc = idx >> 6;
+            const int64_t i = i0 + ii < hi ? i0 + ii : hi - 1;
+            const int64_t ca = a0 + cc < r ? a0 + cc : r - 1, cb = b0 + cc < r ? b0 + cc : r - 1;
+            ra[q] = Y[i + ca * ldy];
+            rb[q] = Y[i + cb * ldy];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int idx = t + 256 * q, ii = idx & 63, cc = idx >> 6;
+            const bool in = i0 + ii < hi;
+            sa[ii][cc] = in && a0 + cc < r ? (double)ra[q] : 0.0;
+            sb[ii][cc] = in && b0 + cc < r ? (double)rb[q] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int ii = 0; ii < 64; ++ii) {
+            const double x0 = sa[ii][ta], x1 = sa[ii][ta + 16];
+            const double y0 = sb[ii][tb], y1 = sb[ii][tb + 16];
+            acc[0][0] = fma(x0, y0, acc[0][0]);
+            acc[0][1] = fma(x0, y1, acc[0][1]);
+            acc[1][0] = fma(x1, y0, acc[1][0]);
+            acc[1][1] = fma(x1, y1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+    double* P = partial + (int64_t)blockIdx.y * r * r;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int64_t a = a0 + ta + 16 * u, b = b0 + tb + 16 * v;
+            if (a < r && b < r) P[a + b * r] = acc[u][v];
+        }
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restrict__ partial,
+                                                          int64_t r, int64_t slices,
+                                                          double* __restrict__ C, int64_t ldc) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= r * r) return;
+    const int64_t a = idx % r, b = idx / r;
+    if (a < b) return;  // the mirror entry writes it
+    double v = 0.0;
+    for (int64_t s = 0; s < slices; ++s) v += partial[s * r * r + idx];
+    if (a == b) v += 1.0;
+    C[a + b * ldc] = v;
+    C[b + a * ldc] = v;
+}
+
 __global__ void p_update_kernel(double* __restrict__ p, const double* __restrict__ z,
                                 double* __restrict__ sc, int64_t n, int first) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,6 +439,41 @@ extern "C" int qmlh_woodbury_setup(const float* L, int64_t n, int64_t ldl, float
     dim3 g((unsigned)ceil_div(r, 16), (unsigned)ceil_div(r, 16));
     hipLaunchKernelGGL(gram_kernel, g, dim3(256), 0, s, V, ldv, n, r, C, ldc);
     QMLH_CHECK_LAUNCH("woodbury gram launch");
+    return potrf_impl<double>(C, r, ldc, info, s);
+}
+
+extern "C" size_t qmlh_woodbury_capacitance_workspace(int64_t n, int64_t r) {
+    if (n <= 0 || r <= 0) return 0;
+    return (size_t)qmlh::gram_slices(n) * (size_t)r * (size_t)r * sizeof(double);
+}
+
+extern "C" int qmlh_woodbury_capacitance(const float* Y, int64_t n, int64_t ldy, int64_t r,
+                                         double* C, int64_t ldc, int32_t* info, void* ws,
+                                         size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || r < 0 || ldy < (n > 1 ? n : 1) || ldc < r)
+        return set_error(QMLH_ERR_ARG, "woodbury_capacitance: bad shape");
+    if (r > WB_MAX_RANK)
+        return set_error(QMLH_ERR_ARG, "woodbury_capacitance: rank %lld above %d", (long long)r,
+                         WB_MAX_RANK);
+    if (r == 0) return QMLH_OK;
+    if (!C || !info || (n > 0 && (!Y || !ws)))
+        return set_error(QMLH_ERR_ARG, "woodbury_capacitance: null pointer");
+    if (ws_bytes < qmlh_woodbury_capacitance_workspace(n, r))
+        return set_error(QMLH_ERR_WORKSPACE, "woodbury_capacitance: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    double* partial = static_cast<double*>(ws);
+    const int64_t slices = n > 0 ? gram_slices(n) : 0;
+    if (slices > 0) {
+        const int64_t nbk = ceil_div(r, (int64_t)GR_T);
+        const int64_t slice_rows = ceil_div(ceil_div(n, slices), (int64_t)64) * 64;
+        hipLaunchKernelGGL(gram_partial_kernel, dim3((unsigned)(nbk * (nbk + 1) / 2), (unsigned)slices),
+                           dim3(256), 0, s, Y, ldy, n, r, slice_rows, partial);
+        QMLH_CHECK_LAUNCH("woodbury gram launch");
+    }
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)ceil_div(r * r, (int64_t)256)), dim3(256),
+                       0, s, partial, r, slices, C, ldc);
+    QMLH_CHECK_LAUNCH("woodbury gram reduce launch");
     return potrf_impl<double>(C, r, ldc, info, s);
 }
 

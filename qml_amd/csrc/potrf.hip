@@ -140,23 +140,35 @@ static int launch_panel64(T* A, int64_t lda, int w, int64_t m, int64_t col_base,
 
 __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 
-// Factor the block column A[k:n, k:k+kb] (its trailing updates from earlier columns applied):
-// per 64-wide sub-panel the diagonal factor, the rows below, and the update of the rest of the
-// block column.
+// Factor the block column A[k:rows, k:k+kb] (its trailing updates from earlier columns
+// applied): per 64-wide sub-panel the diagonal factor, the rows below, and the update of the
+// rest of the block column. rows = n, or n + r when r extra rows below the matrix are solved
+// along (they are panel rows like any other).
 template <typename T>
-static int factor_block_column(T* A, int64_t lda, int64_t n, int64_t k, int64_t kb, int32_t* info,
-                               hipStream_t s) {
+static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64_t kb,
+                               int32_t* info, hipStream_t s) {
     for (int64_t jj = 0; jj < kb; jj += PF_IB) {
         const int64_t c = k + jj;
         const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
         T* Acc = A + c + c * lda;
-        QMLH_TRY(launch_panel64<T>(Acc, lda, w, n - c - w, c, info, s));
+        QMLH_TRY(launch_panel64<T>(Acc, lda, w, rows - c - w, c, info, s));
         const int64_t ncols = kb - jj - w;  // rest of this block column
         if (ncols > 0)
-            QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, n - c - w, ncols, w,
-                                      Acc + w + w * lda, lda, info, s));
+            QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, rows - c - w, ncols,
+                                      w, Acc + w + w * lda, lda, info, s));
     }
     return QMLH_OK;
+}
+
+// Extra rows A[n:n+r, c0:n] -= A[n:n+r, k:k+kb] A[c0:n, k:k+kb]^T: the trailing update of the
+// r rows solved along with the factorisation (FP32/FP64 MFMA in the storage type, also when
+// the matrix itself takes split-BF16 updates).
+template <typename T>
+static int extra_rows_update(T* A, int64_t lda, int64_t n, int64_t r, int64_t k, int64_t kb,
+                             int64_t c0, int32_t* info, hipStream_t s) {
+    if (r <= 0 || c0 >= n) return QMLH_OK;
+    return launch_update<T>(UPD_FULL, A + n + k * lda, lda, A + c0 + k * lda, lda, r, n - c0, kb,
+                            A + n + c0 * lda, lda, info, s);
 }
 
 // Lower triangle of C (m x m) -= P P^T, P (m x kb) the panel below the block column.
@@ -172,21 +184,29 @@ static int trailing_syrk(T* P, int64_t lda, int64_t m, int64_t kb, T* C, int32_t
 
 // Matrices at least this large are factored with look-ahead.
 static constexpr int64_t PF_LOOKAHEAD_MIN = 4096;
+// Trailing matrices at least this large keep the extra rows' update on the side stream.
+static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 
 // Look-ahead schedule. Step k's trailing update is split into the next block column's part
 // (an MFMA update on the caller's stream) and the rest; the next block column is factored on a
 // second stream while the rest of the update runs, so the latency-bound panel work overlaps the
 // bandwidth/MFMA-bound trailing update. Events order the two streams; the caller's stream waits
 // for the last panel, so on return all work is ordered on `s`.
+//
+// With r extra rows (A[n:n+r, :]), the next block column's update and panel include them. The
+// rest of their trailing update touches nothing the trailing update or the panel reads or
+// writes: while the trailing matrix is large (the caller's stream is the critical path) it runs
+// on the side stream after the panel, later (the panels are the critical path) on the caller's
+// stream after the trailing update.
 template <typename T>
-static int potrf_lookahead(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
-                           void* bf16_ws, int64_t nb) {
+static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
+                           hipStream_t s, void* bf16_ws, int64_t nb) {
     hipStream_t s2 = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
     int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming), "potrf: event");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_p, hipEventDisableTiming), "potrf: event");
-    if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n, 0, n < nb ? n : nb, info, s);
+    if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s);
     for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
         const int64_t m = n - k - kb;
@@ -194,13 +214,18 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int32_t* info, hipStrea
         const int64_t kb2 = m < nb ? m : nb;  // next block column
         T* P = A + (k + kb) + k * lda;
         T* C = P + kb * lda;  // A(k + kb, k + kb)
-        rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m, kb2, kb, C, lda, info, s);
+        rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
-        if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n, k + kb, kb2, info, s2);
+        if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2);
+        const bool ext_side = m >= PF_EXT_SIDE_MIN;
+        if (rc == QMLH_OK && ext_side)
+            rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s2);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
         if (rc == QMLH_OK && m > kb2)
             rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s, bf16_ws);
+        if (rc == QMLH_OK && !ext_side)
+            rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
     }
     // Destroying them here is safe: pending work completes first, and `s` already waits for it.
@@ -211,9 +236,10 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int32_t* info, hipStrea
 }
 
 template <typename T>
-int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* bf16_ws) {
-    if (n < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0");
-    if (lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n");
+int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* bf16_ws,
+               int64_t r) {
+    if (n < 0 || r < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0 or r < 0");
+    if (lda < (n + r > 1 ? n + r : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n + r");
     if (!info) return set_error(QMLH_ERR_ARG, "potrf: info is null");
     hipLaunchKernelGGL(clear_info_kernel, dim3(1), dim3(1), 0, s, info);
     QMLH_CHECK_LAUNCH("potrf: clear info");
@@ -222,13 +248,15 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     TimerScope ts(QMLH_TIMER_POTRF, (double)n * (double)n * (double)n / 3.0, s);
     const bool bf16 = std::is_same<T, float>::value && bf16_ws;
     const int64_t nb = bf16 ? PF_NB_BF16 : PF_NB;
-    if (n >= PF_LOOKAHEAD_MIN) return potrf_lookahead<T>(A, n, lda, info, s, bf16 ? bf16_ws : nullptr, nb);
+    if (n >= PF_LOOKAHEAD_MIN)
+        return potrf_lookahead<T>(A, n, lda, r, info, s, bf16 ? bf16_ws : nullptr, nb);
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
-        QMLH_TRY(factor_block_column<T>(A, lda, n, k, kb, info, s));
+        QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s));
         const int64_t m = n - k - kb;
         T* P = A + (k + kb) + k * lda;
         QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, bf16 ? bf16_ws : nullptr));
+        QMLH_TRY(extra_rows_update<T>(A, lda, n, r, k, kb, k + kb, info, s));
     }
     return QMLH_OK;
 }
@@ -470,8 +498,41 @@ int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64
     return QMLH_OK;
 }
 
-template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*);
-template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*);
+// out(j, i) = in(i, j) for the R x Cc column-major `in`: 64 x 64 tiles through LDS (padded
+// rows, so the transposed reads hit distinct banks); one-dimensional grid of tiles.
+__global__ __launch_bounds__(256) void transpose_f32_kernel(const float* __restrict__ in,
+                                                            int64_t ldi, int64_t R, int64_t Cc,
+                                                            float* __restrict__ out, int64_t ldo,
+                                                            int64_t tiles_r) {
+    __shared__ float t[64][65];
+    const int64_t r0 = (blockIdx.x % tiles_r) * 64, c0 = (blockIdx.x / tiles_r) * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = ty; q < 64; q += 4) {
+        const int64_t i = r0 + tx, j = c0 + q;
+        if (i < R && j < Cc) t[q][tx] = in[i + j * ldi];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = ty; q < 64; q += 4) {
+        const int64_t i = r0 + q, j = c0 + tx;
+        if (i < R && j < Cc) out[j + i * ldo] = t[tx][q];
+    }
+}
+
+static int transpose_f32(const float* in, int64_t ldi, int64_t R, int64_t Cc, float* out,
+                         int64_t ldo, hipStream_t s) {
+    if (R <= 0 || Cc <= 0) return QMLH_OK;
+    const int64_t tr = ceil_div(R, 64), tc = ceil_div(Cc, 64);
+    hipLaunchKernelGGL(transpose_f32_kernel, dim3((unsigned)(tr * tc)), dim3(256), 0, s, in, ldi,
+                       R, Cc, out, ldo, tr);
+    QMLH_CHECK_LAUNCH("transpose launch");
+    return QMLH_OK;
+}
+
+template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*,
+                                int64_t);
+template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*, int64_t);
 template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, int32_t*,
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
@@ -511,6 +572,27 @@ extern "C" int qmlh_spotrf_bf16x3(float* A, int64_t n, int64_t lda, int32_t* inf
     if (reinterpret_cast<uintptr_t>(ws) & 15)
         return set_error(QMLH_ERR_ARG, "spotrf_bf16x3: workspace must be 16-byte aligned");
     return potrf_impl<float>(A, n, lda, info, (hipStream_t)stream, ws);
+}
+
+extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
+                                   int64_t r, int32_t bf16x3, int32_t* info, void* ws,
+                                   size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || r < 0 || lda < n + r || (r > 0 && ldv < n))
+        return set_error(QMLH_ERR_ARG, "spotrf_lowrank: bad shape (lda >= n + r, ldv >= n)");
+    if (n > 0 && (!A || !info || (r > 0 && !V)))
+        return set_error(QMLH_ERR_ARG, "spotrf_lowrank: null pointer");
+    if (bf16x3 && n > 0) {
+        if (!ws || ws_bytes < qmlh_spotrf_bf16x3_workspace(n))
+            return set_error(QMLH_ERR_WORKSPACE, "spotrf_lowrank: workspace too small");
+        if (reinterpret_cast<uintptr_t>(ws) & 15)
+            return set_error(QMLH_ERR_ARG, "spotrf_lowrank: workspace must be 16-byte aligned");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (n > 0) QMLH_TRY(transpose_f32(V, ldv, n, r, A + n, lda, s));
+    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, bf16x3 ? ws : nullptr, r));
+    if (n > 0) QMLH_TRY(transpose_f32(A + n, lda, r, n, V, ldv, s));
+    return QMLH_OK;
 }
 
 extern "C" size_t qmlh_potrs_workspace(int64_t n) {

@@ -47,8 +47,10 @@ class KRR:
       "mixed" -- K in FP64 for the residuals, an FP32 Cholesky (FP32 MFMA trailing updates,
                  FP64 diagonal blocks) as preconditioner, FP64 conjugate-gradient refinement to
                  |y - K alpha| <= tol |y|. For the Gaussian kernel the factor is of the
-                 deflated matrix B = K - u u^T (u_i = exp(-|x_i|^2 / 2 sigma^2)), with the
-                 rank-one term restored by Sherman-Morrison inside the preconditioner.
+                 deflated matrix B = K - V V^T (V = u [1, x / sigma], u_i =
+                 exp(-|x_i|^2 / 2 sigma^2); the constant and linear Taylor terms of the
+                 kernel), with the rank-(d+1) term restored by a Woodbury correction inside the
+                 preconditioner; Y = L^-1 V is solved during the factorisation itself.
     """
 
     def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9,
@@ -125,14 +127,16 @@ class KRR:
         rep = FitReport(n=n, info=0, precision="mixed")
         t0 = time.perf_counter()
         K = torch.empty((n, n), dtype=torch.float64, device=dev)
-        B = torch.empty((n, n), dtype=torch.float32, device=dev)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         norms = u = V = C = None
-        r = 0
+        r = d + 1 if self.kernel == "gaussian" else 0
+        # B32 column-major with ld = n + r: the r rows below the matrix carry V^T through the
+        # factorisation, which solves them into (L^-1 V)^T along the way.
+        ldb = n + r
+        B = torch.empty((n, ldb), dtype=torch.float32, device=dev)
         if self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
             u = torch.empty(n, dtype=torch.float64, device=dev)
-            r = d + 1
             V = torch.empty((r, n), dtype=torch.float32, device=dev)  # (n, r) column-major
             C = torch.empty((r, r), dtype=torch.float64, device=dev)
         else:
@@ -146,18 +150,18 @@ class KRR:
             if self.kernel == "gaussian":
                 _lib.check(L.qmlh_gaussian_deflated(
                     _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(norms), d, self.sigma, self.lam, shift,
-                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), n, _lib.ptr(V), n,
+                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), ldb, _lib.ptr(V), n,
                     _lib.ptr(u), h))
             else:
-                _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), n, shift, h))
-            if self.factor == "bf16x3":
-                if bws is None:
-                    nb = L.qmlh_spotrf_bf16x3_workspace(n)
-                    bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
-                _lib.check(L.qmlh_spotrf_bf16x3(_lib.ptr(B), n, n, _lib.ptr(info), _lib.ptr(bws),
-                                                bws.numel() * 8, h))
-            else:
-                _lib.check(L.qmlh_spotrf(_lib.ptr(B), n, n, _lib.ptr(info), h))
+                _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), ldb, shift, h))
+            bf16 = self.factor == "bf16x3"
+            if bf16 and bws is None:
+                nb = L.qmlh_spotrf_bf16x3_workspace(n)
+                bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
+            _lib.check(L.qmlh_spotrf_lowrank(_lib.ptr(B), n, ldb, _lib.ptr(V) if r else None, n,
+                                             r, int(bf16), _lib.ptr(info),
+                                             _lib.ptr(bws) if bf16 else None,
+                                             bws.numel() * 8 if bf16 else 0, h))
             rep.info = _lib.host_int(info, stream)
             rep.shift = shift
             if rep.info == 0:
@@ -166,12 +170,13 @@ class KRR:
             if raise_on_error:
                 raise CholeskyError(f"FP32 factorisation failed at every shift: info={rep.info}")
             return rep
-        if r:
-            wws = torch.empty(max(1, L.qmlh_woodbury_workspace(n) // 8), dtype=torch.float64,
-                              device=dev)
-            _lib.check(L.qmlh_woodbury_setup(_lib.ptr(B), n, n, _lib.ptr(V), n, r, _lib.ptr(C), r,
-                                             _lib.ptr(info), _lib.ptr(wws), wws.numel() * 8, h))
-            del wws
+        if r:  # V now holds Y = L^-1 V
+            gws = torch.empty(max(1, L.qmlh_woodbury_capacitance_workspace(n, r) // 8),
+                              dtype=torch.float64, device=dev)
+            _lib.check(L.qmlh_woodbury_capacitance(_lib.ptr(V), n, n, r, _lib.ptr(C), r,
+                                                   _lib.ptr(info), _lib.ptr(gws),
+                                                   gws.numel() * 8, h))
+            del gws
             if _lib.host_int(info, stream) != 0:
                 raise CholeskyError("Woodbury capacitance matrix is not positive definite")
         alpha = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -180,7 +185,7 @@ class KRR:
         ws = torch.empty(max(1, ws_bytes // 8 + 1), dtype=torch.float64, device=dev)
         iters = ctypes.c_int32()
         relres = ctypes.c_double()
-        _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), n, _lib.ptr(V), n, r,
+        _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), ldb, _lib.ptr(V), n, r,
                                     _lib.ptr(C), r, _lib.ptr(yd), _lib.ptr(alpha), self.tol,
                                     self.max_iter, _lib.ptr(ws), ws.numel() * 8,
                                     ctypes.byref(iters), ctypes.byref(relres), h))

@@ -205,6 +205,54 @@ def test_woodbury_preconditioner_is_exact_for_exact_factor(n, r):
         np.linalg.solve(A, y))
 
 
+@pytest.mark.parametrize("n,r,bf16", [(700, 9, 0), (1333, 150, 1), (4500, 437, 1), (4500, 1, 0)])
+def test_spotrf_lowrank_factor_and_solve(n, r, bf16):
+    # One pass factors A and solves Y = L^-1 V as extra panel rows: the factor is the plain
+    # one bit for bit, Y matches a host triangular solve with that factor, and the capacitance
+    # factor matches numpy's Cholesky of I + Y^T Y.
+    from scipy.linalg import solve_triangular
+
+    from qml_amd import _lib
+
+    L = _lib.lib()
+    rng = np.random.default_rng(n + r)
+    M = rng.standard_normal((n, n)) / np.sqrt(n)
+    A = (M @ M.T + 0.5 * np.eye(n)).astype(np.float32)
+    Vm = (rng.standard_normal((n, r)) * 3).astype(np.float32)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = torch.empty(L.qmlh_spotrf_bf16x3_workspace(n) // 8 + 1, dtype=torch.float64,
+                     device="cuda")
+    Ad = torch.from_numpy(A.copy()).cuda()
+    if bf16:
+        _lib.check(L.qmlh_spotrf_bf16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
+                                        ws.numel() * 8, None))
+    else:
+        _lib.check(L.qmlh_spotrf(Ad.data_ptr(), n, n, info.data_ptr(), None))
+    assert int(info.item()) == 0
+    # lda = n + r; the rows below the matrix start as NaN and must be overwritten
+    Ab = torch.full((n, n + r), float("nan"), dtype=torch.float32, device="cuda")
+    Ab[:, :n] = torch.from_numpy(A).cuda()
+    Vd = torch.from_numpy(np.ascontiguousarray(Vm.T)).cuda()  # (n, r) column-major
+    _lib.check(L.qmlh_spotrf_lowrank(Ab.data_ptr(), n, n + r, Vd.data_ptr(), n, r, bf16,
+                                     info.data_ptr(), ws.data_ptr() if bf16 else None,
+                                     ws.numel() * 8 if bf16 else 0, None))
+    assert int(info.item()) == 0
+    Lg = np.tril(Ab[:, :n].cpu().numpy().T)
+    assert np.array_equal(Lg, np.tril(Ad.cpu().numpy().T))
+    Yref = solve_triangular(Lg.astype(np.float64), Vm.astype(np.float64), lower=True)
+    Yg = Vd.cpu().numpy().T.astype(np.float64)
+    assert np.linalg.norm(Yg - Yref) <= 1e-5 * np.linalg.norm(Yref)
+    Cd = torch.empty((r, r), dtype=torch.float64, device="cuda")
+    gws = torch.empty(L.qmlh_woodbury_capacitance_workspace(n, r) // 8 + 1, dtype=torch.float64,
+                      device="cuda")
+    _lib.check(L.qmlh_woodbury_capacitance(Vd.data_ptr(), n, n, r, Cd.data_ptr(), r,
+                                           info.data_ptr(), gws.data_ptr(), gws.numel() * 8,
+                                           None))
+    assert int(info.item()) == 0
+    Cref = np.linalg.cholesky(np.eye(r) + Yg.T @ Yg)
+    assert np.linalg.norm(np.tril(Cd.cpu().numpy().T) - Cref) <= 1e-12 * np.linalg.norm(Cref)
+
+
 @pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "bf16x3"),
                                                        ("gaussian", 4000.0, 1e-6, 3000, "fp32"),
                                                        ("laplacian", 724.0, 1e-8, 2000, "fp32")])
