@@ -9,8 +9,6 @@ namespace qmlh {
 
 static constexpr int CP_B = 64;  // diagonal block / sub-panel width
 
-static constexpr int CP_S = 16;  // step of the blocked diagonal factor
-
 // Lane `lane`'s value of v in every lane (two scalar reads; lane is a compile-time constant
 // where this is used, so no LDS round trip).
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -76,96 +74,59 @@ __device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __re
     }
 }
 
-// Cholesky of the w x w (w <= 64) lower block in packed LDS sp (FP64), all 256 lanes, in four
-// 16-wide steps: the 16 x 16 diagonal block is factored in the registers of lanes 0..15 of
-// wave 0 (pivot and column entries broadcast by lane shuffles), the rows below it are solved
-// against it (one lane per row), and the trailing lower block gets the rank-16 update (lanes
-// spread over its entries). On exit s holds L in its lower triangle. Returns 0, or c + 1 for
-// the first non-positive (or NaN) pivot c, with the Schur value left on that diagonal entry
-// (as LAPACK leaves it); the value is the same in every lane.
-__device__ __forceinline__ int potf2_block64(double* __restrict__ sp, int w) {
+// Cholesky of the w x w (w <= 64) lower block in packed LDS sp (FP64) by one wavefront, all
+// in registers: lane i holds row i. Per column c: the pivot is read from lane c, lanes below
+// scale their entry, the column goes through a 64-entry LDS buffer, and every lane subtracts
+// L(i, c) L(j, c) from its entries j > c (independent FMAs; entries above the diagonal are
+// scratch). Called by all threads of the workgroup (barriers); waves 1.. only wait. On exit sp
+// holds L in its lower triangle. Returns 0, or c + 1 for the first non-positive (or NaN) pivot
+// c, with the Schur value left on that diagonal entry (as LAPACK leaves it); the value is the
+// same in every lane.
+__device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w) {
+    __shared__ double colbuf[CP_B];
     __shared__ int s_fail;
-    __shared__ double rinv[CP_S];
     const int t = threadIdx.x;
-    if (t == 0) s_fail = 0;
-    __syncthreads();
+    if (t < CP_B) {
+        const int lane = t;
+        double a[CP_B];
 #pragma unroll
-    for (int kb = 0; kb < CP_B; kb += CP_S) {
-        if (kb >= w) break;
-        const int bw = w - kb < CP_S ? w - kb : CP_S;
-        // 1. factor the diagonal 16 x 16 block (pivots and column entries broadcast with
-        //    scalar lane reads; the reciprocal pivots go to rinv for step 2)
-        if (t < 64) {
-            const int lane = t;
-            double a[CP_S];
-#pragma unroll
-            for (int j = 0; j < CP_S; ++j)
-                a[j] = (lane < bw && j <= lane) ? sp[pk(kb + lane, kb + j)] : 0.0;
-            int f = 0;
-#pragma unroll
-            for (int c = 0; c < CP_S; ++c) {
-                if (c < bw && f == 0) {
-                    const double piv = readlane_f64(a[c], c);
-                    if (!(piv > 0.0)) {
-                        f = c + 1;
-                    } else {
-                        const double l = sqrt(piv);
-                        const double rl = 1.0 / l;
-                        a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
-                        if (lane == 0) rinv[c] = rl;
-#pragma unroll
-                        for (int j = c + 1; j < CP_S; ++j) {
-                            const double lj = readlane_f64(a[c], j);  // L(kb + j, kb + c)
-                            a[j] = fma(-a[c], lj, a[j]);  // rows above j: unused upper values
-                        }
-                    }
-                }
-            }
-            if (lane < bw) {
-#pragma unroll
-                for (int j = 0; j < CP_S; ++j)
-                    if (j <= lane) sp[pk(kb + lane, kb + j)] = a[j];
-            }
-            if (lane == 0 && f) s_fail = kb + f;
+        for (int j = 0; j < CP_B; ++j) {
+            const double v = sp[pk(lane > j ? lane : j, j)];  // in range for every lane
+            a[j] = (lane >= j && lane < w) ? v : 0.0;
         }
-        __syncthreads();
-        if (s_fail) return s_fail;
-        const int r0 = kb + bw, rem = w - r0;  // rows below the block
-        // 2. rows below: X := B L^-T (one lane per row)
-        if (t < rem) {
-            const int i = r0 + t;
-            double x[CP_S];
+        int f = 0;
 #pragma unroll
-            for (int c = 0; c < CP_S; ++c) x[c] = c < bw ? sp[pk(i, kb + c)] : 0.0;
-#pragma unroll
-            for (int c = 0; c < CP_S; ++c) {
-                if (c < bw) {
-                    x[c] = x[c] * rinv[c];
-#pragma unroll
-                    for (int j = c + 1; j < CP_S; ++j) x[j] = fma(-x[c], sp[pk(kb + j, kb + c)], x[j]);
-                }
+        for (int c = 0; c < CP_B; ++c) {
+            if (c >= w || f) break;
+            const double piv = readlane_f64(a[c], c);
+            if (!(piv > 0.0)) {
+                f = c + 1;
+                break;
             }
+            const double l = sqrt(piv);
+            const double rl = 1.0 / l;
+            a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
+            if (c + 1 < CP_B) {
+                colbuf[lane] = a[c];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int c = 0; c < CP_S; ++c)
-                if (c < bw) sp[pk(i, kb + c)] = x[c];
-        }
-        __syncthreads();
-        // 3. trailing lower block: L(i, j) -= sum_c L(i, kb + c) L(j, kb + c), r0 <= j <= i < w
-        constexpr int RM = CP_B - CP_S;  // largest trailing order (48); entries beyond rem skip
-        for (int idx = t; idx < RM * RM; idx += 256) {
-            const int io = idx % RM, jo = idx / RM;
-            if (io < rem && jo <= io) {
-                const int i = r0 + io, j = r0 + jo;
-                double v = sp[pk(i, j)];
-#pragma unroll
-                for (int c = 0; c < CP_S; ++c)
-                    if (c < bw) v = fma(-sp[pk(i, kb + c)], sp[pk(j, kb + c)], v);
-                sp[pk(i, j)] = v;
+                for (int j = c + 1; j < CP_B; ++j) a[j] = fma(-a[c], colbuf[j], a[j]);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();  // every lane has read before the next write
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
-        __syncthreads();
+        if (lane < w) {
+#pragma unroll
+            for (int j = 0; j < CP_B; ++j)
+                if (j <= lane) sp[pk(lane, j)] = a[j];
+        }
+        if (lane == 0) s_fail = f;
     }
-    return 0;
+    __syncthreads();
+    return s_fail;
 }
 
 // Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L packed in sp and the
@@ -218,7 +179,7 @@ __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w
     const int t = threadIdx.x;
     stage_packed64(sp, A, lda, w);
     __syncthreads();
-    const int fail = potf2_block64(sp, w);
+    const int fail = potf2_wave64(sp, w);
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;

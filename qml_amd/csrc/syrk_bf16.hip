@@ -14,6 +14,12 @@
 //
 // Tiles: 256 lanes own a 128 x 128 lower tile of C (2 x 2 waves of 64 x 64); chunks of 32
 // features are double buffered through LDS (4 panels x 8 KB per buffer).
+//
+// Tile order. Workgroups are handed to the 8 XCDs round robin (workgroup b to XCD b % 8), and
+// each XCD has its own L2. The tiles are grouped in 8 x 8 super-blocks, and the workgroup
+// index is remapped so that the 64 workgroups an XCD holds at a time are the 64 tiles of one
+// super-block: they share 8 row blocks and 8 column blocks of the panel, which that XCD's L2
+// then fetches once instead of once per tile row.
 #include "common.h"
 #include "timer.h"
 
@@ -85,6 +91,16 @@ __device__ __forceinline__ bf16x8_t sb_neg(bf16x8_t v) {
 // issued together with the first panel chunk, so their latency hides behind the prologue --
 // and the products are subtracted by negating the second operand's fragments; the epilogue
 // only stores.
+static constexpr int SB_SUPER = 8;  // super-block side in tiles
+static constexpr int SB_XCDS = 8;
+
+// Launch size for t x t tiles: 64 workgroups per lower super-block, padded to a multiple of
+// 8 super-blocks so that the remap below is a bijection.
+__host__ __device__ inline int64_t sb_super_count(int64_t t) {
+    const int64_t T = (t + SB_SUPER - 1) / SB_SUPER;
+    return T * (T + 1) / 2;
+}
+
 __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __restrict__ Ph,
                                                              const __bf16* __restrict__ Pl,
                                                              int64_t m, int64_t m_pad,
@@ -94,11 +110,19 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
     // buffers x {row hi, row lo, col hi, col lo}
     __shared__ __attribute__((aligned(16))) __bf16 lds[2][4][SB_PANEL];
     if (info && *info) return;
+    // workgroup -> (XCD x, its l-th workgroup) -> super-block 8 (l / 64) + x, tile l % 64
     const int64_t bid = blockIdx.x;
-    int64_t bi = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
-    while (bi * (bi + 1) / 2 > bid) --bi;
-    while ((bi + 1) * (bi + 2) / 2 <= bid) ++bi;
-    const int64_t bj = bid - bi * (bi + 1) / 2;  // bi >= bj
+    const int64_t x = bid % SB_XCDS, l = bid / SB_XCDS;
+    const int64_t sbi = (l / (SB_SUPER * SB_SUPER)) * SB_XCDS + x;
+    const int64_t lt = l % (SB_SUPER * SB_SUPER);
+    const int64_t t_tiles = m_pad / SB_T;
+    if (sbi >= sb_super_count(t_tiles)) return;
+    int64_t I = (int64_t)((sqrt(8.0 * (double)sbi + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > sbi) --I;
+    while ((I + 1) * (I + 2) / 2 <= sbi) ++I;
+    const int64_t J = sbi - I * (I + 1) / 2;  // I >= J
+    const int64_t bi = I * SB_SUPER + lt / SB_SUPER, bj = J * SB_SUPER + lt % SB_SUPER;
+    if (bi >= t_tiles || bj > bi) return;
     const int64_t rows0 = bi * SB_T, cols0 = bj * SB_T;
 
     const int t = threadIdx.x;
@@ -229,8 +253,10 @@ int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, in
                        dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl);
     QMLH_CHECK_LAUNCH("split_panel launch");
     const int64_t tm = m_pad / SB_T;
+    const int64_t groups = ceil_div(sb_super_count(tm), (int64_t)SB_XCDS);
     TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
-    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3((unsigned)(tm * (tm + 1) / 2)), dim3(256), 0, s,
+    hipLaunchKernelGGL(syrk_bf16x3_kernel,
+                       dim3((unsigned)(groups * SB_XCDS * SB_SUPER * SB_SUPER)), dim3(256), 0, s,
                        Ph, Pl, m, m_pad, kb_pad, C, ldc, info);
     QMLH_CHECK_LAUNCH("syrk_bf16x3 launch");
     return QMLH_OK;
