@@ -35,30 +35,23 @@ __global__ void trsv_prepare_kernel(int32_t* __restrict__ sync, int64_t count,
     if (i < n) out[i] = __builtin_bit_cast(double, TS_PENDING);
 }
 
-// The published value at p, waiting (bounded) while it is pending. Read with an agent-scope
-// atomic load, which goes to the point of coherence, instead of an agent-scope acquire fence:
-// such a fence invalidates the whole L2 of the XCD and would throw away the tiles every other
-// workgroup has in flight. On timeout sets *err, clears ok and returns 0.
-__device__ __forceinline__ double wait_value(double* p, int32_t* err, bool& ok) {
-    unsigned spins = 0;
-    while (true) {
-        const double v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__builtin_bit_cast(unsigned long long, v) != TS_PENDING) return v;
-        if (++spins > TS_SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = false;
-            return 0.0;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
 template <typename T, bool FWD>
 __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L, int64_t ldl,
                                             int64_t n, int64_t r0, int w, int64_t j, int lane,
                                             int grp) {
     const int64_t c0 = j * TS_B;
     const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
+    // Full tiles (all but the last row or column block) load without any condition, so the 16
+    // loads stay in flight together; a condition on the lane would put each load in its own
+    // branch and wait for it there.
+    if (w == TS_B && wj == TS_B) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int cc = 16 * grp + c;
+            v[c] = FWD ? L[(r0 + lane) + (c0 + cc) * ldl] : L[(c0 + lane) + (r0 + cc) * ldl];
+        }
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int cc = 16 * grp + c;
@@ -69,11 +62,83 @@ __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L,
     }
 }
 
+// One dependency block in flight: the lane's 16 tile values and one published value (FWD:
+// z of column c0 + lane of block j, shared by the wave through lane reads; BWD: x of row
+// c0 + lane of block j, the lane's own).
+template <typename T>
+struct TsStage {
+    T l[16];
+    double v;
+};
+
+// Issue the loads of dependency q (tile + published value); nothing if q >= nj.
+template <typename T, bool FWD>
+__device__ __forceinline__ void ts_issue(TsStage<T>& st, const T* __restrict__ L, int64_t ldl,
+                                         int64_t n, int64_t r0, int w, int64_t nblocks,
+                                         int64_t q, int64_t nj, const double* out, int lane,
+                                         int grp) {
+    if (q >= nj) return;
+    const int64_t j = FWD ? q : nblocks - 1 - q;
+    load_tile16<T, FWD>(st.l, L, ldl, n, r0, w, j, lane, grp);
+    const int64_t c0 = j * TS_B;
+    const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
+    st.v = lane < wj ? __hip_atomic_load(&out[c0 + lane], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT)
+                     : 0.0;
+}
+
+// Wait (bounded) until no lane of the wave holds a pending value, re-reading the pending ones.
+__device__ __forceinline__ void ts_settle(double& v, const double* p, bool live, int32_t* err,
+                                          bool& ok) {
+    unsigned spins = 0;
+    while (true) {
+        const bool pending = live && __builtin_bit_cast(unsigned long long, v) == TS_PENDING;
+        if (!__any(pending)) return;
+        if (++spins > TS_SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = false;
+            v = 0.0;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (pending) v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Consume dependency q: settle its published values, then accumulate its contributions.
+template <typename T, bool FWD>
+__device__ __forceinline__ void ts_consume(TsStage<T>& st, int64_t n, int64_t nblocks,
+                                           int64_t q, int64_t nj, const double* out, int lane,
+                                           int grp_s, int32_t* err, bool& ok, double& acc1,
+                                           double (&acc16)[16]) {
+    if (q >= nj) return;
+    const int64_t j = FWD ? q : nblocks - 1 - q;
+    const int64_t c0 = j * TS_B;
+    const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
+    if (ok) ts_settle(st.v, &out[c0 + (lane < wj ? lane : 0)], lane < wj, err, ok);
+    if (!ok) st.v = 0.0;
+    if (FWD) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            acc1 = fma((double)st.l[c], readlane_f64(st.v, 16 * grp_s + c), acc1);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc16[c] = fma((double)st.l[c], st.v, acc16[c]);
+    }
+}
+
 // Solve block b of one system: wait for the values of the blocks it depends on (the earlier
 // blocks in solve order), accumulate their contributions, solve the diagonal block, publish.
 // `err` is the shared error word. With dinv (the FP64 inverses of the 64 x 64 diagonal blocks,
 // trsv_diag_inverse) the diagonal solve is a matrix-vector product spread over all lanes
 // instead of a 64-step substitution on the critical path. All 256 lanes call this.
+//
+// Dependencies are pipelined TS_DEPTH deep: the tile and published values of the next blocks
+// are in flight while the current one is accumulated, so a block whose dependencies are long
+// finished pays about one memory round trip per TS_DEPTH dependencies, not one per dependency.
+// The accumulation order is unchanged (dependencies in solve order).
+static constexpr int TS_DEPTH = 4;
+
 template <typename T, bool FWD>
 __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl, int64_t n,
                                            const double* __restrict__ rhs, double* out,
@@ -90,6 +155,7 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     const int64_t r0 = b * TS_B;
     const int w = (int)(n - r0 < TS_B ? n - r0 : TS_B);
     const int lane = t & 63, grp = t >> 6;  // 4 groups of 16 columns
+    const int grp_s = __builtin_amdgcn_readfirstlane(grp);  // wave-uniform
 
     // The diagonal block does not depend on other blocks: stage it first.
     if (dinv) {
@@ -118,46 +184,21 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     for (int c = 0; c < 16; ++c) acc16[c] = 0.0;
     bool ok = true;
     const int64_t nj = FWD ? b : nblocks - 1 - b;
-    T lv[16], ln[16];
-    if (nj > 0) load_tile16<T, FWD>(lv, L, ldl, n, r0, w, FWD ? 0 : nblocks - 1, lane, grp);
-    for (int64_t q = 0; q < nj; ++q) {
-        const int64_t j = FWD ? q : nblocks - 1 - q;
-        if (q + 1 < nj) load_tile16<T, FWD>(ln, L, ldl, n, r0, w, FWD ? q + 1 : j - 1, lane, grp);
-        const int64_t c0 = j * TS_B;
-        const int wj = (int)(n - c0 < TS_B ? n - c0 : TS_B);
-        if (FWD) {
-            // all 16 values in one round of loads; poll again only while one is pending
-            double zc[16];
-            unsigned spins = 0;
-            while (true) {
-                bool pending = false;
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int cc = 16 * grp + c;
-                    zc[c] = (cc < wj && ok) ? __hip_atomic_load(&out[c0 + cc], __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT)
-                                            : 0.0;
-                    pending |= __builtin_bit_cast(unsigned long long, zc[c]) == TS_PENDING;
-                }
-                if (!pending) break;
-                if (++spins > TS_SPIN_LIMIT) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = false;
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) zc[c] = 0.0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-#pragma unroll
-            for (int c = 0; c < 16; ++c) acc1 = fma((double)lv[c], zc[c], acc1);
-        } else {
-            const double xj = (lane < wj && ok) ? wait_value(&out[c0 + lane], err, ok) : 0.0;
-#pragma unroll
-            for (int c = 0; c < 16; ++c) acc16[c] = fma((double)lv[c], xj, acc16[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < 16; ++c) lv[c] = ln[c];
+    static_assert(TS_DEPTH == 4, "the ring below is written out for depth 4");
+    TsStage<T> s0, s1, s2, s3;
+    ts_issue<T, FWD>(s0, L, ldl, n, r0, w, nblocks, 0, nj, out, lane, grp);
+    ts_issue<T, FWD>(s1, L, ldl, n, r0, w, nblocks, 1, nj, out, lane, grp);
+    ts_issue<T, FWD>(s2, L, ldl, n, r0, w, nblocks, 2, nj, out, lane, grp);
+    ts_issue<T, FWD>(s3, L, ldl, n, r0, w, nblocks, 3, nj, out, lane, grp);
+    for (int64_t q = 0; q < nj; q += TS_DEPTH) {
+        ts_consume<T, FWD>(s0, n, nblocks, q, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s0, L, ldl, n, r0, w, nblocks, q + 4, nj, out, lane, grp);
+        ts_consume<T, FWD>(s1, n, nblocks, q + 1, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s1, L, ldl, n, r0, w, nblocks, q + 5, nj, out, lane, grp);
+        ts_consume<T, FWD>(s2, n, nblocks, q + 2, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s2, L, ldl, n, r0, w, nblocks, q + 6, nj, out, lane, grp);
+        ts_consume<T, FWD>(s3, n, nblocks, q + 3, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s3, L, ldl, n, r0, w, nblocks, q + 7, nj, out, lane, grp);
     }
     if (!ok) s_fail = 1;  // benign race: every writer stores 1
 
