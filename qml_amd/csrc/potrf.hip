@@ -214,7 +214,21 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         const int64_t kb2 = m < nb ? m : nb;  // next block column
         T* P = A + (k + kb) + k * lda;
         T* C = P + kb * lda;  // A(k + kb, k + kb)
-        rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
+        const bool split = std::is_same<T, float>::value && bf16_ws && m > kb2;
+        if (split) {
+            // split-BF16 trailing updates: split the panel once; the next block column gets
+            // the same split-BF16 update as the rest (its extra rows FP32), then the rest of
+            // the trailing matrix reuses the split from row kb2 on
+            float* Pf = reinterpret_cast<float*>(P);
+            float* Cf = reinterpret_cast<float*>(C);
+            rc = bf16x3_split(Pf, lda, m, kb, bf16_ws, s);
+            if (rc == QMLH_OK)
+                rc = bf16x3_update(bf16_ws, m, kb, 0, m, kb2, Cf, lda, info, s);
+            if (rc == QMLH_OK && r > 0)
+                rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda, info, s);
+        } else {
+            rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
+        }
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
         if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2);
@@ -222,8 +236,14 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         if (rc == QMLH_OK && ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s2);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
-        if (rc == QMLH_OK && m > kb2)
-            rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s, bf16_ws);
+        if (rc == QMLH_OK && m > kb2) {
+            if (split)
+                rc = bf16x3_update(bf16_ws, m, kb, kb2, m - kb2, 0,
+                                   reinterpret_cast<float*>(C + kb2 + kb2 * lda), lda, info, s);
+            else
+                rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s,
+                                      bf16_ws);
+        }
         if (rc == QMLH_OK && !ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");

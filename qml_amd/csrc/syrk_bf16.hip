@@ -40,19 +40,32 @@ __device__ __forceinline__ int64_t split_index(int64_t row, int64_t k, int64_t m
 }
 
 // Ph, Pl <- split of P (column-major FP32, m x kb); rows m .. m_pad-1 and features
-// kb .. kb_pad-1 are zero.
-__global__ void split_panel_kernel(const float* __restrict__ P, int64_t ldp, int64_t m,
-                                   int64_t m_pad, int64_t kb, int64_t kb_pad,
-                                   __bf16* __restrict__ Ph, __bf16* __restrict__ Pl) {
+// kb .. kb_pad-1 are zero. One lane per (row, group of 8 features): 8 strided loads (lanes
+// run down the rows, so each is coalesced) and one 16-byte store into each of Ph and Pl.
+__global__ __launch_bounds__(256) void split_panel_kernel(const float* __restrict__ P,
+                                                          int64_t ldp, int64_t m, int64_t m_pad,
+                                                          int64_t kb, int64_t kb_pad,
+                                                          __bf16* __restrict__ Ph,
+                                                          __bf16* __restrict__ Pl) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t k = blockIdx.y;
-    if (row >= m_pad || k >= kb_pad) return;
-    const float v = (row < m && k < kb) ? P[row + k * ldp] : 0.0f;
-    const __bf16 h = (__bf16)v;
-    const __bf16 l = (__bf16)(v - (float)h);
-    const int64_t idx = split_index(row, k, m_pad);
-    Ph[idx] = h;
-    Pl[idx] = l;
+    const int64_t k0 = (int64_t)blockIdx.y * 8;
+    if (row >= m_pad || k0 >= kb_pad) return;
+    // loads from clamped indices (no per-load branch), zero selected afterwards
+    const int64_t rr = row < m ? row : m - 1;
+    float raw[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) raw[e] = P[rr + (k0 + e < kb ? k0 + e : kb - 1) * ldp];
+    bf16x8_t h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float v = (row < m && k0 + e < kb) ? raw[e] : 0.0f;
+        const __bf16 hv = (__bf16)v;
+        h[e] = hv;
+        l[e] = (__bf16)(v - (float)hv);
+    }
+    const int64_t idx = split_index(row, k0, m_pad);  // k0 % 8 == 0: 8 contiguous elements
+    *reinterpret_cast<bf16x8_t*>(Ph + idx) = h;
+    *reinterpret_cast<bf16x8_t*>(Pl + idx) = l;
 }
 
 // Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) into an LDS panel.
@@ -101,28 +114,41 @@ __host__ __device__ inline int64_t sb_super_count(int64_t t) {
     return T * (T + 1) / 2;
 }
 
+// Ph / Pl point at row `off` of a split panel with row stride m_stride (its m_pad); the update
+// covers m rows from there. ncols == 0: the lower triangle of the m x m block (super-block
+// order); ncols > 0: the entries i >= j of its first ncols columns (the next block column of a
+// look-ahead factorisation), tiles in row order.
 __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __restrict__ Ph,
                                                              const __bf16* __restrict__ Pl,
                                                              int64_t m, int64_t m_pad,
-                                                             int64_t kb_pad,
+                                                             int64_t kb_pad, int64_t ncols,
                                                              float* __restrict__ C, int64_t ldc,
                                                              const int32_t* __restrict__ info) {
     // buffers x {row hi, row lo, col hi, col lo}
     __shared__ __attribute__((aligned(16))) __bf16 lds[2][4][SB_PANEL];
     if (info && *info) return;
-    // workgroup -> (XCD x, its l-th workgroup) -> super-block 8 (l / 64) + x, tile l % 64
     const int64_t bid = blockIdx.x;
-    const int64_t x = bid % SB_XCDS, l = bid / SB_XCDS;
-    const int64_t sbi = (l / (SB_SUPER * SB_SUPER)) * SB_XCDS + x;
-    const int64_t lt = l % (SB_SUPER * SB_SUPER);
-    const int64_t t_tiles = m_pad / SB_T;
-    if (sbi >= sb_super_count(t_tiles)) return;
-    int64_t I = (int64_t)((sqrt(8.0 * (double)sbi + 1.0) - 1.0) * 0.5);
-    while (I * (I + 1) / 2 > sbi) --I;
-    while ((I + 1) * (I + 2) / 2 <= sbi) ++I;
-    const int64_t J = sbi - I * (I + 1) / 2;  // I >= J
-    const int64_t bi = I * SB_SUPER + lt / SB_SUPER, bj = J * SB_SUPER + lt % SB_SUPER;
+    const int64_t t_tiles = (m + SB_T - 1) / SB_T;
+    int64_t bi, bj;
+    if (ncols > 0) {
+        const int64_t tc = (ncols + SB_T - 1) / SB_T;
+        bi = bid / tc;
+        bj = bid % tc;
+    } else {
+        // workgroup -> (XCD x, its l-th workgroup) -> super-block 8 (l / 64) + x, tile l % 64
+        const int64_t x = bid % SB_XCDS, l = bid / SB_XCDS;
+        const int64_t sbi = (l / (SB_SUPER * SB_SUPER)) * SB_XCDS + x;
+        const int64_t lt = l % (SB_SUPER * SB_SUPER);
+        if (sbi >= sb_super_count(t_tiles)) return;
+        int64_t I = (int64_t)((sqrt(8.0 * (double)sbi + 1.0) - 1.0) * 0.5);
+        while (I * (I + 1) / 2 > sbi) --I;
+        while ((I + 1) * (I + 2) / 2 <= sbi) ++I;
+        const int64_t J = sbi - I * (I + 1) / 2;  // I >= J
+        bi = I * SB_SUPER + lt / SB_SUPER;
+        bj = J * SB_SUPER + lt % SB_SUPER;
+    }
     if (bi >= t_tiles || bj > bi) return;
+    const int64_t ncol = ncols > 0 ? ncols : m;  // column bound
     const int64_t rows0 = bi * SB_T, cols0 = bj * SB_T;
 
     const int t = threadIdx.x;
@@ -133,11 +159,11 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
     const int fr = 64 * (w & 1), sr = 64 * (w >> 1);
     const int64_t col0 = cols0 + fr;
     const int64_t row0 = rows0 + sr;
-    const bool live = row0 < m && col0 < m && col0 <= row0 + 63;
+    const bool live = row0 < m && col0 < ncol && col0 <= row0 + 63;
 
     // interior: the wave's 64 x 64 block lies inside the matrix and on or below the diagonal
     // in every entry, so its loads and stores need no masks.
-    const bool interior = live && row0 + 63 < m && col0 + 63 <= row0;
+    const bool interior = live && row0 + 63 < m && col0 + 63 < ncol && col0 + 63 <= row0;
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 acc[4][4];
     if (interior) {
@@ -157,7 +183,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
                 for (int e = 0; e < 4; ++e) {
                     const int64_t i = row0 + 16 * b + r;
                     const int64_t j = col0 + 16 * a + 4 * g + e;  // accumulator rows 4 g + e
-                    acc[a][b][e] = (live && i < m && j <= i) ? C[i + j * ldc] : 0.0f;
+                    acc[a][b][e] = (live && i < m && j <= i && j < ncol) ? C[i + j * ldc] : 0.0f;
                 }
     }
 
@@ -227,7 +253,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int64_t j = col0 + 16 * a + 4 * g + e;
-                if (j > i) continue;
+                if (j > i || j >= ncol) continue;
                 C[i + j * ldc] = acc[a][b][e];
             }
         }
@@ -240,26 +266,53 @@ size_t syrk_bf16_workspace(int64_t n, int64_t kb) {
     return (size_t)(2 * m_pad * kb_pad) * sizeof(__bf16);
 }
 
-// C (m x m, lower) -= P P^T with P (m x kb) column-major FP32; ws holds
-// syrk_bf16_workspace(m, kb) bytes.
-int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
-                const int32_t* info, void* ws, hipStream_t s) {
+int bf16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
     __bf16* Ph = static_cast<__bf16*>(ws);
     __bf16* Pl = Ph + m_pad * kb_pad;
-    hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)kb_pad),
+    hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)(kb_pad / 8)),
                        dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl);
     QMLH_CHECK_LAUNCH("split_panel launch");
-    const int64_t tm = m_pad / SB_T;
-    const int64_t groups = ceil_div(sb_super_count(tm), (int64_t)SB_XCDS);
-    TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
-    hipLaunchKernelGGL(syrk_bf16x3_kernel,
-                       dim3((unsigned)(groups * SB_XCDS * SB_SUPER * SB_SUPER)), dim3(256), 0, s,
-                       Ph, Pl, m, m_pad, kb_pad, C, ldc, info);
+    return QMLH_OK;
+}
+
+int bf16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
+                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s) {
+    if (m <= 0 || kb <= 0) return QMLH_OK;
+    const int64_t m_pad = round_up(m_split, SB_T);
+    const int64_t kb_pad = round_up(kb, SB_KC);
+    if (off % SB_T != 0 || off + round_up(m, SB_T) > m_pad)
+        return set_error(QMLH_ERR_ARG, "bf16x3_update: rows %lld + %lld outside the split",
+                         (long long)off, (long long)m);
+    const __bf16* Ph = static_cast<const __bf16*>(ws) + off * 8;
+    const __bf16* Pl = static_cast<const __bf16*>(ws) + m_pad * kb_pad + off * 8;
+    const int64_t tm = ceil_div(m, (int64_t)SB_T);
+    int64_t grid;
+    double flops;
+    if (ncols > 0) {
+        grid = tm * ceil_div(ncols, (int64_t)SB_T);
+        flops = ((double)m * (double)ncols - (double)ncols * (double)(ncols - 1) / 2.0) * 2.0 *
+                (double)kb;
+    } else {
+        grid = ceil_div(sb_super_count(tm), (int64_t)SB_XCDS) * SB_XCDS * SB_SUPER * SB_SUPER;
+        flops = (double)m * (double)(m + 1) * (double)kb;
+    }
+    TimerScope ts(ncols > 0 ? QMLH_TIMER_GEMM : QMLH_TIMER_SYRK, flops, s);
+    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Ph, Pl, m,
+                       m_pad, kb_pad, ncols, C, ldc, info);
     QMLH_CHECK_LAUNCH("syrk_bf16x3 launch");
     return QMLH_OK;
+}
+
+// C (m x m, lower) -= P P^T with P (m x kb) column-major FP32; ws holds
+// syrk_bf16_workspace(m, kb) bytes.
+int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
+                const int32_t* info, void* ws, hipStream_t s) {
+    if (m <= 0 || kb <= 0) return QMLH_OK;
+    QMLH_TRY(bf16x3_split(P, ldp, m, kb, ws, s));
+    return bf16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s);
 }
 
 }  // namespace qmlh
