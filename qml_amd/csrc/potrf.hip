@@ -193,19 +193,21 @@ static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 // bandwidth/MFMA-bound trailing update. Events order the two streams; the caller's stream waits
 // for the last panel, so on return all work is ordered on `s`.
 //
-// With r extra rows (A[n:n+r, :]), the next block column's update and panel include them. The
-// rest of their trailing update touches nothing the trailing update or the panel reads or
-// writes: while the trailing matrix is large (the caller's stream is the critical path) it runs
-// on the side stream after the panel, later (the panels are the critical path) on the caller's
-// stream after the trailing update.
+// With r extra rows (A[n:n+r, :]), the next block column's update and panel include them (with
+// split-BF16 updates, their part of the next block column is an FP32 product on the side
+// stream). The rest of their trailing update touches nothing the trailing update or the panel
+// reads or writes: while the trailing matrix is large (the caller's stream is the critical
+// path) it runs on the side stream after the panel, later (the panels are the critical path)
+// on the caller's stream after the trailing update.
 template <typename T>
 static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
                            hipStream_t s, void* bf16_ws, int64_t nb) {
     hipStream_t s2 = nullptr;
-    hipEvent_t ev_a = nullptr, ev_p = nullptr;
+    hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr;
     int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming), "potrf: event");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_p, hipEventDisableTiming), "potrf: event");
+    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_x, hipEventDisableTiming), "potrf: event");
     if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s);
     for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
@@ -221,11 +223,20 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
             // the trailing matrix reuses the split from row kb2 on
             float* Pf = reinterpret_cast<float*>(P);
             float* Cf = reinterpret_cast<float*>(C);
-            rc = bf16x3_split(Pf, lda, m, kb, bf16_ws, s);
+            // the extra rows' part (a small FP32 product) on the side stream, beside the split
+            // and the BF16 update: it needs the finished panel and the extra rows' earlier
+            // updates (both ordered before ev_x on the caller's stream), and the side stream
+            // runs it before the next panel
+            if (r > 0) {
+                rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
+                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_x, 0), "potrf: wait");
+                if (rc == QMLH_OK)
+                    rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
+                                          info, s2);
+            }
+            if (rc == QMLH_OK) rc = bf16x3_split(Pf, lda, m, kb, bf16_ws, s);
             if (rc == QMLH_OK)
                 rc = bf16x3_update(bf16_ws, m, kb, 0, m, kb2, Cf, lda, info, s);
-            if (rc == QMLH_OK && r > 0)
-                rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda, info, s);
         } else {
             rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
         }
@@ -251,6 +262,7 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
     // Destroying them here is safe: pending work completes first, and `s` already waits for it.
     if (ev_a) (void)hipEventDestroy(ev_a);
     if (ev_p) (void)hipEventDestroy(ev_p);
+    if (ev_x) (void)hipEventDestroy(ev_x);
     if (s2) (void)hipStreamDestroy(s2);
     return rc;
 }
