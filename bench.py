@@ -54,7 +54,9 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=10000,
                    help="training molecules in the CPU-baseline sample (0 = skip)")
     p.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    p.add_argument("--tol", type=float, default=1e-10)
+    # Refinement stops at |y - K alpha| <= tol |y|. 1e-9 keeps the prediction difference to the
+    # FP64 fit near 1e-8 of std(y) (measured below), far inside the 1e-6 target.
+    p.add_argument("--tol", type=float, default=1e-9)
     p.add_argument("--serve-rate", type=float, default=500.0,
                    help="Poisson arrival rate (requests/s) of the serving-load phase")
     p.add_argument("--serve-seconds", type=float, default=4.0)

@@ -35,6 +35,17 @@ template <typename T>
 int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
                   int32_t* sync, bool forward, hipStream_t s, const double* dinv = nullptr);
 
+// Orders from which the triangular solves run as one sync-free launch.
+static constexpr int64_t TRSV_SYNCFREE_MIN = 1024;
+// 128-row-block solves for a factor whose 128 x 128 diagonal inverses are known: workspace
+// doubles, the inverses (from the 64 x 64 ones, written to dinv64 first), and the solve.
+size_t trsv_dinv128_doubles(int64_t n);
+template <typename T>
+int trsv_inverse128(const T* L, int64_t n, int64_t ldl, double* dinv64, double* dinv128,
+                    hipStream_t s);
+template <typename T>
+int trsv_syncfree128(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
+                     int32_t* sync, bool forward, hipStream_t s, const double* dinv128);
 // Doubles trsv_diag_inverse writes for order n, and the inverses themselves.
 size_t trsv_dinv_doubles(int64_t n);
 template <typename T>

@@ -341,8 +341,8 @@ struct PcgCtx {
     hipStream_t s;
     int64_t n;
     double* partial;
-    int32_t* sync;        // triangular-solve synchronisation words
-    const double* dinv;   // FP64 inverses of the factor's 64 x 64 diagonal blocks
+    int32_t* sync;           // triangular-solve synchronisation words
+    const double* dinv128;   // FP64 inverses of the factor's 128 x 128 diagonal blocks
 };
 
 static inline unsigned grid1(int64_t n) { return (unsigned)ceil_div(n, 256); }
@@ -375,7 +375,11 @@ struct Woodbury {
 static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury& wb,
                    const double* r, double* z, double* q, double* tmp) {
     QMLH_TRY(copy(c, r, tmp));
-    QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, c.sync, c.s, c.dinv));
+    const bool big = c.n >= TRSV_SYNCFREE_MIN;
+    if (big)
+        QMLH_TRY(trsv_syncfree128<float>(L, c.n, ldl, tmp, q, c.sync, true, c.s, c.dinv128));
+    else
+        QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, nullptr, c.s));
     if (wb.r > 0) {
         hipLaunchKernelGGL(yt_q_kernel, dim3((unsigned)wb.r), dim3(256), 0, c.s, wb.Y, wb.ldy, c.n,
                            q, wb.t);
@@ -385,7 +389,8 @@ static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury&
                            c.n, wb.r, wb.t, q);
         QMLH_CHECK_LAUNCH("pcg q - Y t launch");
     }
-    return trsv_lower_bwd<float>(L, c.n, ldl, q, z, c.sync, c.s, c.dinv);
+    if (big) return trsv_syncfree128<float>(L, c.n, ldl, q, z, c.sync, false, c.s, c.dinv128);
+    return trsv_lower_bwd<float>(L, c.n, ldl, q, z, nullptr, c.s);
 }
 
 }  // namespace qmlh
@@ -481,7 +486,7 @@ extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
     if (n <= 0) return 0;
     if (r < 0) r = 0;
     return (size_t)(5 * n + 2 * r + qmlh::RED_BLOCKS + 8) * sizeof(double) +
-           qmlh::trsv_dinv_doubles(n) * sizeof(double) +
+           (qmlh::trsv_dinv_doubles(n) + qmlh::trsv_dinv128_doubles(n)) * sizeof(double) +
            qmlh::symv_ws_doubles(n) * sizeof(double) +
            qmlh::trsv_sync_ints(n) * sizeof(int32_t);
 }
@@ -513,14 +518,15 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     double* partial = tmp_r + r;
     double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 6 rz_new
     double* dinv = sc + 8;
-    double* symws = dinv + trsv_dinv_doubles(n);
+    double* dinv128 = dinv + trsv_dinv_doubles(n);
+    double* symws = dinv128 + trsv_dinv128_doubles(n);
     int32_t* sync = reinterpret_cast<int32_t*>(symws + symv_ws_doubles(n));
-    PcgCtx c{s, n, partial, sync, dinv};
+    PcgCtx c{s, n, partial, sync, dinv128};
     Woodbury wb{Y, ldy, r, C, ldc, t, tmp_r};
-    // The preconditioner's triangular solves use explicit inverses of the diagonal blocks of
-    // the low-precision factor (computed once here), so each 64-row block's diagonal solve is
-    // a parallel product instead of a sequential substitution.
-    QMLH_TRY(trsv_diag_inverse<float>(L, n, ldl, dinv, s));
+    // The preconditioner's triangular solves use explicit FP64 inverses of the 128 x 128
+    // diagonal blocks of the low-precision factor (computed once here), so each block's
+    // diagonal solve is one parallel product instead of a sequential substitution.
+    if (n >= TRSV_SYNCFREE_MIN) QMLH_TRY(trsv_inverse128<float>(L, n, ldl, dinv, dinv128, s));
 
     double yy_h = 0.0;
     QMLH_TRY(dot(c, y, y, sc + 2));

@@ -397,7 +397,6 @@ __global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const T* __restrict_
 }
 
 // Below this order the per-step kernels are used (few steps, no spin-waiting).
-static constexpr int64_t TRSV_SYNCFREE_MIN = 1024;
 
 template <typename T>
 int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,

@@ -336,4 +336,214 @@ template int trsv_syncfree<float>(const float*, int64_t, int64_t, const double*,
 template int trsv_diag_inverse<double>(const double*, int64_t, int64_t, double*, hipStream_t);
 template int trsv_diag_inverse<float>(const float*, int64_t, int64_t, double*, hipStream_t);
 
+// ------------------------------------------------------------------------------------------
+// 128-row blocks for solves that have diagonal-block inverses (the preconditioner's). One
+// workgroup of 512 lanes per 128-row block: each 256-lane half accumulates the dependencies of
+// its 64 rows (FWD) or 64 columns (BWD) exactly as trsv_block does, then the 128 residual
+// values are multiplied by the FP64 inverse of the 128 x 128 diagonal block in one step. The
+// dependency chain of the solve has half as many links, and each link costs one diagonal
+// product instead of two solves and a hand-over. Published values are read by later blocks in
+// 64-column units, as before.
+// ------------------------------------------------------------------------------------------
+static constexpr int TS_B2 = 2 * TS_B;
+
+// dinv128 block B := L_BB^-1 (128 x 128, column-major) from the 64 x 64 inverses D0 = dinv64
+// block 2B and D1 = block 2B + 1 and the off-diagonal block L10:
+//     [D0 0; X D1],  X = -D1 (L10 D0),
+// all in FP64. Entries outside the matrix are zero. 256 lanes per block.
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_inverse128_kernel(const T* __restrict__ L,
+                                                              int64_t ldl, int64_t n,
+                                                              const double* __restrict__ dinv64,
+                                                              double* __restrict__ dinv128) {
+    __shared__ double sa[TS_B][TS_B + 1], sb[TS_B][TS_B + 1];
+    const int64_t B = blockIdx.x;
+    const int64_t r0 = B * TS_B2;
+    const int w1 = (int)(n - r0 - TS_B < TS_B ? n - r0 - TS_B : TS_B);  // rows of the 2nd half
+    const int t = threadIdx.x;
+    const int i = t & 63, cg = t >> 6;  // output row, group of 16 output columns
+    const double* D0 = dinv64 + (2 * B) * (TS_B * TS_B);
+    double* Dout = dinv128 + B * (TS_B2 * TS_B2);
+    double xr[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) xr[c] = 0.0;
+    if (w1 > 0) {  // uniform
+        const double* D1 = dinv64 + (2 * B + 1) * (TS_B * TS_B);
+        // sa = L10 (rows r0 + 64 .., columns r0 ..), sb = D0
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int ii = idx & 63, kk = idx >> 6;
+            sa[ii][kk] = ii < w1 ? (double)L[(r0 + TS_B + ii) + (r0 + kk) * ldl] : 0.0;
+            sb[ii][kk] = D0[ii + kk * TS_B];
+        }
+        __syncthreads();
+        double tr[16];  // T = L10 D0, row i, columns 16 cg ..
+#pragma unroll
+        for (int c = 0; c < 16; ++c) tr[c] = 0.0;
+        for (int k = 0; k < TS_B; ++k) {
+            const double a = sa[i][k];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) tr[c] = fma(a, sb[k][16 * cg + c], tr[c]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 16; ++c) sa[i][16 * cg + c] = tr[c];  // sa = T
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int ii = idx & 63, kk = idx >> 6;
+            sb[ii][kk] = D1[ii + kk * TS_B];  // sb = D1
+        }
+        __syncthreads();
+        for (int k = 0; k < TS_B; ++k) {  // X = -D1 T
+            const double a = sb[i][k];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) xr[c] = fma(-a, sa[k][16 * cg + c], xr[c]);
+        }
+        // D1 into the lower right quarter
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int ii = idx & 63, kk = idx >> 6;
+            Dout[(TS_B + ii) + (TS_B + kk) * TS_B2] = D1[ii + kk * TS_B];
+        }
+    } else {
+        for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+            const int ii = idx & 63, kk = idx >> 6;
+            Dout[(TS_B + ii) + (TS_B + kk) * TS_B2] = 0.0;
+        }
+    }
+    for (int idx = t; idx < TS_B * TS_B; idx += 256) {
+        const int ii = idx & 63, kk = idx >> 6;
+        Dout[ii + kk * TS_B2] = D0[ii + kk * TS_B];           // upper left: D0
+        Dout[ii + (TS_B + kk) * TS_B2] = 0.0;                 // upper right: 0
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) Dout[(TS_B + i) + (16 * cg + c) * TS_B2] = xr[c];  // lower left
+}
+
+template <typename T, bool FWD>
+__global__ __launch_bounds__(512) void trsv128_kernel(const T* __restrict__ L, int64_t ldl,
+                                                      int64_t n, const double* __restrict__ rhs,
+                                                      double* out, int32_t* __restrict__ sync,
+                                                      const double* __restrict__ dinv128) {
+    __shared__ double red[2][TS_B][TS_B + 1];
+    __shared__ double sres[TS_B2];
+    __shared__ double red2[TS_B2][4];
+    __shared__ int s_block, s_fail;
+    const int t = threadIdx.x;
+    const int h = t >> 8, tt = t & 255;
+    const int lane = tt & 63, grp = tt >> 6;
+    const int grp_s = __builtin_amdgcn_readfirstlane(grp);
+    const int64_t nblocks = (n + TS_B - 1) / TS_B;     // 64-row blocks
+    const int64_t nb2 = (n + TS_B2 - 1) / TS_B2;        // 128-row blocks
+    int32_t* err = &sync[1];
+    if (t == 0) {
+        const int ticket = atomicAdd(&sync[0], 1);
+        s_block = FWD ? ticket : (int)(nb2 - 1 - ticket);
+        s_fail = 0;
+    }
+    __syncthreads();
+    const int64_t B = s_block;
+    const int64_t r0 = B * TS_B2;
+    const int w = (int)(n - r0 < TS_B2 ? n - r0 : TS_B2);
+    const int64_t rh = r0 + TS_B * h;                     // this half's 64 rows / columns
+    const int wh = w - TS_B * h < 0 ? 0 : (w - TS_B * h > TS_B ? TS_B : w - TS_B * h);
+
+    // The diagonal product's operands do not depend on other blocks: load them first.
+    // Lane (i, part) forms x_i's partial over columns 32 part .. 32 part + 31:
+    // FWD x = D s reads D(i, c); BWD x = D^T s reads D(c, i), a contiguous run.
+    const int di = t & (TS_B2 - 1), dpart = t >> 7;
+    const double* D = dinv128 + B * (TS_B2 * TS_B2);
+    double dv[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int cc = 32 * dpart + c;
+        dv[c] = FWD ? D[di + cc * TS_B2] : D[cc + di * TS_B2];
+    }
+
+    double acc1 = 0.0;
+    double acc16[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc16[c] = 0.0;
+    bool ok = true;
+    // dependencies: the 64-blocks before this block (FWD) or after it (BWD), in solve order
+    const int64_t nj = FWD ? 2 * B : nblocks - 2 * B - 2 > 0 ? nblocks - 2 * B - 2 : 0;
+    TsStage<T> s0, s1, s2, s3;
+    ts_issue<T, FWD>(s0, L, ldl, n, rh, wh, nblocks, 0, nj, out, lane, grp);
+    ts_issue<T, FWD>(s1, L, ldl, n, rh, wh, nblocks, 1, nj, out, lane, grp);
+    ts_issue<T, FWD>(s2, L, ldl, n, rh, wh, nblocks, 2, nj, out, lane, grp);
+    ts_issue<T, FWD>(s3, L, ldl, n, rh, wh, nblocks, 3, nj, out, lane, grp);
+    for (int64_t q = 0; q < nj; q += TS_DEPTH) {
+        ts_consume<T, FWD>(s0, n, nblocks, q, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s0, L, ldl, n, rh, wh, nblocks, q + 4, nj, out, lane, grp);
+        ts_consume<T, FWD>(s1, n, nblocks, q + 1, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s1, L, ldl, n, rh, wh, nblocks, q + 5, nj, out, lane, grp);
+        ts_consume<T, FWD>(s2, n, nblocks, q + 2, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s2, L, ldl, n, rh, wh, nblocks, q + 6, nj, out, lane, grp);
+        ts_consume<T, FWD>(s3, n, nblocks, q + 3, nj, out, lane, grp_s, err, ok, acc1, acc16);
+        ts_issue<T, FWD>(s3, L, ldl, n, rh, wh, nblocks, q + 7, nj, out, lane, grp);
+    }
+    if (!ok) s_fail = 1;  // benign race: every writer stores 1
+
+    if (FWD) {
+        red[h][lane][grp] = acc1;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) red[h][lane][16 * grp + c] = acc16[c];
+    }
+    __syncthreads();
+    if (tt < TS_B) {  // residual of this half's 64 rows
+        double sum = 0.0;
+        if (FWD) {
+            sum = ((red[h][tt][0] + red[h][tt][1]) + red[h][tt][2]) + red[h][tt][3];
+        } else {
+            for (int i = 0; i < TS_B; ++i) sum += red[h][i][tt];
+        }
+        sres[TS_B * h + tt] = (tt < wh ? rhs[rh + tt] : 0.0) - sum;
+    }
+    __syncthreads();
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) p = fma(dv[c], sres[32 * dpart + c], p);
+    red2[di][dpart] = p;
+    __syncthreads();
+    if (t < w) {
+        const double xi = ((red2[t][0] + red2[t][1]) + red2[t][2]) + red2[t][3];
+        __hip_atomic_store(&out[r0 + t], s_fail ? __builtin_nan("") : xi, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+size_t trsv_dinv128_doubles(int64_t n) {
+    return (size_t)((n + TS_B2 - 1) / TS_B2) * TS_B2 * TS_B2;
+}
+
+template <typename T>
+int trsv_inverse128(const T* L, int64_t n, int64_t ldl, double* dinv64, double* dinv128,
+                    hipStream_t s) {
+    if (n <= 0) return QMLH_OK;
+    QMLH_TRY(trsv_diag_inverse<T>(L, n, ldl, dinv64, s));
+    hipLaunchKernelGGL(trsv_inverse128_kernel<T>, dim3((unsigned)((n + TS_B2 - 1) / TS_B2)),
+                       dim3(256), 0, s, L, ldl, n, dinv64, dinv128);
+    QMLH_CHECK_LAUNCH("trsv 128 inverse launch");
+    return QMLH_OK;
+}
+
+template <typename T>
+int trsv_syncfree128(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
+                     int32_t* sync, bool forward, hipStream_t s, const double* dinv128) {
+    if (n <= 0) return QMLH_OK;
+    const int64_t nb2 = (n + TS_B2 - 1) / TS_B2;
+    hipLaunchKernelGGL(trsv_prepare_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, sync,
+                       (int64_t)2, out, n);
+    if (forward)
+        hipLaunchKernelGGL((trsv128_kernel<T, true>), dim3((unsigned)nb2), dim3(512), 0, s, L, ldl,
+                           n, rhs, out, sync, dinv128);
+    else
+        hipLaunchKernelGGL((trsv128_kernel<T, false>), dim3((unsigned)nb2), dim3(512), 0, s, L,
+                           ldl, n, rhs, out, sync, dinv128);
+    QMLH_CHECK_LAUNCH("trsv 128 launch");
+    return QMLH_OK;
+}
+
+template int trsv_inverse128<float>(const float*, int64_t, int64_t, double*, double*, hipStream_t);
+template int trsv_syncfree128<float>(const float*, int64_t, int64_t, const double*, double*,
+                                     int32_t*, bool, hipStream_t, const double*);
+
 }  // namespace qmlh

@@ -163,7 +163,7 @@ def test_pcg_solve_random_spd(n):
     assert np.linalg.norm(x.cpu().numpy() - xs) <= 1e-11 * np.linalg.norm(xs)
 
 
-@pytest.mark.parametrize("n,r", [(700, 9), (1333, 150)])
+@pytest.mark.parametrize("n,r", [(700, 9), (1333, 150), (4200, 30), (4500, 20)])
 def test_woodbury_preconditioner_is_exact_for_exact_factor(n, r):
     # With an exact factor of B, M = B + V V^T equals A, so PCG converges in one iteration.
     from qml_amd import _lib
