@@ -52,13 +52,23 @@ __device__ __forceinline__ void load_tile16(T (&v)[16], const T* __restrict__ L,
         }
         return;
     }
+    // Partial tiles (the last row or column block): unconditional loads from indices clamped
+    // into the tile, with no mask. Entries past the matrix then repeat its last row or column
+    // (finite values of a computed factor) and only meet zeros: FWD rows past w and BWD
+    // columns past w feed residuals that are never published and that the diagonal step
+    // multiplies by zero; FWD columns past wj and BWD rows past wj meet published values that
+    // the loader set to zero. A mask (or select) would be compiled into a conditional load
+    // that waits in its own branch.
+    if (w <= 0 || wj <= 0) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = T(0);
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int cc = 16 * grp + c;
-        if (FWD)  // L(r0 + lane, c0 + cc)
-            v[c] = (lane < w && cc < wj) ? L[(r0 + lane) + (c0 + cc) * ldl] : T(0);
-        else      // L(c0 + lane, r0 + cc): rows of block j, columns of block b
-            v[c] = (lane < wj && cc < w) ? L[(c0 + lane) + (r0 + cc) * ldl] : T(0);
+        v[c] = FWD ? L[(r0 + cp_clamp(lane, w)) + (c0 + cp_clamp(cc, wj)) * ldl]
+                   : L[(c0 + cp_clamp(lane, wj)) + (r0 + cp_clamp(cc, w)) * ldl];
     }
 }
 
@@ -87,22 +97,36 @@ __device__ __forceinline__ void ts_issue(TsStage<T>& st, const T* __restrict__ L
                      : 0.0;
 }
 
-// Wait (bounded) until no lane of the wave holds a pending value, re-reading the pending ones.
+// Wait (bounded) until no lane of the wave holds a pending value. Two polls are kept in
+// flight, checked oldest first and re-issued at once, so a value that lands is seen about half
+// a memory round trip after it lands instead of a full one. (Published values never change
+// again, so re-reading a settled lane is harmless; lanes that are not live keep their value,
+// which the loader set to zero.)
+__device__ __forceinline__ bool ts_pending(double v, bool live) {
+    return live && __builtin_bit_cast(unsigned long long, v) == TS_PENDING;
+}
+
 __device__ __forceinline__ void ts_settle(double& v, const double* p, bool live, int32_t* err,
                                           bool& ok) {
-    unsigned spins = 0;
-    while (true) {
-        const bool pending = live && __builtin_bit_cast(unsigned long long, v) == TS_PENDING;
-        if (!__any(pending)) return;
-        if (++spins > TS_SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = false;
-            v = 0.0;
+    if (!__any(ts_pending(v, live))) return;
+    double a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_sleep(2);
+    double b = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned spins = 0; spins < TS_SPIN_LIMIT; spins += 2) {
+        if (!__any(ts_pending(a, live))) {
+            if (live) v = a;
             return;
         }
-        __builtin_amdgcn_s_sleep(1);
-        if (pending) v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!__any(ts_pending(b, live))) {
+            if (live) v = b;
+            return;
+        }
+        b = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ok = false;
+    v = 0.0;
 }
 
 // Consume dependency q: settle its published values, then accumulate its contributions.
@@ -147,6 +171,7 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     __shared__ double sL[TS_B][TS_B + 1];
     __shared__ double sInv[TS_B];
     __shared__ double red[TS_B][TS_B + 1];
+    __shared__ double redp[4][TS_B];
     __shared__ double sv[TS_B];
     __shared__ int s_fail;
     const int t = threadIdx.x;
@@ -178,6 +203,7 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
 
     // Forward: thread (row lane, columns 16 grp ..) keeps one partial sum.
     // Backward: thread (row lane of block j, columns 16 grp .. of block b) keeps 16.
+    const double rhs_v = rhs[r0 + cp_clamp(lane, w)];  // used after the dependencies
     double acc1 = 0.0;
     double acc16[16];
 #pragma unroll
@@ -209,14 +235,23 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
         for (int c = 0; c < 16; ++c) red[lane][16 * grp + c] = acc16[c];
     }
     __syncthreads();
+    if (!FWD) {
+        // column sums over the 64 rows: four 16-row partials per column (all 256 lanes, the
+        // 16 LDS reads of a lane in flight together), then added in a fixed order
+        double ps = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ps += red[16 * grp + i][lane];
+        redp[grp][lane] = ps;
+        __syncthreads();
+    }
     if (t < TS_B) {
         double s = 0.0;
         if (FWD) {
             s = ((red[t][0] + red[t][1]) + red[t][2]) + red[t][3];
         } else {
-            for (int i = 0; i < TS_B; ++i) s += red[i][t];
+            s = ((redp[0][t] + redp[1][t]) + redp[2][t]) + redp[3][t];
         }
-        sv[t] = (t < w ? rhs[r0 + t] : 0.0) - s;
+        sv[t] = (t < w ? rhs_v : 0.0) - s;
         if (t < w && !dinv) sInv[t] = 1.0 / sL[t][t];
     }
     __syncthreads();
@@ -423,6 +458,7 @@ __global__ __launch_bounds__(512) void trsv128_kernel(const T* __restrict__ L, i
                                                       double* out, int32_t* __restrict__ sync,
                                                       const double* __restrict__ dinv128) {
     __shared__ double red[2][TS_B][TS_B + 1];
+    __shared__ double redp[2][4][TS_B];
     __shared__ double sres[TS_B2];
     __shared__ double red2[TS_B2][4];
     __shared__ int s_block, s_fail;
@@ -457,6 +493,9 @@ __global__ __launch_bounds__(512) void trsv128_kernel(const T* __restrict__ L, i
         dv[c] = FWD ? D[di + cc * TS_B2] : D[cc + di * TS_B2];
     }
 
+    // this lane's right-hand side value (used after the dependencies): loaded now, off the
+    // critical path
+    const double rhs_v = wh > 0 ? rhs[rh + cp_clamp(lane, wh)] : 0.0;
     double acc1 = 0.0;
     double acc16[16];
 #pragma unroll
@@ -488,14 +527,21 @@ __global__ __launch_bounds__(512) void trsv128_kernel(const T* __restrict__ L, i
         for (int c = 0; c < 16; ++c) red[h][lane][16 * grp + c] = acc16[c];
     }
     __syncthreads();
+    if (!FWD) {  // column sums: four 16-row partials per column, added in a fixed order
+        double ps = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ps += red[h][16 * grp + i][lane];
+        redp[h][grp][lane] = ps;
+        __syncthreads();
+    }
     if (tt < TS_B) {  // residual of this half's 64 rows
         double sum = 0.0;
         if (FWD) {
             sum = ((red[h][tt][0] + red[h][tt][1]) + red[h][tt][2]) + red[h][tt][3];
         } else {
-            for (int i = 0; i < TS_B; ++i) sum += red[h][i][tt];
+            sum = ((redp[h][0][tt] + redp[h][1][tt]) + redp[h][2][tt]) + redp[h][3][tt];
         }
-        sres[TS_B * h + tt] = (tt < wh ? rhs[rh + tt] : 0.0) - sum;
+        sres[TS_B * h + tt] = (tt < wh ? rhs_v : 0.0) - sum;
     }
     __syncthreads();
     double p = 0.0;
