@@ -150,12 +150,14 @@ int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stre
  * the iterations and the final relative residual. Synchronises the stream once per iteration.
  * ---------------------------------------------------------------------------------------- */
 int qmlh_spotrf(float *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
-/* qmlh_spotrf with the O(n^3) trailing updates in split BF16: P P^T ~ Ph Ph^T + Ph Pl^T +
- * Pl Ph^T (Ph = bf16(P), Pl = bf16(P - Ph)) on BF16 MFMA with FP32 accumulation, about
- * 2^-16 relative per update; for factors that only precondition (qmlh_pcg_solve).
- * ws: qmlh_spotrf_bf16x3_workspace(n) bytes, 16-byte aligned. */
-size_t qmlh_spotrf_bf16x3_workspace(int64_t n);
-int qmlh_spotrf_bf16x3(float *A, int64_t n, int64_t lda, int32_t *info, void *ws,
+/* qmlh_spotrf with the O(n^3) trailing updates in split FP16: with the panel scaled by a power
+ * of two 2^e set from the largest diagonal entry (so it lies below 2^14), P P^T ~ 2^-2e (Ph Ph^T
+ * + Ph Pl^T + Pl Ph^T), Ph = fp16(2^e P), Pl = fp16(2^e P - Ph), on FP16 MFMA with FP32
+ * accumulation, about 2^-22 relative per update (the dropped Pl Pl^T term); for factors that
+ * only precondition (qmlh_pcg_solve).
+ * ws: qmlh_spotrf_f16x3_workspace(n) bytes, 16-byte aligned. */
+size_t qmlh_spotrf_f16x3_workspace(int64_t n);
+int qmlh_spotrf_f16x3(float *A, int64_t n, int64_t lda, int32_t *info, void *ws,
                        size_t ws_bytes, qmlh_stream_t stream);
 int qmlh_gaussian_deflated(const double *X, int64_t n, int64_t ldx, const double *norms,
                            int64_t d, double sigma, double lambda, double shift, double *K,
@@ -165,7 +167,7 @@ size_t qmlh_woodbury_workspace(int64_t n);
 int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_t ldv,
                         int64_t r, double *C, int64_t ldc, int32_t *info, void *ws,
                         size_t ws_bytes, qmlh_stream_t stream);
-/* qmlh_spotrf (bf16x3 = 0) or qmlh_spotrf_bf16x3 (bf16x3 != 0; ws as there, else ws may be
+/* qmlh_spotrf (f16x3 = 0) or qmlh_spotrf_f16x3 (f16x3 != 0; ws as there, else ws may be
  * NULL) of A that also overwrites V (n x r, column-major, ldv >= n) with L^-1 V in the same
  * pass: V^T is placed in the r rows below the matrix (lda >= n + r; A[n:n+r, :] is
  * overwritten), solved as extra panel rows while the trailing updates run (their own updates in
@@ -173,7 +175,7 @@ int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_
  * failed factorisation (info != 0) V is unspecified. Replaces qmlh_spotrf* + the solve of
  * qmlh_woodbury_setup. */
 int qmlh_spotrf_lowrank(float *A, int64_t n, int64_t lda, float *V, int64_t ldv, int64_t r,
-                        int32_t bf16x3, int32_t *info, void *ws, size_t ws_bytes,
+                        int32_t f16x3, int32_t *info, void *ws, size_t ws_bytes,
                         qmlh_stream_t stream);
 /* C = chol(I + Y^T Y) (r x r, lower, FP64; r <= 1024) for Y (n x r, column-major FP32): the
  * rest of qmlh_woodbury_setup once Y = L^-1 V is known. The Gram matrix is summed in FP64 over

@@ -59,8 +59,8 @@ _SIGNATURES = {
     "qmlh_potrs_workspace": (_sz, [_i64]),
     "qmlh_potrs": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "qmlh_spotrf": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
-    "qmlh_spotrf_bf16x3_workspace": (_sz, [_i64]),
-    "qmlh_spotrf_bf16x3": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "qmlh_spotrf_f16x3_workspace": (_sz, [_i64]),
+    "qmlh_spotrf_f16x3": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "qmlh_gaussian_deflated": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _vp,
                                               _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "qmlh_woodbury_workspace": (_sz, [_i64]),

@@ -7,23 +7,26 @@
 namespace qmlh {
 
 // Lower Cholesky in place (column-major), storage T in {double, float}; LAPACK info rules.
-// For float, a non-null bf16_ws (qmlh_spotrf_bf16x3_workspace(n) bytes) selects split-BF16
+// For float, a non-null split_ws (qmlh_spotrf_f16x3_workspace(n) bytes) selects split-FP16
 // trailing updates instead of FP32 MFMA ones. r > 0: the r rows below the matrix
 // (A[n:n+r, 0:n], lda >= n + r) are solved along, ending as W L^-T.
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
-               void* bf16_ws = nullptr, int64_t r = 0);
+               void* split_ws = nullptr, int64_t r = 0);
 
-size_t syrk_bf16_workspace(int64_t n, int64_t kb);
-// Split P (m x kb, FP32 column-major) into the BF16 hi/lo panels in ws.
-int bf16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s);
-// C -= P P^T in split BF16 from a split of m_split rows, for the m rows starting at row off
+size_t syrk_f16_workspace(int64_t n, int64_t kb);
+// Record the scale of the split panels (from the largest diagonal entry of the matrix about
+// to be factored) in ws; before the first split.
+int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t s);
+// Split P (m x kb, FP32 column-major) into the scaled FP16 hi/lo panels in ws.
+int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s);
+// C -= P P^T in split FP16 from a split of m_split rows, for the m rows starting at row off
 // (a multiple of 128): ncols == 0 the lower triangle of the m x m block, ncols > 0 the entries
 // i >= j of its first ncols columns.
-int bf16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
+int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
                   int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s);
-// C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three BF16 products.
-int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
+// C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three FP16 products.
+int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s);
 
 // Synchronisation words the single-launch triangular solve needs for order n.

@@ -3,7 +3,7 @@
 // Replaces LAPACK dpotrf("U") / dpotrs("U", n, 1) as called at qml/fcho_solve.f90:35,46 (see
 // the header for why "U" on the Fortran view is the lower triangle here).
 //
-// Factorisation, outer block nb (256; 512 with split-BF16 trailing updates), per step k
+// Factorisation, outer block nb (256; 512 with split-FP16 trailing updates), per step k
 // (from order 4096 on with look-ahead: the next block column is factored on a second stream
 // while the rest of the trailing update runs, see potrf_lookahead):
 //   for each 64-wide sub-panel of the nb-wide block column (left to right):
@@ -27,9 +27,9 @@
 namespace qmlh {
 
 static constexpr int PF_NB = 256;
-// Block size when the trailing updates run as split-BF16 products: the update of C costs a
+// Block size when the trailing updates run as split-FP16 products: the update of C costs a
 // fixed read and write of the FP32 trailing matrix per step, so a deeper update pays off.
-static constexpr int PF_NB_BF16 = 512;
+static constexpr int PF_NB_SPLIT = 512;
 static constexpr int PF_IB = CP_B;
 static constexpr int UP_TILE = WG_T;
 
@@ -162,7 +162,7 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
 
 // Extra rows A[n:n+r, c0:n] -= A[n:n+r, k:k+kb] A[c0:n, k:k+kb]^T: the trailing update of the
 // r rows solved along with the factorisation (FP32/FP64 MFMA in the storage type, also when
-// the matrix itself takes split-BF16 updates).
+// the matrix itself takes split-FP16 updates).
 template <typename T>
 static int extra_rows_update(T* A, int64_t lda, int64_t n, int64_t r, int64_t k, int64_t kb,
                              int64_t c0, int32_t* info, hipStream_t s) {
@@ -174,11 +174,11 @@ static int extra_rows_update(T* A, int64_t lda, int64_t n, int64_t r, int64_t k,
 // Lower triangle of C (m x m) -= P P^T, P (m x kb) the panel below the block column.
 template <typename T>
 static int trailing_syrk(T* P, int64_t lda, int64_t m, int64_t kb, T* C, int32_t* info,
-                         hipStream_t s, void* bf16_ws) {
+                         hipStream_t s, void* split_ws) {
     if (m <= 0) return QMLH_OK;
-    if (std::is_same<T, float>::value && bf16_ws)
-        return syrk_bf16x3(reinterpret_cast<const float*>(P), lda, m, kb,
-                           reinterpret_cast<float*>(C), lda, info, bf16_ws, s);
+    if (std::is_same<T, float>::value && split_ws)
+        return syrk_f16x3(reinterpret_cast<const float*>(P), lda, m, kb,
+                           reinterpret_cast<float*>(C), lda, info, split_ws, s);
     return launch_update<T>(UPD_SYRK, P, lda, P, lda, m, m, kb, C, lda, info, s);
 }
 
@@ -194,14 +194,14 @@ static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 // for the last panel, so on return all work is ordered on `s`.
 //
 // With r extra rows (A[n:n+r, :]), the next block column's update and panel include them (with
-// split-BF16 updates, their part of the next block column is an FP32 product on the side
+// split-FP16 updates, their part of the next block column is an FP32 product on the side
 // stream). The rest of their trailing update touches nothing the trailing update or the panel
 // reads or writes: while the trailing matrix is large (the caller's stream is the critical
 // path) it runs on the side stream after the panel, later (the panels are the critical path)
 // on the caller's stream after the trailing update.
 template <typename T>
 static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
-                           hipStream_t s, void* bf16_ws, int64_t nb) {
+                           hipStream_t s, void* split_ws, int64_t nb) {
     hipStream_t s2 = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr;
     int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
@@ -216,15 +216,15 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         const int64_t kb2 = m < nb ? m : nb;  // next block column
         T* P = A + (k + kb) + k * lda;
         T* C = P + kb * lda;  // A(k + kb, k + kb)
-        const bool split = std::is_same<T, float>::value && bf16_ws && m > kb2;
+        const bool split = std::is_same<T, float>::value && split_ws && m > kb2;
         if (split) {
-            // split-BF16 trailing updates: split the panel once; the next block column gets
-            // the same split-BF16 update as the rest (its extra rows FP32), then the rest of
+            // split-FP16 trailing updates: split the panel once; the next block column gets
+            // the same split-FP16 update as the rest (its extra rows FP32), then the rest of
             // the trailing matrix reuses the split from row kb2 on
             float* Pf = reinterpret_cast<float*>(P);
             float* Cf = reinterpret_cast<float*>(C);
             // the extra rows' part (a small FP32 product) on the side stream, beside the split
-            // and the BF16 update: it needs the finished panel and the extra rows' earlier
+            // and the FP16 update: it needs the finished panel and the extra rows' earlier
             // updates (both ordered before ev_x on the caller's stream), and the side stream
             // runs it before the next panel
             if (r > 0) {
@@ -234,9 +234,9 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
                     rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
                                           info, s2);
             }
-            if (rc == QMLH_OK) rc = bf16x3_split(Pf, lda, m, kb, bf16_ws, s);
+            if (rc == QMLH_OK) rc = f16x3_split(Pf, lda, m, kb, split_ws, s);
             if (rc == QMLH_OK)
-                rc = bf16x3_update(bf16_ws, m, kb, 0, m, kb2, Cf, lda, info, s);
+                rc = f16x3_update(split_ws, m, kb, 0, m, kb2, Cf, lda, info, s);
         } else {
             rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
         }
@@ -249,11 +249,11 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
         if (rc == QMLH_OK && m > kb2) {
             if (split)
-                rc = bf16x3_update(bf16_ws, m, kb, kb2, m - kb2, 0,
+                rc = f16x3_update(split_ws, m, kb, kb2, m - kb2, 0,
                                    reinterpret_cast<float*>(C + kb2 + kb2 * lda), lda, info, s);
             else
                 rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s,
-                                      bf16_ws);
+                                      split_ws);
         }
         if (rc == QMLH_OK && !ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
@@ -268,7 +268,7 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
 }
 
 template <typename T>
-int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* bf16_ws,
+int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* split_ws,
                int64_t r) {
     if (n < 0 || r < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0 or r < 0");
     if (lda < (n + r > 1 ? n + r : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n + r");
@@ -278,16 +278,17 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     if (n == 0) return QMLH_OK;
     if (!A) return set_error(QMLH_ERR_ARG, "potrf: A is null");
     TimerScope ts(QMLH_TIMER_POTRF, (double)n * (double)n * (double)n / 3.0, s);
-    const bool bf16 = std::is_same<T, float>::value && bf16_ws;
-    const int64_t nb = bf16 ? PF_NB_BF16 : PF_NB;
+    const bool split16 = std::is_same<T, float>::value && split_ws;
+    const int64_t nb = split16 ? PF_NB_SPLIT : PF_NB;
+    if (split16) QMLH_TRY(f16x3_prepare(reinterpret_cast<const float*>(A), n, lda, split_ws, s));
     if (n >= PF_LOOKAHEAD_MIN)
-        return potrf_lookahead<T>(A, n, lda, r, info, s, bf16 ? bf16_ws : nullptr, nb);
+        return potrf_lookahead<T>(A, n, lda, r, info, s, split16 ? split_ws : nullptr, nb);
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
         QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s));
         const int64_t m = n - k - kb;
         T* P = A + (k + kb) + k * lda;
-        QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, bf16 ? bf16_ws : nullptr));
+        QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, split16 ? split_ws : nullptr));
         QMLH_TRY(extra_rows_update<T>(A, lda, n, r, k, kb, k + kb, info, s));
     }
     return QMLH_OK;
@@ -591,37 +592,37 @@ extern "C" int qmlh_spotrf(float* A, int64_t n, int64_t lda, int32_t* info,
     return qmlh::potrf_impl<float>(A, n, lda, info, (hipStream_t)stream);
 }
 
-extern "C" size_t qmlh_spotrf_bf16x3_workspace(int64_t n) {
-    return n > 0 ? qmlh::syrk_bf16_workspace(n, qmlh::PF_NB_BF16) : 0;
+extern "C" size_t qmlh_spotrf_f16x3_workspace(int64_t n) {
+    return n > 0 ? qmlh::syrk_f16_workspace(n, qmlh::PF_NB_SPLIT) : 0;
 }
 
-extern "C" int qmlh_spotrf_bf16x3(float* A, int64_t n, int64_t lda, int32_t* info, void* ws,
+extern "C" int qmlh_spotrf_f16x3(float* A, int64_t n, int64_t lda, int32_t* info, void* ws,
                                   size_t ws_bytes, qmlh_stream_t stream) {
     using namespace qmlh;
-    if (n > 0 && (!ws || ws_bytes < qmlh_spotrf_bf16x3_workspace(n)))
-        return set_error(QMLH_ERR_WORKSPACE, "spotrf_bf16x3: workspace too small");
+    if (n > 0 && (!ws || ws_bytes < qmlh_spotrf_f16x3_workspace(n)))
+        return set_error(QMLH_ERR_WORKSPACE, "spotrf_f16x3: workspace too small");
     if (reinterpret_cast<uintptr_t>(ws) & 15)
-        return set_error(QMLH_ERR_ARG, "spotrf_bf16x3: workspace must be 16-byte aligned");
+        return set_error(QMLH_ERR_ARG, "spotrf_f16x3: workspace must be 16-byte aligned");
     return potrf_impl<float>(A, n, lda, info, (hipStream_t)stream, ws);
 }
 
 extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
-                                   int64_t r, int32_t bf16x3, int32_t* info, void* ws,
+                                   int64_t r, int32_t f16x3, int32_t* info, void* ws,
                                    size_t ws_bytes, qmlh_stream_t stream) {
     using namespace qmlh;
     if (n < 0 || r < 0 || lda < n + r || (r > 0 && ldv < n))
         return set_error(QMLH_ERR_ARG, "spotrf_lowrank: bad shape (lda >= n + r, ldv >= n)");
     if (n > 0 && (!A || !info || (r > 0 && !V)))
         return set_error(QMLH_ERR_ARG, "spotrf_lowrank: null pointer");
-    if (bf16x3 && n > 0) {
-        if (!ws || ws_bytes < qmlh_spotrf_bf16x3_workspace(n))
+    if (f16x3 && n > 0) {
+        if (!ws || ws_bytes < qmlh_spotrf_f16x3_workspace(n))
             return set_error(QMLH_ERR_WORKSPACE, "spotrf_lowrank: workspace too small");
         if (reinterpret_cast<uintptr_t>(ws) & 15)
             return set_error(QMLH_ERR_ARG, "spotrf_lowrank: workspace must be 16-byte aligned");
     }
     hipStream_t s = (hipStream_t)stream;
     if (n > 0) QMLH_TRY(transpose_f32(V, ldv, n, r, A + n, lda, s));
-    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, bf16x3 ? ws : nullptr, r));
+    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, f16x3 ? ws : nullptr, r));
     if (n > 0) QMLH_TRY(transpose_f32(A + n, lda, r, n, V, ldv, s));
     return QMLH_OK;
 }

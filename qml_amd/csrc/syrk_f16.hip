@@ -1,10 +1,16 @@
-// Split-BF16 trailing update for the mixed-precision factor: C -= P P^T (lower tiles) with
-// P = Ph + Pl, Ph = bf16(P), Pl = bf16(P - Ph), evaluated as three BF16 products accumulated
-// in FP32 on v_mfma_f32_16x16x32_bf16:
-//     P P^T ~ Ph Ph^T + Ph Pl^T + Pl Ph^T
-// (each BF16 x BF16 product is exact in FP32; the dropped Pl Pl^T term is ~2^-16 relative).
-// This is the dense O(n^3) part of the mixed factorisation; the panel, the diagonal blocks and
-// the substitutions stay in FP32 storage / FP64 arithmetic.
+// Split-FP16 trailing update for the mixed-precision factor: C -= P P^T (lower tiles). The
+// panel is scaled by a power of two, 2^e (chosen from the largest diagonal entry of the matrix
+// so that every entry lies below 2^14, see sb_scale_exp), and split as Ph = fp16(2^e P),
+// Pl = fp16(2^e P - Ph); then
+//     2^2e P P^T ~ Ph Ph^T + Ph Pl^T + Pl Ph^T
+// on v_mfma_f32_16x16x32_f16 with FP32 accumulation (each FP16 x FP16 product is exact in
+// FP32; Ph + Pl carries 22 bits of each entry, and the dropped Pl Pl^T term is ~2^-22
+// relative). C is scaled by 2^2e on load and by 2^-2e on store (exact). FP16 has the BF16
+// MFMA rate and three more mantissa bits per half than BF16, so the split reaches almost
+// FP32 accuracy (a 2^-16 split-BF16 version needed 11-13 refinement iterations where this
+// needs 8); its narrower exponent range is what the scale is for. This is the dense O(n^3)
+// part of the mixed factorisation; the panel, the diagonal blocks and the substitutions stay
+// in FP32 storage / FP64 arithmetic.
 //
 // Panel layout. The split kernel writes Ph and Pl in MFMA fragment order: element (row, k)
 // of the (m_pad x kb) panel at ((c * 4 + q) * m_pad + row) * 8 + e with k = 32 c + 8 q + e,
@@ -25,18 +31,50 @@
 
 namespace qmlh {
 
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 sb_elem;
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 // 16-byte staging registers. A native vector type: arrays of the HIP uint4 struct are copied
 // with memcpy and end up in scratch memory instead of registers.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 static constexpr int SB_T = 128;          // tile rows
 static constexpr int SB_KC = 32;          // features per chunk
-static constexpr int SB_PANEL = SB_T * SB_KC;  // bf16 elements per panel per chunk (8 KB)
+static constexpr int SB_PANEL = SB_T * SB_KC;  // FP16 elements per panel per chunk (8 KB)
 
 __device__ __forceinline__ int64_t split_index(int64_t row, int64_t k, int64_t m_pad) {
     const int64_t c = k >> 5, q = (k >> 3) & 3, e = k & 7;
     return ((c * 4 + q) * m_pad + row) * 8 + e;
+}
+
+// Panel scale. Every entry of a Cholesky factor obeys |L(i, j)| <= sqrt(A(i, i)), so with
+// dmax the largest diagonal entry of the matrix being factored, the panels scaled by
+// 2^e = 2^(14 - (ilogb(sqrt(dmax)) + 1)) lie below 2^14 in magnitude: inside FP16's range with
+// room, and far above its subnormals for every entry that matters. hdr[0] holds dmax's bits.
+static constexpr size_t SB_HDR_BYTES = 256;
+
+__device__ __forceinline__ int sb_scale_exp(const unsigned* hdr) {
+    const float dmax = __uint_as_float(hdr[0]);
+    if (!(dmax > 0.0f) || !(dmax < 3.0e38f)) return 0;
+    const int e = 14 - (ilogbf(sqrtf(dmax)) + 1);
+    return e < -40 ? -40 : (e > 40 ? 40 : e);
+}
+
+__global__ void sb_hdr_clear_kernel(unsigned* hdr) { hdr[threadIdx.x] = 0u; }
+
+// hdr[0] = bits of max_i A(i, i) over the positive diagonal entries (positive floats order
+// like their bit patterns).
+__global__ __launch_bounds__(256) void sb_diag_max_kernel(const float* __restrict__ A,
+                                                          int64_t n, int64_t lda,
+                                                          unsigned* __restrict__ hdr) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float v = i < n ? A[i + i * lda] : 0.0f;
+    const unsigned u = (v > 0.0f && v < 3.0e38f) ? __float_as_uint(v) : 0u;
+    unsigned m = u;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(hdr, m);
 }
 
 // Ph, Pl <- split of P (column-major FP32, m x kb); rows m .. m_pad-1 and features
@@ -45,43 +83,45 @@ __device__ __forceinline__ int64_t split_index(int64_t row, int64_t k, int64_t m
 __global__ __launch_bounds__(256) void split_panel_kernel(const float* __restrict__ P,
                                                           int64_t ldp, int64_t m, int64_t m_pad,
                                                           int64_t kb, int64_t kb_pad,
-                                                          __bf16* __restrict__ Ph,
-                                                          __bf16* __restrict__ Pl) {
+                                                          sb_elem* __restrict__ Ph,
+                                                          sb_elem* __restrict__ Pl,
+                                                          const unsigned* __restrict__ hdr) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t k0 = (int64_t)blockIdx.y * 8;
     if (row >= m_pad || k0 >= kb_pad) return;
+    const float sc = ldexpf(1.0f, sb_scale_exp(hdr));
     // loads from clamped indices (no per-load branch), zero selected afterwards
     const int64_t rr = row < m ? row : m - 1;
     float raw[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) raw[e] = P[rr + (k0 + e < kb ? k0 + e : kb - 1) * ldp];
-    bf16x8_t h, l;
+    f16x8_t h, l;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const float v = (row < m && k0 + e < kb) ? raw[e] : 0.0f;
-        const __bf16 hv = (__bf16)v;
+        const float v = (row < m && k0 + e < kb) ? raw[e] * sc : 0.0f;
+        const sb_elem hv = (sb_elem)v;
         h[e] = hv;
-        l[e] = (__bf16)(v - (float)hv);
+        l[e] = (sb_elem)(v - (float)hv);
     }
     const int64_t idx = split_index(row, k0, m_pad);  // k0 % 8 == 0: 8 contiguous elements
-    *reinterpret_cast<bf16x8_t*>(Ph + idx) = h;
-    *reinterpret_cast<bf16x8_t*>(Pl + idx) = l;
+    *reinterpret_cast<f16x8_t*>(Ph + idx) = h;
+    *reinterpret_cast<f16x8_t*>(Pl + idx) = l;
 }
 
 // Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) into an LDS panel.
-__device__ __forceinline__ void sb_load(u32x4_t (&r)[2], const __bf16* __restrict__ src,
+__device__ __forceinline__ void sb_load(u32x4_t (&r)[2], const sb_elem* __restrict__ src,
                                         int64_t m_pad, int64_t row0, int64_t c) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int p = t + 256 * h;  // 0 .. 511
         const int q = p >> 7, row = p & 127;
-        const __bf16* g = src + ((c * 4 + q) * m_pad + row0 + row) * 8;
+        const sb_elem* g = src + ((c * 4 + q) * m_pad + row0 + row) * 8;
         r[h] = *reinterpret_cast<const u32x4_t*>(g);
     }
 }
 
-__device__ __forceinline__ void sb_store(__bf16* __restrict__ panel, const u32x4_t (&r)[2]) {
+__device__ __forceinline__ void sb_store(sb_elem* __restrict__ panel, const u32x4_t (&r)[2]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -90,14 +130,14 @@ __device__ __forceinline__ void sb_store(__bf16* __restrict__ panel, const u32x4
     }
 }
 
-__device__ __forceinline__ bf16x8_t sb_frag(const __bf16* __restrict__ panel, int row, int g) {
-    return *reinterpret_cast<const bf16x8_t*>(panel + (g * SB_T + row) * 8);
+__device__ __forceinline__ f16x8_t sb_frag(const sb_elem* __restrict__ panel, int row, int g) {
+    return *reinterpret_cast<const f16x8_t*>(panel + (g * SB_T + row) * 8);
 }
 
-// Flip the sign of 8 BF16 values (exact).
-__device__ __forceinline__ bf16x8_t sb_neg(bf16x8_t v) {
+// Flip the sign of 8 FP16 values (exact).
+__device__ __forceinline__ f16x8_t sb_neg(f16x8_t v) {
     const u32x4_t u = __builtin_bit_cast(u32x4_t, v) ^ 0x80008000u;
-    return __builtin_bit_cast(bf16x8_t, u);
+    return __builtin_bit_cast(f16x8_t, u);
 }
 
 // C(i, j) -= sum_k P(i, k) P(j, k) for i >= j. The accumulators start from C -- its loads are
@@ -118,15 +158,20 @@ __host__ __device__ inline int64_t sb_super_count(int64_t t) {
 // covers m rows from there. ncols == 0: the lower triangle of the m x m block (super-block
 // order); ncols > 0: the entries i >= j of its first ncols columns (the next block column of a
 // look-ahead factorisation), tiles in row order.
-__global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __restrict__ Ph,
-                                                             const __bf16* __restrict__ Pl,
+__global__ __launch_bounds__(256, 2) void syrk_f16x3_kernel(const sb_elem* __restrict__ Ph,
+                                                             const sb_elem* __restrict__ Pl,
                                                              int64_t m, int64_t m_pad,
                                                              int64_t kb_pad, int64_t ncols,
                                                              float* __restrict__ C, int64_t ldc,
-                                                             const int32_t* __restrict__ info) {
+                                                             const int32_t* __restrict__ info,
+                                                             const unsigned* __restrict__ hdr) {
     // buffers x {row hi, row lo, col hi, col lo}
-    __shared__ __attribute__((aligned(16))) __bf16 lds[2][4][SB_PANEL];
+    __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SB_PANEL];
     if (info && *info) return;
+    // the panel is scaled by 2^e, so the products carry 2^(2e): C is scaled up on load and
+    // back on store (exact powers of two)
+    const int e2 = 2 * sb_scale_exp(hdr);
+    const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
     const int64_t bid = blockIdx.x;
     const int64_t t_tiles = (m + SB_T - 1) / SB_T;
     int64_t bi, bj;
@@ -173,7 +218,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
             for (int b = 0; b < 4; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    acc[a][b][e] = C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc];
+                    acc[a][b][e] = C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] * up;
     } else {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -183,7 +228,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
                 for (int e = 0; e < 4; ++e) {
                     const int64_t i = row0 + 16 * b + r;
                     const int64_t j = col0 + 16 * a + 4 * g + e;  // accumulator rows 4 g + e
-                    acc[a][b][e] = (live && i < m && j <= i && j < ncol) ? C[i + j * ldc] : 0.0f;
+                    acc[a][b][e] = (live && i < m && j <= i && j < ncol) ? C[i + j * ldc] * up : 0.0f;
                 }
     }
 
@@ -207,7 +252,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
             sb_load(ch, Ph, m_pad, cols0, c + 1);
             sb_load(cl, Pl, m_pad, cols0, c + 1);
         }
-        bf16x8_t fh[4], fl[4], sh[4], sl[4];
+        f16x8_t fh[4], fl[4], sh[4], sl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             fh[q] = sb_frag(lds[st][2], fr + 16 * q + r, g);
@@ -219,9 +264,9 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[a], sh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], sl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], sh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fl[a], sh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fh[a], sl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fh[a], sh[b], acc[a][b], 0, 0, 0);
             }
         if (more) {
             sb_store(lds[st ^ 1][0], rh);
@@ -241,7 +286,7 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
             for (int b = 0; b < 4; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] = acc[a][b][e];
+                    C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] = acc[a][b][e] * down;
         return;
     }
 #pragma unroll
@@ -254,40 +299,55 @@ __global__ __launch_bounds__(256, 2) void syrk_bf16x3_kernel(const __bf16* __res
             for (int e = 0; e < 4; ++e) {
                 const int64_t j = col0 + 16 * a + 4 * g + e;
                 if (j > i || j >= ncol) continue;
-                C[i + j * ldc] = acc[a][b][e];
+                C[i + j * ldc] = acc[a][b][e] * down;
             }
         }
     }
 }
 
-size_t syrk_bf16_workspace(int64_t n, int64_t kb) {
+size_t syrk_f16_workspace(int64_t n, int64_t kb) {
     const int64_t m_pad = round_up(n > 0 ? n : 1, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
-    return (size_t)(2 * m_pad * kb_pad) * sizeof(__bf16);
+    return SB_HDR_BYTES + (size_t)(2 * m_pad * kb_pad) * sizeof(sb_elem);
 }
 
-int bf16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s) {
+int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t s) {
+    unsigned* hdr = static_cast<unsigned*>(ws);
+    hipLaunchKernelGGL(sb_hdr_clear_kernel, dim3(1), dim3(SB_HDR_BYTES / 4), 0, s, hdr);
+    if (n > 0)
+        hipLaunchKernelGGL(sb_diag_max_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                           A, n, lda, hdr);
+    QMLH_CHECK_LAUNCH("split scale launch");
+    return QMLH_OK;
+}
+
+static sb_elem* sb_panels(void* ws) {
+    return reinterpret_cast<sb_elem*>(static_cast<char*>(ws) + SB_HDR_BYTES);
+}
+
+int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
-    __bf16* Ph = static_cast<__bf16*>(ws);
-    __bf16* Pl = Ph + m_pad * kb_pad;
+    sb_elem* Ph = sb_panels(ws);
+    sb_elem* Pl = Ph + m_pad * kb_pad;
     hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)(kb_pad / 8)),
-                       dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl);
+                       dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl,
+                       static_cast<const unsigned*>(ws));
     QMLH_CHECK_LAUNCH("split_panel launch");
     return QMLH_OK;
 }
 
-int bf16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
+int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
                   int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m_split, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
     if (off % SB_T != 0 || off + round_up(m, SB_T) > m_pad)
-        return set_error(QMLH_ERR_ARG, "bf16x3_update: rows %lld + %lld outside the split",
+        return set_error(QMLH_ERR_ARG, "f16x3_update: rows %lld + %lld outside the split",
                          (long long)off, (long long)m);
-    const __bf16* Ph = static_cast<const __bf16*>(ws) + off * 8;
-    const __bf16* Pl = static_cast<const __bf16*>(ws) + m_pad * kb_pad + off * 8;
+    const sb_elem* Ph = sb_panels(const_cast<void*>(ws)) + off * 8;
+    const sb_elem* Pl = sb_panels(const_cast<void*>(ws)) + m_pad * kb_pad + off * 8;
     const int64_t tm = ceil_div(m, (int64_t)SB_T);
     int64_t grid;
     double flops;
@@ -300,19 +360,19 @@ int bf16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int6
         flops = (double)m * (double)(m + 1) * (double)kb;
     }
     TimerScope ts(ncols > 0 ? QMLH_TIMER_GEMM : QMLH_TIMER_SYRK, flops, s);
-    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Ph, Pl, m,
-                       m_pad, kb_pad, ncols, C, ldc, info);
-    QMLH_CHECK_LAUNCH("syrk_bf16x3 launch");
+    hipLaunchKernelGGL(syrk_f16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Ph, Pl, m,
+                       m_pad, kb_pad, ncols, C, ldc, info, static_cast<const unsigned*>(ws));
+    QMLH_CHECK_LAUNCH("syrk_f16x3 launch");
     return QMLH_OK;
 }
 
 // C (m x m, lower) -= P P^T with P (m x kb) column-major FP32; ws holds
-// syrk_bf16_workspace(m, kb) bytes.
-int syrk_bf16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
+// syrk_f16_workspace(m, kb) bytes, prepared by f16x3_prepare for the matrix being factored.
+int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
-    QMLH_TRY(bf16x3_split(P, ldp, m, kb, ws, s));
-    return bf16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s);
+    QMLH_TRY(f16x3_split(P, ldp, m, kb, ws, s));
+    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s);
 }
 
 }  // namespace qmlh

@@ -66,12 +66,12 @@ class KRR:
         self.precision = precision
         self.tol = float(tol)
         self.max_iter = int(max_iter)
-        # Mixed-precision factor: "bf16x3" (split-BF16 trailing updates) needs the deflated
+        # Mixed-precision factor: "f16x3" (split-FP16 trailing updates) needs the deflated
         # Gaussian remainder to stay accurate relative to lambda; "fp32" everywhere else.
         if factor is None:
-            factor = "bf16x3" if kernel == "gaussian" else "fp32"
-        if factor not in ("fp32", "bf16x3"):
-            raise ValueError("factor must be 'fp32' or 'bf16x3'")
+            factor = "f16x3" if kernel == "gaussian" else "fp32"
+        if factor not in ("fp32", "f16x3"):
+            raise ValueError("factor must be 'fp32' or 'f16x3'")
         self.factor = factor
         self.Xd = None
         self.norms = None
@@ -154,14 +154,14 @@ class KRR:
                     _lib.ptr(u), h))
             else:
                 _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), ldb, shift, h))
-            bf16 = self.factor == "bf16x3"
-            if bf16 and bws is None:
-                nb = L.qmlh_spotrf_bf16x3_workspace(n)
+            split16 = self.factor == "f16x3"
+            if split16 and bws is None:
+                nb = L.qmlh_spotrf_f16x3_workspace(n)
                 bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
             _lib.check(L.qmlh_spotrf_lowrank(_lib.ptr(B), n, ldb, _lib.ptr(V) if r else None, n,
-                                             r, int(bf16), _lib.ptr(info),
-                                             _lib.ptr(bws) if bf16 else None,
-                                             bws.numel() * 8 if bf16 else 0, h))
+                                             r, int(split16), _lib.ptr(info),
+                                             _lib.ptr(bws) if split16 else None,
+                                             bws.numel() * 8 if split16 else 0, h))
             rep.info = _lib.host_int(info, stream)
             rep.shift = shift
             if rep.info == 0:

@@ -50,8 +50,8 @@ def test_spotrf_fp32_factor_vs_lapack(n):
 
 
 @pytest.mark.parametrize("n", [257, 700, 1500, 5000])
-def test_spotrf_bf16x3_factor_accuracy(n):
-    # Split-BF16 trailing updates: each update ~2^-16 relative, so the factor agrees with
+def test_spotrf_f16x3_factor_accuracy(n):
+    # Split-FP16 trailing updates: each update ~2^-22 relative, so the factor agrees with
     # LAPACK's to ~1e-5 normwise on a well-conditioned matrix.
     from scipy.linalg import lapack
 
@@ -63,15 +63,15 @@ def test_spotrf_bf16x3_factor_accuracy(n):
     A = (M @ M.T / n + np.eye(n)).astype(np.float32)
     Ad = torch.from_numpy(A.copy()).cuda()
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    nb = L.qmlh_spotrf_bf16x3_workspace(n)
+    nb = L.qmlh_spotrf_f16x3_workspace(n)
     ws = torch.empty(nb // 8 + 1, dtype=torch.float64, device="cuda")
-    _lib.check(L.qmlh_spotrf_bf16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
+    _lib.check(L.qmlh_spotrf_f16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
                                     ws.numel() * 8, None))
     assert int(info.item()) == 0
     Lg = np.tril(Ad.cpu().numpy().T.astype(np.float64))
     Lref = np.tril(lapack.dpotrf(A.astype(np.float64), lower=1)[0])
     err = np.linalg.norm(Lg - Lref) / np.linalg.norm(Lref)
-    assert 1e-9 < err <= 5e-5, err  # lower bound: the BF16 path really ran
+    assert 1e-9 < err <= 5e-5, err  # lower bound: the split path really ran
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
@@ -205,8 +205,8 @@ def test_woodbury_preconditioner_is_exact_for_exact_factor(n, r):
         np.linalg.solve(A, y))
 
 
-@pytest.mark.parametrize("n,r,bf16", [(700, 9, 0), (1333, 150, 1), (4500, 437, 1), (4500, 1, 0)])
-def test_spotrf_lowrank_factor_and_solve(n, r, bf16):
+@pytest.mark.parametrize("n,r,split16", [(700, 9, 0), (1333, 150, 1), (4500, 437, 1), (4500, 1, 0)])
+def test_spotrf_lowrank_factor_and_solve(n, r, split16):
     # One pass factors A and solves Y = L^-1 V as extra panel rows: the factor is the plain
     # one bit for bit, Y matches a host triangular solve with that factor, and the capacitance
     # factor matches numpy's Cholesky of I + Y^T Y.
@@ -220,11 +220,11 @@ def test_spotrf_lowrank_factor_and_solve(n, r, bf16):
     A = (M @ M.T + 0.5 * np.eye(n)).astype(np.float32)
     Vm = (rng.standard_normal((n, r)) * 3).astype(np.float32)
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    ws = torch.empty(L.qmlh_spotrf_bf16x3_workspace(n) // 8 + 1, dtype=torch.float64,
+    ws = torch.empty(L.qmlh_spotrf_f16x3_workspace(n) // 8 + 1, dtype=torch.float64,
                      device="cuda")
     Ad = torch.from_numpy(A.copy()).cuda()
-    if bf16:
-        _lib.check(L.qmlh_spotrf_bf16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
+    if split16:
+        _lib.check(L.qmlh_spotrf_f16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
                                         ws.numel() * 8, None))
     else:
         _lib.check(L.qmlh_spotrf(Ad.data_ptr(), n, n, info.data_ptr(), None))
@@ -233,9 +233,9 @@ def test_spotrf_lowrank_factor_and_solve(n, r, bf16):
     Ab = torch.full((n, n + r), float("nan"), dtype=torch.float32, device="cuda")
     Ab[:, :n] = torch.from_numpy(A).cuda()
     Vd = torch.from_numpy(np.ascontiguousarray(Vm.T)).cuda()  # (n, r) column-major
-    _lib.check(L.qmlh_spotrf_lowrank(Ab.data_ptr(), n, n + r, Vd.data_ptr(), n, r, bf16,
-                                     info.data_ptr(), ws.data_ptr() if bf16 else None,
-                                     ws.numel() * 8 if bf16 else 0, None))
+    _lib.check(L.qmlh_spotrf_lowrank(Ab.data_ptr(), n, n + r, Vd.data_ptr(), n, r, split16,
+                                     info.data_ptr(), ws.data_ptr() if split16 else None,
+                                     ws.numel() * 8 if split16 else 0, None))
     assert int(info.item()) == 0
     Lg = np.tril(Ab[:, :n].cpu().numpy().T)
     assert np.array_equal(Lg, np.tril(Ad.cpu().numpy().T))
@@ -253,7 +253,7 @@ def test_spotrf_lowrank_factor_and_solve(n, r, bf16):
     assert np.linalg.norm(np.tril(Cd.cpu().numpy().T) - Cref) <= 1e-12 * np.linalg.norm(Cref)
 
 
-@pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "bf16x3"),
+@pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "f16x3"),
                                                        ("gaussian", 4000.0, 1e-6, 3000, "fp32"),
                                                        ("laplacian", 724.0, 1e-8, 2000, "fp32")])
 def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n, factor):
