@@ -171,10 +171,12 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_
 
 // Diagonal block of a sub-panel, one workgroup: factor the w x w block at A in wave 0, write
 // L back, set info = col_base + c + 1 on a failed pivot. `info` already set: nothing is done.
+// sp: CP_PACK doubles of LDS (potf2_body brings its own; a kernel that has LDS to spare at
+// that point passes it).
 template <typename T>
-__device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
-                                           int64_t col_base, int32_t* __restrict__ info) {
-    __shared__ double sp[CP_PACK];
+__device__ __forceinline__ void potf2_body_sp(T* __restrict__ A, int64_t lda, int w,
+                                              int64_t col_base, int32_t* __restrict__ info,
+                                              double* __restrict__ sp) {
     if (*info) return;
     const int t = threadIdx.x;
     stage_packed64(sp, A, lda, w);
@@ -187,6 +189,13 @@ __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w
         if (i < w && j < w && i >= j) A[i + (int64_t)j * lda] = (T)sp[pk(i, j)];
     }
     if (t == 0 && fail) *info = (int32_t)(col_base + fail);
+}
+
+template <typename T>
+__device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
+                                           int64_t col_base, int32_t* __restrict__ info) {
+    __shared__ double sp[CP_PACK];
+    potf2_body_sp<T>(A, lda, w, col_base, info, sp);
 }
 
 // Rows of a sub-panel, workgroup `local` of the launch: rows 256 local .. of the m rows below

@@ -150,14 +150,18 @@ __global__ __launch_bounds__(256, 2) void gaussian_deflated_kernel(
                 const double c = t.acc[a][b][e];
                 double kv, bv;
                 const double xv = c * inv_s2;
+                // B'' = u_i u_j (expm1(c/s^2) - c/s^2) = k_ij - u_i u_j (1 + c/s^2), since
+                // k_ij = u_i u_j exp(c/s^2): one exponential per entry. The subtraction loses
+                // about three of FP64's sixteen digits (B'' ~ 1e-3 k), far below the FP32 it
+                // is stored in.
                 if (i == j) {
                     kv = 1.0 + lambda;
-                    bv = uj * uj * (expm1(xv) - xv) + (lambda + shift);
+                    bv = (1.0 - uj * uj * (1.0 + xv)) + (lambda + shift);
                 } else {
                     double d2 = nrm[i] + nj - 2.0 * c;
                     d2 = d2 > 0.0 ? d2 : 0.0;
                     kv = exp(scale * d2);
-                    bv = u[i] * uj * (expm1(xv) - xv);
+                    bv = kv - u[i] * uj * (1.0 + xv);
                 }
                 if (K) K[j + i * ldk] = kv;
                 B32[j + i * ldb] = (float)bv;

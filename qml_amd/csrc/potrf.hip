@@ -10,7 +10,8 @@
 //     1. the 64 x 64 diagonal block is factored in the registers of one wavefront (FP64
 //        arithmetic), then every row of the column below is solved against it (256 rows per
 //        workgroup);
-//     2. an MFMA update of the block column's remaining columns (all rows below, i >= j);
+//     2. an MFMA update of the block column's remaining columns (all rows below, i >= j),
+//        whose workgroup for the top-left tile then factors the next diagonal block;
 //   3. the trailing update A[k+kb:n, k+kb:n] -= P P^T on the lower tiles only (MFMA).
 // The storage type T is double (FP64 MFMA updates) or float (FP32 MFMA updates, the
 // mixed-precision factor); diagonal blocks and substitutions always run in FP64.
@@ -35,8 +36,11 @@ static constexpr int UP_TILE = WG_T;
 
 // Update shapes: FULL -- every entry of the M x N block; SYRK -- the lower triangle of an
 // M x M block (lower tiles only); TRAP -- the entries i >= j of an M x N block whose (0, 0)
-// sits on the diagonal (tiles entirely above the diagonal do nothing).
-enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2 };
+// sits on the diagonal (tiles entirely above the diagonal do nothing); TRAP_POTF2 -- TRAP, and
+// the workgroup of tile (0, 0) then factors the w_next x w_next diagonal block at C (the next
+// sub-panel's, which only that workgroup writes), so the next sub-panel needs no launch of its
+// own for it.
+enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2, UPD_TRAP_POTF2 = 3 };
 
 // ------------------------------------------------------------------------------------------
 // C(i, j) -= sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL: B(j, k) at
@@ -47,8 +51,8 @@ enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2 };
 template <typename T, int MODE, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
-    int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, const int32_t* __restrict__ info,
-    int64_t tiles_m) {
+    int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, int32_t* __restrict__ info,
+    int64_t tiles_m, int w_next, int64_t col_base_next) {
     __shared__ __attribute__((aligned(16))) T lds[WG_LDS_ELEMS];
     constexpr bool MASK = MODE != UPD_FULL;
     if (info && *info) return;
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     } else {
         bi = bid % tiles_m;
         bj = bid / tiles_m;
-        if (MODE == UPD_TRAP && bj > bi) return;  // tile entirely above the diagonal
+        if (MODE >= UPD_TRAP && bj > bi) return;  // tile entirely above the diagonal
     }
     // First MFMA operand: B rows (output columns); second: A rows (output rows), so the 16
     // lanes of one accumulator element run down 16 consecutive rows of column-major C.
@@ -77,35 +81,49 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     WaveTile<T> t;
     wave_tile_load_c<T, MASK>(t, C, ldc, row0, col0, M, N, live, interior);
     wg_tile_mma_lds<T, LB, LAYOUT_COL, true>(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
-    if (!live) return;
-    wave_tile_store_c<T, MASK>(t, C, ldc, row0, col0, M, N, interior);
+    if (live) wave_tile_store_c<T, MASK>(t, C, ldc, row0, col0, M, N, interior);
+    if (MODE == UPD_TRAP_POTF2 && bid == 0 && w_next > 0) {
+        // every wave of this workgroup got here; the barrier makes the tile's stores visible
+        // to the whole workgroup and frees the operand LDS for the factor
+        __syncthreads();
+        potf2_body_sp<T>(C, ldc, w_next, col_base_next, info, reinterpret_cast<double*>(lds));
+    }
 }
 
 // C -= A B^T in the given shape. timer_class: the measurement class the launch is counted in
 // (by default SYRK for the trailing update shape and GEMM for the others).
 template <typename T, int LB = LAYOUT_COL>
 static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
-                         int64_t N, int64_t Kd, T* C, int64_t ldc, const int32_t* info,
-                         hipStream_t s, int timer_class = -1) {
+                         int64_t N, int64_t Kd, T* C, int64_t ldc, int32_t* info,
+                         hipStream_t s, int timer_class = -1, int w_next = 0,
+                         int64_t col_base_next = 0) {
     if (M <= 0 || N <= 0 || Kd <= 0) return QMLH_OK;
     const int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
     if (timer_class < 0) timer_class = mode == UPD_SYRK ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
     double flops = 2.0 * (double)M * (double)N * (double)Kd;
     if (mode == UPD_SYRK) flops = (double)M * (double)(M + 1) * (double)Kd;
-    if (mode == UPD_TRAP) flops = ((double)M * (double)N - (double)N * (double)(N - 1) / 2.0) * 2.0 * (double)Kd;
+    if (mode >= UPD_TRAP) flops = ((double)M * (double)N - (double)N * (double)(N - 1) / 2.0) * 2.0 * (double)Kd;
     TimerScope ts(timer_class, flops, s);
     switch (mode) {
         case UPD_SYRK:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
-                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
+                               (int64_t)0);
             break;
         case UPD_TRAP:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP, LB>), dim3((unsigned)(tm * tn)),
-                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
+                               (int64_t)0);
+            break;
+        case UPD_TRAP_POTF2:
+            hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP_POTF2, LB>), dim3((unsigned)(tm * tn)),
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, w_next,
+                               col_base_next);
             break;
         default:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_FULL, LB>), dim3((unsigned)(tm * tn)),
-                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm);
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
+                               (int64_t)0);
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
@@ -125,25 +143,14 @@ __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ A, int64
     trsm_body<T>(A, lda, w, m, info, blockIdx.x);
 }
 
-template <typename T>
-static int launch_panel64(T* A, int64_t lda, int w, int64_t m, int64_t col_base, int32_t* info,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, A, lda, w, col_base, info);
-    QMLH_CHECK_LAUNCH("potf2 launch");
-    if (m > 0) {
-        hipLaunchKernelGGL(trsm_rows_kernel<T>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0,
-                           s, A, lda, w, m, info);
-        QMLH_CHECK_LAUNCH("panel rows launch");
-    }
-    return QMLH_OK;
-}
-
 __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 
 // Factor the block column A[k:rows, k:k+kb] (its trailing updates from earlier columns
 // applied): per 64-wide sub-panel the diagonal factor, the rows below, and the update of the
 // rest of the block column. rows = n, or n + r when r extra rows below the matrix are solved
 // along (they are panel rows like any other).
+// The update after each sub-panel also factors the next sub-panel's diagonal block
+// (UPD_TRAP_POTF2), so only the first diagonal block of the column has a launch of its own.
 template <typename T>
 static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64_t kb,
                                int32_t* info, hipStream_t s) {
@@ -151,11 +158,22 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
         const int64_t c = k + jj;
         const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
         T* Acc = A + c + c * lda;
-        QMLH_TRY(launch_panel64<T>(Acc, lda, w, rows - c - w, c, info, s));
+        const int64_t m = rows - c - w;  // rows below the diagonal block
+        if (jj == 0) {
+            hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Acc, lda, w, c, info);
+            QMLH_CHECK_LAUNCH("potf2 launch");
+        }
+        if (m > 0) {
+            hipLaunchKernelGGL(trsm_rows_kernel<T>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0,
+                               s, Acc, lda, w, m, info);
+            QMLH_CHECK_LAUNCH("panel rows launch");
+        }
         const int64_t ncols = kb - jj - w;  // rest of this block column
-        if (ncols > 0)
-            QMLH_TRY(launch_update<T>(UPD_TRAP, Acc + w, lda, Acc + w, lda, rows - c - w, ncols,
-                                      w, Acc + w + w * lda, lda, info, s));
+        if (ncols > 0) {
+            const int w2 = (int)(ncols < PF_IB ? ncols : PF_IB);
+            QMLH_TRY(launch_update<T>(UPD_TRAP_POTF2, Acc + w, lda, Acc + w, lda, m, ncols, w,
+                                      Acc + w + w * lda, lda, info, s, -1, w2, c + w));
+        }
     }
     return QMLH_OK;
 }
