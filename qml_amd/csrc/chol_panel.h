@@ -129,13 +129,10 @@ __device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w) {
     return s_fail;
 }
 
-// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L packed in sp and the
-// reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded whole first (64 loads
-// in flight together); then, column by column, x_c is scaled and subtracted from the later
-// entries (independent FMAs; per entry the same operations in the same order as a row
-// substitution); then the row is stored. The branch on w per column keeps the compiler from
-// hoisting every column's LDS reads at once (which would spill). Addresses are a uniform
-// column pointer plus the lane's 32-bit byte offset (rows below 2^31 / 8).
+// Row `row` of the m x w panel B (column-major, ldb) := B L^-T with L packed by rows in sp
+// and the reciprocal pivots in inv; FP64 arithmetic, storage T. The row is loaded whole first
+// (64 loads in flight together), solved right-looking (below), and stored. Addresses are a
+// uniform column pointer plus the lane's 32-bit byte offset (rows below 2^31 / 8).
 template <typename T>
 __device__ __forceinline__ const T& at_byte(const T* base, uint32_t off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
@@ -145,10 +142,39 @@ __device__ __forceinline__ T& at_byte(T* base, uint32_t off) {
     return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
 }
 
+// Row-major packed lower triangle (row i holds columns 0 .. i at rp(i, 0) ..): the 8 entries
+// L(j, c .. c + 7) a lane needs together are adjacent.
+__host__ __device__ constexpr int rp(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// LDS (row-major packed) <- the lower triangle of the w x w block at A, zero outside w.
+template <typename T>
+__device__ __forceinline__ void stage_packed64_rows(double* __restrict__ sp,
+                                                    const T* __restrict__ A, int64_t lda, int w) {
+    const int t = threadIdx.x;
+    T raw[CP_B * CP_B / 256];
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        raw[q] = A[cp_clamp(i, w) + (int64_t)cp_clamp(j, w) * lda];
+    }
+#pragma unroll
+    for (int q = 0; q < CP_B * CP_B / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int i = idx & (CP_B - 1), j = idx / CP_B;
+        if (i >= j) sp[rp(i, j)] = (i < w && j < w) ? (double)raw[q] : 0.0;
+    }
+}
+
+// The substitution runs in blocks of 8 columns: the 8 columns of a block are finished among
+// themselves, then every later entry takes the block's 8 contributions at once from 8
+// adjacent LDS values -- independent per entry, so the reads stay in flight together instead
+// of one or two at a time. Per entry the updates come in the same order as column by column.
 template <typename T>
 __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_t row, int w,
                                          const double* __restrict__ sp,
                                          const double* __restrict__ inv) {
+    constexpr int CB = 8;
     const uint32_t off = row * (uint32_t)sizeof(T);
     T raw[CP_B];
 #pragma unroll
@@ -157,11 +183,29 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_
 #pragma unroll
     for (int c = 0; c < CP_B; ++c) x[c] = c < w ? (double)raw[c] : 0.0;
 #pragma unroll
-    for (int c = 0; c < CP_B; ++c) {
-        if (c < w) {
-            x[c] = x[c] * inv[c];
+    for (int cb = 0; cb < CP_B; cb += CB) {
+        if (cb < w) {  // inv and sp are zero past w, which keeps x zero there
 #pragma unroll
-            for (int j = c + 1; j < CP_B; ++j) x[j] = fma(-x[c], sp[pk(j, c)], x[j]);
+            for (int c = cb; c < cb + CB; ++c) {
+                x[c] = x[c] * inv[c];
+#pragma unroll
+                for (int j = c + 1; j < cb + CB; ++j) x[j] = fma(-x[c], sp[rp(j, c)], x[j]);
+            }
+            // four entries at a time: their 32 LDS values loaded together, then four
+            // independent FMA chains interleaved
+#pragma unroll
+            for (int j = cb + CB; j < CP_B; j += 4) {
+                double l[4][CB];
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) l[jj][c] = sp[rp(j + jj, cb + c)];
+#pragma unroll
+                for (int c = 0; c < CB; ++c)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        x[j + jj] = fma(-x[cb + c], l[jj][c], x[j + jj]);
+            }
         }
     }
 #pragma unroll
@@ -208,9 +252,9 @@ __device__ __forceinline__ void trsm_body(T* __restrict__ A, int64_t lda, int w,
     __shared__ double inv[CP_B];
     if (*info) return;
     const int t = threadIdx.x;
-    stage_packed64(sp, A, lda, w);
+    stage_packed64_rows(sp, A, lda, w);
     __syncthreads();
-    if (t < CP_B) inv[t] = t < w ? 1.0 / sp[pk(t, t)] : 0.0;
+    if (t < CP_B) inv[t] = t < w ? 1.0 / sp[rp(t, t)] : 0.0;
     __syncthreads();
     const int64_t row = local * 256 + t;
     if (row < m) trsm_row<T>(A + w, lda, (uint32_t)row, w, sp, inv);
