@@ -75,6 +75,31 @@ def test_spotrf_f16x3_factor_accuracy(n):
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
+@pytest.mark.parametrize("n", [1500, 5000])
+def test_spotrf_f16x3_panel_scale_is_exact(n):
+    # The split panels are scaled by a power of two set from the largest diagonal entry, so
+    # scaling A by 4^k scales L by exactly 2^k -- also where the unscaled entries would overflow
+    # FP16 (k = 20: |L| ~ 1e6) or fall into its subnormals (k = -20: |L| ~ 1e-6).
+    from qml_amd import _lib
+
+    L = _lib.lib()
+    rng = np.random.default_rng(n + 5)
+    M = rng.standard_normal((n, n))
+    A = (M @ M.T / n + np.eye(n)).astype(np.float32)
+    ws = torch.empty(L.qmlh_spotrf_f16x3_workspace(n) // 8 + 1, dtype=torch.float64,
+                     device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    factors = {}
+    for k in (0, 20, -20):
+        Ad = torch.from_numpy(A * np.float32(4.0 ** k)).cuda()
+        _lib.check(L.qmlh_spotrf_f16x3(Ad.data_ptr(), n, n, info.data_ptr(), ws.data_ptr(),
+                                       ws.numel() * 8, None))
+        assert int(info.item()) == 0
+        factors[k] = np.tril(Ad.cpu().numpy().T)
+    for k in (20, -20):
+        assert np.array_equal(factors[k], factors[0] * np.float32(2.0 ** k))
+
+
 @pytest.mark.parametrize("n,bad", [(300, 130), (4400, 1000), (4400, 4390)])
 def test_spotrf_info_matches_lapack(n, bad):
     from scipy.linalg import lapack
