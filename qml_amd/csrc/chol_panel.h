@@ -103,8 +103,15 @@ __device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w) {
                 f = c + 1;
                 break;
             }
-            const double l = sqrt(piv);
-            const double rl = 1.0 / l;
+            // 1/sqrt(piv) from the hardware estimate and two Newton steps (each squares the
+            // relative error; about one unit in the last place after the second), then
+            // sqrt(piv) = piv / sqrt(piv): five dependent operations on the critical path
+            // instead of a correctly rounded square root followed by a division.
+            double rl = __builtin_amdgcn_rsq(piv);
+            const double hp = 0.5 * piv;
+            rl = rl * fma(-hp * rl, rl, 1.5);
+            rl = rl * fma(-hp * rl, rl, 1.5);
+            const double l = piv * rl;
             a[c] = lane == c ? l : (lane > c ? a[c] * rl : a[c]);
             if (c + 1 < CP_B) {
                 colbuf[lane] = a[c];

@@ -153,7 +153,7 @@ __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 // (UPD_TRAP_POTF2), so only the first diagonal block of the column has a launch of its own.
 template <typename T>
 static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64_t kb,
-                               int32_t* info, hipStream_t s) {
+                               int32_t* info, hipStream_t s, hipEvent_t rows_wait = nullptr) {
     for (int64_t jj = 0; jj < kb; jj += PF_IB) {
         const int64_t c = k + jj;
         const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
@@ -163,6 +163,8 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
             hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Acc, lda, w, c, info);
             QMLH_CHECK_LAUNCH("potf2 launch");
         }
+        if (jj == 0 && rows_wait)  // the rows' last update may still run on another stream
+            QMLH_TRY(check_hip(hipStreamWaitEvent(s, rows_wait, 0), "potrf: wait"));
         if (m > 0) {
             hipLaunchKernelGGL(trsm_rows_kernel<T>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0,
                                s, Acc, lda, w, m, info);
@@ -211,21 +213,25 @@ static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 // bandwidth/MFMA-bound trailing update. Events order the two streams; the caller's stream waits
 // for the last panel, so on return all work is ordered on `s`.
 //
-// With r extra rows (A[n:n+r, :]), the next block column's update and panel include them (with
-// split-FP16 updates, their part of the next block column is an FP32 product on the side
-// stream). The rest of their trailing update touches nothing the trailing update or the panel
-// reads or writes: while the trailing matrix is large (the caller's stream is the critical
-// path) it runs on the side stream after the panel, later (the panels are the critical path)
-// on the caller's stream after the trailing update.
+// With r extra rows (A[n:n+r, :]), the next block column's update and panel include them. The
+// rest of their trailing update (an FP32 product) touches nothing the trailing update or the
+// panel reads or writes: while the trailing matrix is large (the caller's stream is the
+// critical path) it runs on the side stream after the panel, later (the panels are the
+// critical path) on the caller's stream after the trailing update.
 template <typename T>
 static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
                            hipStream_t s, void* split_ws, int64_t nb) {
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr;
+    hipStream_t s2 = nullptr, s3 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr, ev_e = nullptr;
+    const bool third = r > 0 && split_ws;  // extra rows beside split-FP16 updates
     int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming), "potrf: event");
     if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_p, hipEventDisableTiming), "potrf: event");
-    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_x, hipEventDisableTiming), "potrf: event");
+    if (rc == QMLH_OK && third) {
+        rc = check_hip(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking), "potrf: third stream");
+        if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_x, hipEventDisableTiming), "potrf: event");
+        if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_e, hipEventDisableTiming), "potrf: event");
+    }
     if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s);
     for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
@@ -235,22 +241,24 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         T* P = A + (k + kb) + k * lda;
         T* C = P + kb * lda;  // A(k + kb, k + kb)
         const bool split = std::is_same<T, float>::value && split_ws && m > kb2;
+        hipEvent_t rows_wait = nullptr;  // what the next panel's rows wait for
         if (split) {
             // split-FP16 trailing updates: split the panel once; the next block column gets
-            // the same split-FP16 update as the rest (its extra rows FP32), then the rest of
-            // the trailing matrix reuses the split from row kb2 on
+            // the same split-FP16 update as the rest, then the rest of the trailing matrix
+            // reuses the split from row kb2 on. The extra rows stay out of the split (their
+            // entries are not bounded by the diagonal that sets the split's scale): their part
+            // of the next block column is an FP32 product on a third stream, which the next
+            // panel waits for only before its rows are solved.
             float* Pf = reinterpret_cast<float*>(P);
             float* Cf = reinterpret_cast<float*>(C);
-            // the extra rows' part (a small FP32 product) on the side stream, beside the split
-            // and the FP16 update: it needs the finished panel and the extra rows' earlier
-            // updates (both ordered before ev_x on the caller's stream), and the side stream
-            // runs it before the next panel
             if (r > 0) {
                 rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
-                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_x, 0), "potrf: wait");
+                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
                 if (rc == QMLH_OK)
                     rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
-                                          info, s2);
+                                          info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+                rows_wait = ev_e;
             }
             if (rc == QMLH_OK) rc = f16x3_split(Pf, lda, m, kb, split_ws, s);
             if (rc == QMLH_OK)
@@ -260,7 +268,8 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         }
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
-        if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2);
+        if (rc == QMLH_OK)
+            rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2, rows_wait);
         const bool ext_side = m >= PF_EXT_SIDE_MIN;
         if (rc == QMLH_OK && ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s2);
@@ -278,10 +287,13 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
     }
     // Destroying them here is safe: pending work completes first, and `s` already waits for it.
+    // (the third stream's last work precedes the last panel's rows, which `s` waits for)
     if (ev_a) (void)hipEventDestroy(ev_a);
     if (ev_p) (void)hipEventDestroy(ev_p);
     if (ev_x) (void)hipEventDestroy(ev_x);
+    if (ev_e) (void)hipEventDestroy(ev_e);
     if (s2) (void)hipStreamDestroy(s2);
+    if (s3) (void)hipStreamDestroy(s3);
     return rc;
 }
 
