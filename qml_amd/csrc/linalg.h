@@ -32,11 +32,11 @@ int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int
 // Synchronisation words the single-launch triangular solve needs for order n.
 size_t trsv_sync_ints(int64_t n);
 
-// Single-launch triangular solve (forward z = L^-1 rhs, or backward x = L^-T rhs). dinv:
-// optional FP64 inverses of the 64 x 64 diagonal blocks (trsv_diag_inverse).
+// Single-launch triangular solve (forward z = L^-1 rhs, or backward x = L^-T rhs) in 64-row
+// blocks with FP64 substitution.
 template <typename T>
 int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
-                  int32_t* sync, bool forward, hipStream_t s, const double* dinv = nullptr);
+                  int32_t* sync, bool forward, hipStream_t s);
 
 // Orders from which the triangular solves run as one sync-free launch.
 static constexpr int64_t TRSV_SYNCFREE_MIN = 1024;
@@ -59,15 +59,15 @@ template <typename T>
 int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32_t* sync,
                hipStream_t s);
 
-// z := L^-1 r (r may be clobbered); dinv as for trsv_syncfree (used for large n).
+// z := L^-1 r (r may be clobbered); one sync-free launch for large n when sync is given.
 template <typename T>
 int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
-                   hipStream_t s, const double* dinv = nullptr);
+                   hipStream_t s);
 
 // x := L^-T z (z may be clobbered).
 template <typename T>
 int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
-                   hipStream_t s, const double* dinv = nullptr);
+                   hipStream_t s);
 
 // V := L^-1 V for an (n, nrhs) column-major right-hand side (storage float), L lower float;
 // dinv: the FP64 inverses of L's 64 x 64 diagonal blocks (trsv_diag_inverse).

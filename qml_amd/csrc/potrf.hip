@@ -431,9 +431,8 @@ __global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const T* __restrict_
 
 template <typename T>
 int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int32_t* sync,
-                   hipStream_t s, const double* dinv) {
-    if (n >= TRSV_SYNCFREE_MIN && sync)
-        return trsv_syncfree<T>(L, n, lda, r, z, sync, true, s, dinv);
+                   hipStream_t s) {
+    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, r, z, sync, true, s);
     for (int64_t k = 0; k < n; k += PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
         const int64_t rest = n - k - w;
@@ -451,9 +450,8 @@ int trsv_lower_fwd(const T* L, int64_t n, int64_t lda, double* r, double* z, int
 
 template <typename T>
 int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
-                   hipStream_t s, const double* dinv) {
-    if (n >= TRSV_SYNCFREE_MIN && sync)
-        return trsv_syncfree<T>(L, n, lda, z, x, sync, false, s, dinv);
+                   hipStream_t s) {
+    if (n >= TRSV_SYNCFREE_MIN && sync) return trsv_syncfree<T>(L, n, lda, z, x, sync, false, s);
     const int64_t last = ((n - 1) / PF_IB) * PF_IB;
     for (int64_t k = last; k >= 0; k -= PF_IB) {
         const int w = (int)(n - k < PF_IB ? n - k : PF_IB);
@@ -599,16 +597,14 @@ template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
                                hipStream_t);
-template int trsv_lower_fwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*, hipStream_t,
-                                   const double*);
-template int trsv_lower_bwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*, hipStream_t,
-                                   const double*);
+template int trsv_lower_fwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
+                                   hipStream_t);
+template int trsv_lower_bwd<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
+                                   hipStream_t);
 template int trsv_lower_fwd<double>(const double*, int64_t, int64_t, double*, double*,
-                                    int32_t*, hipStream_t,
-                                   const double*);
+                                    int32_t*, hipStream_t);
 template int trsv_lower_bwd<double>(const double*, int64_t, int64_t, double*, double*,
-                                    int32_t*, hipStream_t,
-                                   const double*);
+                                    int32_t*, hipStream_t);
 
 }  // namespace qmlh
 

@@ -152,10 +152,8 @@ __device__ __forceinline__ void ts_consume(TsStage<T>& st, int64_t n, int64_t nb
 }
 
 // Solve block b of one system: wait for the values of the blocks it depends on (the earlier
-// blocks in solve order), accumulate their contributions, solve the diagonal block, publish.
-// `err` is the shared error word. With dinv (the FP64 inverses of the 64 x 64 diagonal blocks,
-// trsv_diag_inverse) the diagonal solve is a matrix-vector product spread over all lanes
-// instead of a 64-step substitution on the critical path. All 256 lanes call this.
+// blocks in solve order), accumulate their contributions, solve the diagonal block by FP64
+// substitution, publish. `err` is the shared error word. All 256 lanes call this.
 //
 // Dependencies are pipelined TS_DEPTH deep: the tile and published values of the next blocks
 // are in flight while the current one is accumulated, so a block whose dependencies are long
@@ -166,8 +164,7 @@ static constexpr int TS_DEPTH = 4;
 template <typename T, bool FWD>
 __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl, int64_t n,
                                            const double* __restrict__ rhs, double* out,
-                                           int32_t* __restrict__ err, int64_t b,
-                                           const double* __restrict__ dinv) {
+                                           int32_t* __restrict__ err, int64_t b) {
     __shared__ double sL[TS_B][TS_B + 1];
     __shared__ double sInv[TS_B];
     __shared__ double red[TS_B][TS_B + 1];
@@ -183,22 +180,7 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
     const int grp_s = __builtin_amdgcn_readfirstlane(grp);  // wave-uniform
 
     // The diagonal block does not depend on other blocks: stage it first.
-    if (dinv) {
-        // inverse block: FWD x = D s, BWD x = D^T s (lane (i, part) later uses row i, entries
-        // 16 part .. 16 part + 15). Loads first, then the LDS stores.
-        const double* D = dinv + b * (TS_B * TS_B);
-        double v[TS_B * TS_B / 256];
-#pragma unroll
-        for (int q = 0; q < TS_B * TS_B / 256; ++q) v[q] = D[t + 256 * q];
-#pragma unroll
-        for (int q = 0; q < TS_B * TS_B / 256; ++q) {
-            const int idx = t + 256 * q;
-            const int i = idx % TS_B, jj = idx / TS_B;  // D(i, jj)
-            if (FWD) sL[i][jj] = v[q]; else sL[jj][i] = v[q];
-        }
-    } else {
-        stage_lower64<false, TS_B + 1>(sL, L + r0 + r0 * ldl, ldl, w);
-    }
+    stage_lower64<false, TS_B + 1>(sL, L + r0 + r0 * ldl, ldl, w);
     __syncthreads();  // s_fail initialised before any lane may set it
 
     // Forward: thread (row lane, columns 16 grp ..) keeps one partial sum.
@@ -252,22 +234,10 @@ __device__ __forceinline__ void trsv_block(const T* __restrict__ L, int64_t ldl,
             s = ((redp[0][t] + redp[1][t]) + redp[2][t]) + redp[3][t];
         }
         sv[t] = (t < w ? rhs_v : 0.0) - s;
-        if (t < w && !dinv) sInv[t] = 1.0 / sL[t][t];
+        if (t < w) sInv[t] = 1.0 / sL[t][t];
     }
     __syncthreads();
-    if (dinv) {
-        // x_i = sum_c M(i, c) sv_c: four partial sums per row, added in a fixed order.
-        double p = 0.0;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) p = fma(sL[lane][16 * grp + c], sv[16 * grp + c], p);
-        red[lane][grp] = p;
-        __syncthreads();
-        if (t < 64 && t < w) {
-            const double xi = ((red[t][0] + red[t][1]) + red[t][2]) + red[t][3];
-            __hip_atomic_store(&out[r0 + t], s_fail ? __builtin_nan("") : xi, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    } else if (t < 64) {
+    if (t < 64) {
         const int i = t;
         double xi = sv[i];
         if (FWD) {
@@ -322,8 +292,7 @@ template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict__ L, int64_t ldl,
                                                             int64_t n, const double* __restrict__ rhs,
                                                             double* out,
-                                                            int32_t* __restrict__ sync,
-                                                            const double* __restrict__ dinv) {
+                                                            int32_t* __restrict__ sync) {
     __shared__ int s_block;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
     if (threadIdx.x == 0) {
@@ -331,7 +300,7 @@ __global__ __launch_bounds__(256) void trsv_syncfree_kernel(const T* __restrict_
         s_block = FWD ? ticket : (int)(nblocks - 1 - ticket);
     }
     __syncthreads();
-    trsv_block<T, FWD>(L, ldl, n, rhs, out, &sync[1], s_block, dinv);
+    trsv_block<T, FWD>(L, ldl, n, rhs, out, &sync[1], s_block);
 }
 
 size_t trsv_sync_ints(int64_t n) { return (size_t)((n + TS_B - 1) / TS_B + 2); }
@@ -349,25 +318,25 @@ int trsv_diag_inverse(const T* L, int64_t n, int64_t ldl, double* dinv, hipStrea
 
 template <typename T>
 int trsv_syncfree(const T* L, int64_t n, int64_t ldl, const double* rhs, double* out,
-                  int32_t* sync, bool forward, hipStream_t s, const double* dinv) {
+                  int32_t* sync, bool forward, hipStream_t s) {
     if (n <= 0) return QMLH_OK;
     const int64_t nblocks = (n + TS_B - 1) / TS_B;
     hipLaunchKernelGGL(trsv_prepare_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, sync,
                        (int64_t)2, out, n);
     if (forward)
         hipLaunchKernelGGL((trsv_syncfree_kernel<T, true>), dim3((unsigned)nblocks), dim3(256), 0,
-                           s, L, ldl, n, rhs, out, sync, dinv);
+                           s, L, ldl, n, rhs, out, sync);
     else
         hipLaunchKernelGGL((trsv_syncfree_kernel<T, false>), dim3((unsigned)nblocks), dim3(256),
-                           0, s, L, ldl, n, rhs, out, sync, dinv);
+                           0, s, L, ldl, n, rhs, out, sync);
     QMLH_CHECK_LAUNCH("trsv launch");
     return QMLH_OK;
 }
 
 template int trsv_syncfree<double>(const double*, int64_t, int64_t, const double*, double*,
-                                   int32_t*, bool, hipStream_t, const double*);
+                                   int32_t*, bool, hipStream_t);
 template int trsv_syncfree<float>(const float*, int64_t, int64_t, const double*, double*,
-                                  int32_t*, bool, hipStream_t, const double*);
+                                  int32_t*, bool, hipStream_t);
 template int trsv_diag_inverse<double>(const double*, int64_t, int64_t, double*, hipStream_t);
 template int trsv_diag_inverse<float>(const float*, int64_t, int64_t, double*, hipStream_t);
 
