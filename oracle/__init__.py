@@ -67,6 +67,14 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, _c_double_p]
             f.restype = None
+        dbl = ctypes.c_double
+        L.oracle_atomic_cm.argtypes = [_c_double_p, _c_double_p, i64, _c_int64_p, i64, i64,
+                                       ctypes.c_int, dbl, dbl, dbl, dbl, _c_double_p]
+        L.oracle_atomic_cm.restype = i64
+        L.oracle_local_kernels.argtypes = [ctypes.c_int, _c_double_p, _c_int64_p, i64,
+                                           _c_double_p, _c_int64_p, i64, i64, _c_double_p, i64,
+                                           _c_double_p]
+        L.oracle_local_kernels.restype = None
         _lib = L
     return _lib
 
@@ -89,6 +97,14 @@ def ref():
         L.ref_kernel.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p,
                                  ctypes.c_double]
         L.ref_kernel.restype = None
+        dbl = ctypes.c_double
+        _c_int_p = ctypes.POINTER(ctypes.c_int)
+        L.ref_atomic_cm.argtypes = [_c_double_p, _c_double_p, ci, _c_int_p, ci, ci, ci, dbl, dbl,
+                                    dbl, dbl, _c_double_p]
+        L.ref_atomic_cm.restype = None
+        L.ref_local_kernels.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_int_p, ci,
+                                        _c_int_p, ci, _c_double_p, ci, _c_double_p]
+        L.ref_local_kernels.restype = None
         _ref = L
     return _ref
 
@@ -135,6 +151,95 @@ def ref_coulomb_matrix(Z, R, size, sorting="row-norm"):
     ref().ref_cm(_ptr(Z), _ptr(R), Z.shape[0], size, 1 if sorting == "row-norm" else 0,
                  _ptr(out))
     return out
+
+
+ACM_SORTING = {"row-norm": 0, "distance": 1}
+NUCLEAR_CHARGE = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9,
+                  "Ne": 10, "Na": 11, "Mg": 12, "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17,
+                  "Ar": 18}
+
+
+def _acm_indices(Z, indices):
+    # qml/representations.py:205-221: None -> all atoms, an element symbol -> the atoms of that
+    # element (empty -> (0, 0) result), else 0-based indices.
+    if indices is None:
+        return np.arange(Z.shape[0], dtype=np.int64)
+    if isinstance(indices, str):
+        if indices not in NUCLEAR_CHARGE:
+            raise ValueError(f"unknown value {indices!r} for 'indices'")
+        return np.where(Z == NUCLEAR_CHARGE[indices])[0].astype(np.int64)
+    return np.asarray(indices, dtype=np.int64).reshape(-1)
+
+
+def atomic_coulomb_matrix(Z, R, size=23, sorting="distance", central_cutoff=1e6,
+                          central_decay=-1, interaction_cutoff=1e6, interaction_decay=-1,
+                          indices=None):
+    """One molecule; mirrors qml/representations.py:121-233 over the C restatement."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    cent = _acm_indices(Z, indices)
+    if isinstance(indices, str) and cent.size == 0:
+        return np.zeros((0, 0))
+    out = np.zeros((cent.size, size * (size + 1) // 2))
+    rc = lib().oracle_atomic_cm(_ptr(Z), _ptr(R), Z.shape[0], _ptr(cent, _c_int64_p), cent.size,
+                                size, ACM_SORTING[sorting], central_cutoff, central_decay,
+                                interaction_cutoff, interaction_decay, _ptr(out))
+    if rc:
+        raise ValueError(f"{size} size set, but {rc} size needed")
+    return out
+
+
+def ref_atomic_coulomb_matrix(Z, R, size=23, sorting="distance", central_cutoff=1e6,
+                              central_decay=-1, interaction_cutoff=1e6, interaction_decay=-1,
+                              indices=None):
+    """The reference's own fgenerate_{local,atomic}_coulomb_matrix via oracle/_ref."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    cent = _acm_indices(Z, indices)
+    if isinstance(indices, str) and cent.size == 0:
+        return np.zeros((0, 0))
+    cent1 = np.ascontiguousarray(cent + 1, dtype=np.int32)
+    out = np.zeros((cent.size, size * (size + 1) // 2), order="F")
+    ref().ref_atomic_cm(_ptr(Z), _ptr(R), Z.shape[0], _ptr(cent1, ctypes.POINTER(ctypes.c_int)),
+                        cent.size, size, ACM_SORTING[sorting], float(central_cutoff),
+                        float(central_decay), float(interaction_cutoff),
+                        float(interaction_decay), _ptr(out))
+    return np.ascontiguousarray(out)
+
+
+def _local_prep(A, B, na, nb, sigmas):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    na = np.asarray(na, dtype=np.int64).reshape(-1)
+    nb = np.asarray(nb, dtype=np.int64).reshape(-1)
+    assert np.sum(na) == A.shape[0] and np.sum(nb) == B.shape[0] and A.shape[1] == B.shape[1]
+    sig = np.ascontiguousarray(np.asarray(sigmas, dtype=np.float64).reshape(-1))
+    return A, B, na, nb, sig
+
+
+def local_kernels(kind, A, B, na, nb, sigmas):
+    """qml/kernels.py:218-298 over the C restatement: (nsig, nmA, nmB), Fortran order."""
+    A, B, na, nb, sig = _local_prep(A, B, na, nb, sigmas)
+    offA = np.concatenate([[0], np.cumsum(na)]).astype(np.int64)
+    offB = np.concatenate([[0], np.cumsum(nb)]).astype(np.int64)
+    K = np.empty((sig.size, na.size, nb.size), order="F")
+    lib().oracle_local_kernels(1 if kind == "gaussian" else 0, _ptr(A), _ptr(offA, _c_int64_p),
+                               na.size, _ptr(B), _ptr(offB, _c_int64_p), nb.size, A.shape[1],
+                               _ptr(sig), sig.size, _ptr(K))
+    return K
+
+
+def ref_local_kernels(kind, A, B, na, nb, sigmas):
+    """The reference's own fget_local_kernels_{gaussian,laplacian} via oracle/_ref."""
+    A, B, na, nb, sig = _local_prep(A, B, na, nb, sigmas)
+    n1 = np.ascontiguousarray(na, dtype=np.int32)
+    n2 = np.ascontiguousarray(nb, dtype=np.int32)
+    K = np.empty((sig.size, na.size, nb.size), order="F")
+    ip = ctypes.POINTER(ctypes.c_int)
+    ref().ref_local_kernels(1 if kind == "gaussian" else 0, _ptr(A), A.shape[0], _ptr(B),
+                            B.shape[0], A.shape[1], _ptr(n1, ip), na.size, _ptr(n2, ip), nb.size,
+                            _ptr(sig), sig.size, _ptr(K))
+    return K
 
 
 # --------------------------------------------------------------------------------------------

@@ -174,3 +174,212 @@ void oracle_l2(const double *A, int64_t na, const double *B, int64_t nb, int64_t
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Atomic (local) Coulomb matrices: qml/frepresentations.f90:207-413 (row-norm ordering,
+ * fgenerate_local_coulomb_matrix) and 415-598 (distance ordering,
+ * fgenerate_atomic_coulomb_matrix), one central atom at a time, one thread.
+ * ------------------------------------------------------------------------------------------ */
+
+/* Cut-off parameter normalisation shared by both routines (lines 254-273 / 460-479). */
+void oracle_acm_params(double *cc, double *cd, double *ic, double *id) {
+    const double huge = DBL_MAX;
+    if (*cc < 0) *cc = huge;
+    if ((*ic < 0) || (*ic > 2 * *cc)) *ic = 2 * *cc;
+    if (*cd < 0) *cd = 0.0;
+    else if (*cd > *cc) *cd = *cc;
+    if (*id < 0) *id = 0.0;
+    else if (*id > *ic) *id = *ic;
+}
+
+/* |R_b - R_a| for a < b, coordinates summed x, y, z in order (lines 279-285). */
+static double acm_dist(const double *R, int64_t a, int64_t b) {
+    if (a == b) return 0.0;
+    if (a > b) { int64_t t = a; a = b; b = t; }
+    double dx = R[3 * b + 0] - R[3 * a + 0];
+    double dy = R[3 * b + 1] - R[3 * a + 1];
+    double dz = R[3 * b + 2] - R[3 * a + 2];
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return sqrt(s);
+}
+
+/* 0.5 (cos(pi (r - c + w) / w) + 1), the decay factor of lines 321-322 / 565-567. */
+static double acm_decay(double r, double c, double w) {
+    const double pi = 4.0 * atan(1.0);
+    return 0.5 * (cos(pi * (r - c + w) / w) + 1.0);
+}
+
+/* Row-norm ordered matrix of central atom k (0-based) into cm[d]; returns 0, or the needed
+ * size when the atoms within the central cut-off exceed `size` (the reference stops there). */
+static int64_t acm_rownorm_one(const double *Z, const double *R, int64_t n, int64_t k,
+                               int64_t size, double cc, double cd, double ic, double id,
+                               double *cm, double *P, double *rn, double *dk, int64_t *order) {
+    int64_t cnt = 1;
+    for (int64_t x = 0; x < n; ++x) {
+        dk[x] = acm_dist(R, x, k);
+        if (x != k && dk[x] < cc) ++cnt;
+    }
+    if (cnt > size) return cnt;
+    for (int64_t x = 0; x < n * n; ++x) P[x] = 0.0;
+    for (int64_t x = 0; x < n; ++x) rn[x] = 0.0;
+    /* lines 311-351: rows i inside the central cut-off; pair (i, j > i) inside both cut-offs. */
+    for (int64_t i = 0; i < n; ++i) {
+        if (dk[i] > cc) continue;
+        double pref = 1.0;
+        if (dk[i] > cc - cd) pref = acm_decay(dk[i], cc, cd);
+        double pn = pref * pref * 0.5 * pow(Z[i], 2.4);
+        P[i * n + i] = pn;
+        rn[i] = rn[i] + pn * pn;
+        for (int64_t j = i + 1; j < n; ++j) {
+            if (dk[j] > cc) continue;
+            double dij = acm_dist(R, i, j);
+            if (dij > ic) continue;
+            double p = pref * Z[i] * Z[j] / dij;
+            if (dij > ic - id) p = p * 0.5 * (cos(4.0 * atan(1.0) * (dij - ic + id) / id) + 1);
+            if (dk[j] > cc - cd) p = p * 0.5 * (cos(4.0 * atan(1.0) * (dk[j] - cc + cd) / cd) + 1);
+            P[i * n + j] = p;
+            P[j * n + i] = p;
+            double p2 = p * p;
+            rn[i] = rn[i] + p2;
+            rn[j] = rn[j] + p2;
+        }
+    }
+    /* lines 355-373: central atom first, then repeated maxloc (first index of the maximum),
+     * picked entries set to zero. */
+    rn[k] = DBL_MAX;
+    for (int64_t it = 0; it < cnt; ++it) {
+        int64_t best = 0;
+        for (int64_t x = 1; x < n; ++x)
+            if (rn[x] > rn[best]) best = x;
+        order[it] = best;
+        rn[best] = 0.0;
+    }
+    /* lines 383-395: packed lower triangle of the reordered matrix. */
+    memset(cm, 0, sizeof(double) * (size_t)(size * (size + 1) / 2));
+    for (int64_t m = 0; m < cnt; ++m) {
+        int64_t base = m * (m + 1) / 2;
+        for (int64_t c = 0; c <= m; ++c) cm[base + c] = P[order[m] * n + order[c]];
+    }
+    return 0;
+}
+
+/* Distance ordered matrix of central atom k (lines 481-592). */
+static int64_t acm_distance_one(const double *Z, const double *R, int64_t n, int64_t k,
+                                int64_t size, double cc, double cd, double ic, double id,
+                                double *cm, double *P, double *tmp, double *dk,
+                                int64_t *order) {
+    int64_t cnt = 1;
+    for (int64_t x = 0; x < n; ++x) {
+        dk[x] = acm_dist(R, x, k);
+        if (x != k && dk[x] < cc) ++cnt;
+    }
+    if (cnt > size) return cnt;
+    /* lines 510-529: pair matrix with the interaction cut-off only. */
+    for (int64_t i = 0; i < n; ++i) {
+        P[i * n + i] = 0.5 * pow(Z[i], 2.4);
+        for (int64_t j = i + 1; j < n; ++j) {
+            double dij = acm_dist(R, i, j);
+            double p = 0.0;
+            if (!(dij > ic)) {
+                p = Z[i] * Z[j] / dij;
+                if (dij > ic - id) p = p * 0.5 * (cos(4.0 * atan(1.0) * (dij - ic + id) / id) + 1);
+            }
+            P[i * n + j] = p;
+            P[j * n + i] = p;
+        }
+    }
+    /* lines 535-546: repeated minloc over the distances to k, picked entries set to huge. */
+    for (int64_t x = 0; x < n; ++x) tmp[x] = dk[x];
+    for (int64_t it = 0; it < cnt; ++it) {
+        int64_t best = 0;
+        for (int64_t x = 1; x < n; ++x)
+            if (tmp[x] < tmp[best]) best = x;
+        order[it] = best;
+        tmp[best] = DBL_MAX;
+    }
+    /* lines 556-586: rows outside the central cut-off stay zero; entry (m, c) carries the
+     * central decay of the row atom as a prefactor and that of the column atom last. */
+    memset(cm, 0, sizeof(double) * (size_t)(size * (size + 1) / 2));
+    for (int64_t m = 0; m < cnt; ++m) {
+        int64_t i = order[m];
+        if (dk[i] > cc) continue;
+        double pref = 1.0;
+        if (dk[i] > cc - cd) pref = acm_decay(dk[i], cc, cd);
+        int64_t base = m * (m + 1) / 2;
+        for (int64_t c = 0; c <= m; ++c) {
+            int64_t j = order[c];
+            double p = pref * P[i * n + j];
+            if (dk[j] > cc - cd) p = p * 0.5 * (cos(4.0 * atan(1.0) * (dk[j] - cc + cd) / cd) + 1);
+            cm[base + c] = p;
+        }
+    }
+    return 0;
+}
+
+/* One molecule, `ncent` central atoms (0-based indices) -> out (ncent, size(size+1)/2).
+ * sorting: 0 row-norm, 1 distance. Cut-offs are normalised as the reference does. Returns 0,
+ * or the size the first offending central atom needs. */
+int64_t oracle_atomic_cm(const double *Z, const double *R, int64_t n, const int64_t *cent,
+                         int64_t ncent, int64_t size, int sorting, double cc, double cd,
+                         double ic, double id, double *out) {
+    oracle_acm_params(&cc, &cd, &ic, &id);
+    int64_t d = size * (size + 1) / 2;
+    double *P = (double *)malloc(sizeof(double) * (size_t)(n * n + 1));
+    double *rn = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *dk = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    int64_t *order = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    int64_t rc = 0;
+    for (int64_t l = 0; l < ncent && rc == 0; ++l) {
+        if (sorting == 0)
+            rc = acm_rownorm_one(Z, R, n, cent[l], size, cc, cd, ic, id, out + l * d, P, rn, dk,
+                                 order);
+        else
+            rc = acm_distance_one(Z, R, n, cent[l], size, cc, cd, ic, id, out + l * d, P, rn,
+                                  dk, order);
+    }
+    free(P); free(rn); free(dk); free(order);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Local (atom-summed) kernels: qml/fkernels.f90:23-106 (Gaussian) and 108-196 (Laplacian).
+ *   K(s, a, b) = sum over atoms j of b (outer), i of a (inner) of exp(D(i, j) * inv_s)
+ * with D the squared L2 (Gaussian, inv = -0.5/sigma^2) or L1 (Laplacian, inv = -1/sigma)
+ * distance summed serially over the features, and the atom-pair sum in the column-major order
+ * of the Fortran sum() over atomic_distance(:ni, :nj). Output Fortran-ordered
+ * (nsig, nmA, nmB): element (s, a, b) at K[s + nsig (a + nmA b)].
+ * kind: 0 Laplacian, 1 Gaussian. OpenMP over molecules of B.
+ * ------------------------------------------------------------------------------------------ */
+void oracle_local_kernels(int kind, const double *A, const int64_t *offA, int64_t nmA,
+                          const double *B, const int64_t *offB, int64_t nmB, int64_t d,
+                          const double *sigmas, int64_t nsig, double *K) {
+    double *inv = (double *)malloc(sizeof(double) * (size_t)(nsig + 1));
+    for (int64_t s = 0; s < nsig; ++s)
+        inv[s] = kind == 1 ? -0.5 / (sigmas[s] * sigmas[s]) : -1.0 / sigmas[s];
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t b = 0; b < nmB; ++b) {
+        for (int64_t a = 0; a < nmA; ++a) {
+            double *k = K + nsig * (a + nmA * b);
+            for (int64_t s = 0; s < nsig; ++s) k[s] = 0.0;
+            for (int64_t j = offB[b]; j < offB[b + 1]; ++j) {
+                const double *y = B + j * d;
+                for (int64_t i = offA[a]; i < offA[a + 1]; ++i) {
+                    const double *x = A + i * d;
+                    double dist = 0.0;
+                    if (kind == 1) {
+                        for (int64_t f = 0; f < d; ++f) {
+                            double t = x[f] - y[f];
+                            dist = dist + t * t;
+                        }
+                    } else {
+                        for (int64_t f = 0; f < d; ++f) dist = dist + fabs(x[f] - y[f]);
+                    }
+                    for (int64_t s = 0; s < nsig; ++s) k[s] = k[s] + exp(dist * inv[s]);
+                }
+            }
+        }
+    }
+    free(inv);
+}

@@ -81,3 +81,111 @@ subroutine ref_kernel(kind, a, na, b, nb, d, k, sigma) bind(C, name="ref_kernel"
     call fl2_distance(a, b, k)
   end select
 end subroutine
+
+subroutine ref_atomic_cm(z, r_t, natoms, cent1, ncent, nmax, sorting, cc, cd, ic, id, cm) &
+    bind(C, name="ref_atomic_cm")
+  ! cent1: 1-based central atom indices (qml/representations.py:205-220); cm is Fortran
+  ! (ncent, nmax(nmax+1)/2) as the reference returns it (frepresentations.f90:225/428).
+  ! sorting: 0 row-norm (fgenerate_local_coulomb_matrix), 1 distance
+  ! (fgenerate_atomic_coulomb_matrix). The cut-offs are intent(inout) there, so copies go in.
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: natoms, ncent, nmax, sorting
+  real(c_double), intent(in) :: z(natoms), r_t(3, natoms)
+  integer(c_int), intent(in) :: cent1(ncent)
+  real(c_double), value :: cc, cd, ic, id
+  real(c_double), intent(out) :: cm(ncent, (nmax * (nmax + 1)) / 2)
+  real(c_double), allocatable :: coords(:, :)
+  integer, allocatable :: idx(:)
+  double precision :: c1, c2, c3, c4
+  interface
+    subroutine fgenerate_local_coulomb_matrix(central_atom_indices, central_natoms, &
+        & atomic_charges, coordinates, natoms, nmax, cent_cutoff, cent_decay, &
+        & int_cutoff, int_decay, cm)
+      integer, intent(in) :: central_natoms
+      integer, dimension(:), intent(in) :: central_atom_indices
+      double precision, dimension(:), intent(in) :: atomic_charges
+      double precision, dimension(:,:), intent(in) :: coordinates
+      integer, intent(in) :: natoms
+      integer, intent(in) :: nmax
+      double precision, intent(inout) :: cent_cutoff, cent_decay, int_cutoff, int_decay
+      double precision, dimension(central_natoms, ((nmax + 1) * nmax) / 2), intent(out):: cm
+    end subroutine
+    subroutine fgenerate_atomic_coulomb_matrix(central_atom_indices, central_natoms, &
+        & atomic_charges, coordinates, natoms, nmax, cent_cutoff, cent_decay, &
+        & int_cutoff, int_decay, cm)
+      integer, dimension(:), intent(in) :: central_atom_indices
+      integer, intent(in) :: central_natoms
+      double precision, dimension(:), intent(in) :: atomic_charges
+      double precision, dimension(:,:), intent(in) :: coordinates
+      integer, intent(in) :: natoms
+      integer, intent(in) :: nmax
+      double precision, intent(inout) :: cent_cutoff, cent_decay, int_cutoff, int_decay
+      double precision, dimension(central_natoms, ((nmax + 1) * nmax) / 2), intent(out):: cm
+    end subroutine
+  end interface
+  allocate(coords(natoms, 3))
+  allocate(idx(ncent))
+  coords = transpose(r_t)
+  idx = cent1
+  c1 = cc
+  c2 = cd
+  c3 = ic
+  c4 = id
+  if (sorting == 0) then
+    call fgenerate_local_coulomb_matrix(idx, ncent, z, coords, natoms, nmax, c1, c2, c3, c4, cm)
+  else
+    call fgenerate_atomic_coulomb_matrix(idx, ncent, z, coords, natoms, nmax, c1, c2, c3, c4, cm)
+  end if
+  deallocate(coords)
+  deallocate(idx)
+end subroutine
+
+subroutine ref_local_kernels(kind, q1, na, q2, nb, d, n1, nm1, n2, nm2, sigmas, nsig, k) &
+    bind(C, name="ref_local_kernels")
+  ! kind: 0 Laplacian (fget_local_kernels_laplacian), 1 Gaussian (fget_local_kernels_gaussian).
+  ! q1, q2 are C-ordered (atoms, d), i.e. Fortran (d, atoms) as qml/kernels.py:256/298 pass
+  ! A.T; k is Fortran (nsig, nm1, nm2).
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: kind, na, nb, d, nm1, nm2, nsig
+  real(c_double), intent(in) :: q1(d, na), q2(d, nb), sigmas(nsig)
+  integer(c_int), intent(in) :: n1(nm1), n2(nm2)
+  real(c_double), intent(out) :: k(nsig, nm1, nm2)
+  integer, allocatable :: m1(:), m2(:)
+  interface
+    subroutine fget_local_kernels_gaussian(q1, q2, n1, n2, sigmas, nm1, nm2, nsigmas, kernels)
+      double precision, dimension(:,:), intent(in) :: q1
+      double precision, dimension(:,:), intent(in) :: q2
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:), intent(in) :: n2
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nm2
+      integer, intent(in) :: nsigmas
+      double precision, dimension(nsigmas,nm1,nm2), intent(out) :: kernels
+    end subroutine
+    subroutine fget_local_kernels_laplacian(q1, q2, n1, n2, sigmas, nm1, nm2, nsigmas, kernels)
+      double precision, dimension(:,:), intent(in) :: q1
+      double precision, dimension(:,:), intent(in) :: q2
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:), intent(in) :: n2
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nm2
+      integer, intent(in) :: nsigmas
+      double precision, dimension(nsigmas,nm1,nm2), intent(out) :: kernels
+    end subroutine
+  end interface
+  allocate(m1(nm1))
+  allocate(m2(nm2))
+  m1 = n1
+  m2 = n2
+  if (kind == 1) then
+    call fget_local_kernels_gaussian(q1, q2, m1, m2, sigmas, nm1, nm2, nsig, k)
+  else
+    call fget_local_kernels_laplacian(q1, q2, m1, m2, sigmas, nm1, nm2, nsig, k)
+  end if
+  deallocate(m1)
+  deallocate(m2)
+end subroutine

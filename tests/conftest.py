@@ -27,6 +27,24 @@ def ref_vectors():
 
 
 @pytest.fixture(scope="session")
+def acm_fixtures():
+    z = np.load(os.path.join(GOLDEN, "atomic_cm_fixtures.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def local_fixtures():
+    z = np.load(os.path.join(GOLDEN, "local_kernel_fixtures.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def ref_local_vectors():
+    z = np.load(os.path.join(GOLDEN, "ref_local_vectors.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
 def cho_pair():
     z = np.load(os.path.join(GOLDEN, "cho_solve_pair.npz"), allow_pickle=False)
     return z["K"], z["y"]
@@ -44,3 +62,11 @@ def qm7_molecules(qm7, names):
 
 def bits(x):
     return np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
+
+
+def qm7_local_split(qm7, local_fixtures):
+    """tests/test_energy_krr_atomic_cmat.py:62-98: the first 1000 QM7 names (sorted), shuffled
+    by np.random.seed(666) (the committed order), 200 training and the last 100 test."""
+    mols = qm7_molecules(qm7, qm7["krr_names"])
+    mols = [mols[i] for i in local_fixtures["order"]]
+    return mols[:200], mols[-100:]

@@ -13,6 +13,18 @@ Outputs (data only):
   ref_vectors.npz       outputs of the reference Fortran (oracle/_ref, one OpenMP thread) on
                         seeded synthetic molecules and random matrices: sorted and unsorted
                         Coulomb matrices (size 29) and the four pairwise matrices.
+  atomic_cm_fixtures.npz
+                        the reference's tests/data/atomic_coulomb_matrix_representation_*.txt
+                        (tests/test_representations.py:92-132, molecules 0101-0110, size 17),
+                        stored losslessly as float64.
+  local_kernel_fixtures.npz
+                        tests/data/K_local_laplacian.txt, Ks_local_gaussian.txt,
+                        Ks_local_laplacian.txt (tests/test_energy_krr_atomic_cmat.py:59-217;
+                        K_local_gaussian.txt is in cho_solve_pair.npz), and the molecule order
+                        that test's np.random.seed(666) shuffle of the first 1000 QM7 names gives.
+  ref_local_vectors.npz outputs of the reference Fortran (oracle/_ref) on seeded synthetic
+                        molecules: atomic Coulomb matrices (both orderings, with and without
+                        cut-offs) and local kernels (two sigmas).
 """
 
 from __future__ import annotations
@@ -77,6 +89,50 @@ def copy_reference_fixtures():
     np.savez_compressed(os.path.join(HERE, "cho_solve_pair.npz"), K=K, y=y)
 
 
+def local_fixtures():
+    data = os.path.join(REF, "tests/data")
+    names = ["distance_sorted", "distance_sorted_no_indices", "row-norm_sorted",
+             "distance_sorted_with_cutoff", "row-norm_sorted_with_cutoff"]
+    acm = {n.replace("-", "_"): np.loadtxt(os.path.join(
+        data, f"atomic_coulomb_matrix_representation_{n}.txt")) for n in names}
+    np.savez_compressed(os.path.join(HERE, "atomic_cm_fixtures.npz"), **acm)
+    np.random.seed(666)
+    order = np.arange(1000)
+    np.random.shuffle(order)
+    out = dict(order=order)
+    out["K_local_laplacian"] = np.loadtxt(os.path.join(data, "K_local_laplacian.txt"))
+    out["Ks_local_gaussian"] = np.loadtxt(os.path.join(data, "Ks_local_gaussian.txt"))
+    out["Ks_local_laplacian"] = np.loadtxt(os.path.join(data, "Ks_local_laplacian.txt"))
+    np.savez_compressed(os.path.join(HERE, "local_kernel_fixtures.npz"), **out)
+
+
+def ref_local_vectors():
+    os.environ["OMP_NUM_THREADS"] = "1"
+    sys.path.insert(0, ROOT)
+    import oracle
+    from qml_amd import synthetic
+
+    mols = synthetic.generate(24, 7, 3, 23)
+    Z, R, off = mols.ragged()
+    out = dict(Z=Z, R=R, offsets=off)
+    cut = dict(central_cutoff=3.0, central_decay=0.6, interaction_cutoff=4.0,
+               interaction_decay=1.2)
+    for sorting in ("row-norm", "distance"):
+        tag = sorting.replace("-", "_")
+        out[f"acm_{tag}"] = np.concatenate([oracle.ref_atomic_coulomb_matrix(
+            *mols.molecule(i), 23, sorting) for i in range(mols.nmol)])
+        out[f"acm_{tag}_cut"] = np.concatenate([oracle.ref_atomic_coulomb_matrix(
+            *mols.molecule(i), 23, sorting, **cut) for i in range(mols.nmol)])
+    X = out["acm_row_norm"]
+    na = np.diff(off)
+    sig = np.array([30.0, 400.0])
+    out["sigmas"] = sig
+    XA = X[:off[10]]
+    out["K_local_gaussian"] = oracle.ref_local_kernels("gaussian", XA, X, na[:10], na, sig)
+    out["K_local_laplacian"] = oracle.ref_local_kernels("laplacian", XA, X, na[:10], na, sig)
+    np.savez_compressed(os.path.join(HERE, "ref_local_vectors.npz"), **out)
+
+
 def ref_vectors():
     os.environ["OMP_NUM_THREADS"] = "1"
     sys.path.insert(0, ROOT)
@@ -105,4 +161,6 @@ if __name__ == "__main__":
     qm7_subset()
     copy_reference_fixtures()
     ref_vectors()
+    local_fixtures()
+    ref_local_vectors()
     print("golden fixtures written to", HERE)
