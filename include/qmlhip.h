@@ -62,6 +62,31 @@ int qmlh_cm_batch(const double *Z, const double *R, const int64_t *offsets, int6
                   qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
+ * Atomic (local) Coulomb matrices: one row per central atom.
+ * Replaces fgenerate_local_coulomb_matrix (qml/frepresentations.f90:207-413, sorting
+ * QMLH_ACM_ROW_NORM) and fgenerate_atomic_coulomb_matrix (qml/frepresentations.f90:415-598,
+ * QMLH_ACM_DISTANCE), which qml/representations.py:223-231 calls once per molecule with that
+ * molecule's central atom indices; here one call covers central atoms of many molecules.
+ * Z, R, offsets: as qmlh_cm_batch. central: (nrows,) int64 index of the central atom within its
+ * molecule (0-based); cmol: (nrows,) int64 molecule of each row. out: (nrows, ldo) row-major,
+ * row r = the packed lower triangle of central atom central[r] of molecule cmol[r]; columns
+ * beyond size*(size+1)/2 are zero. The four cut-off arguments mean what the reference's do and
+ * are normalised the same way (negative central cut-off: none; negative decays: zero; ...).
+ * Molecules may hold up to QMLH_ACM_MAX_ATOMS atoms. status (device int32, may be NULL): bit 1
+ * when a central atom has more atoms within the central cut-off than `size` rows (the
+ * reference stops the program there), bit 2 when a molecule exceeds QMLH_ACM_MAX_ATOMS, bit 4
+ * when a central or molecule index is out of range; such rows are written as zeros.
+ * ---------------------------------------------------------------------------------------- */
+#define QMLH_ACM_MAX_ATOMS 256
+enum { QMLH_ACM_ROW_NORM = 0, QMLH_ACM_DISTANCE = 1 };
+int qmlh_atomic_cm_batch(const double *Z, const double *R, const int64_t *offsets, int64_t nmol,
+                         const int64_t *central, const int64_t *cmol, int64_t nrows,
+                         int32_t size, int32_t sorting, double central_cutoff,
+                         double central_decay, double interaction_cutoff,
+                         double interaction_decay, double *out, int64_t ldo, int32_t *status,
+                         qmlh_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------- *
  * Pairwise kernels and distances, VALU tile kernels (direct differences).
  * kind QMLH_PAIR_LAPLACIAN: K(i,j) = exp((-1/sigma) * sum_k |A_ik - B_jk|)
  *      replaces flaplacian_kernel (qml/fkernels.f90:374-400) behind qml/kernels.py:36-63.
@@ -258,6 +283,42 @@ int qmlh_krr_batched_factor(const qmlh_batched_plan *plan, int32_t p, int64_t *o
 void qmlh_krr_batched_destroy(qmlh_batched_plan *plan);
 
 /* ---------------------------------------------------------------------------------------- *
+ * Local (atom-summed) kernels between molecules, several sigmas per call:
+ *   K_s(a, b) = sum over atoms i of molecule a, j of molecule b of exp(inv_s * D(A_i, B_j))
+ * Replaces fget_local_kernels_gaussian / fget_local_kernels_laplacian (qml/fkernels.f90:23-196)
+ * behind qml/kernels.py:218-298, and fget_vector_kernels_gaussian / _laplacian
+ * (qml/fkernels.f90:198-337) behind qml/wrappers.py:33-97, which give the same values from
+ * zero-padded per-molecule arrays.
+ * kind QMLH_PAIR_GAUSSIAN: D = squared L2 distance, inv_s = -0.5/sigma_s^2. With normsA and
+ *      normsB (|x|^2 per atom row, qmlh_row_sqnorms) the distance is the expanded form on FP64
+ *      MFMA; with both NULL, direct differences on the VALU.
+ * kind QMLH_PAIR_LAPLACIAN: D = L1 distance, inv_s = -1/sigma_s (VALU, direct differences).
+ * A: (atomsA, lda) row-major atom representations; offsA: (nmA + 1) int64 atom offsets of A's
+ * molecules; panelsA: (npA + 1) int64 panel starts from qmlh_local_panels (copied to the
+ * device). Same for B. sigmas_h: host array of nsig > 0 values. K_s(a, b) is written at
+ * K[a + b*ldk + s*sstride] (column-major per sigma, the reference's kernels(s, :, :)); every
+ * entry is written. flags QMLH_SYMMETRIC: A and B are the same molecules (same pointers and
+ * panels); only panel pairs P <= Q are computed and mirrored, so every K_s is exactly
+ * symmetric.
+ * ---------------------------------------------------------------------------------------- */
+#define QMLH_LOCAL_PANEL_ATOMS 128
+#define QMLH_LOCAL_PANEL_MOLS 32
+int qmlh_local_kernels(int32_t kind, const double *A, int64_t atomsA, int64_t lda,
+                       const double *normsA, const int64_t *offsA, const int64_t *panelsA,
+                       int64_t npA, const double *B, int64_t atomsB, int64_t ldb,
+                       const double *normsB, const int64_t *offsB, const int64_t *panelsB,
+                       int64_t npB, int64_t d, const double *sigmas_h, int32_t nsig, double *K,
+                       int64_t ldk, int64_t sstride, int32_t flags, qmlh_stream_t stream);
+
+/* Panel partition for qmlh_local_kernels (host only, no device work; no counterpart in the
+ * reference, whose loops run molecule pair by molecule pair): consecutive molecules are grouped
+ * into panels of at most QMLH_LOCAL_PANEL_ATOMS atoms and QMLH_LOCAL_PANEL_MOLS molecules; a
+ * molecule with more atoms than QMLH_LOCAL_PANEL_ATOMS is a panel alone. offsets_h: (nmol + 1)
+ * host int64 atom offsets; panels_h: (nmol + 1) host int64 output, panels_h[p] = first molecule
+ * of panel p, panels_h[np] = nmol. Returns np >= 0, or an error status < 0. */
+int64_t qmlh_local_panels(const int64_t *offsets_h, int64_t nmol, int64_t *panels_h);
+
+/* ---------------------------------------------------------------------------------------- *
  * Measurement hooks (no counterpart in the reference; used by bench.py). When enabled, each
  * heavy launch is bracketed by HIP events on its stream and its algorithmic work is added to
  * its class: flops for the MFMA classes and the Cholesky, pair-features for the VALU pairwise
@@ -272,7 +333,8 @@ enum {
     QMLH_TIMER_POTRF = 5,     /* whole factorisation                      n^3/3 flops */
     QMLH_TIMER_POTRS = 6,     /* both triangular solves                     2 n^2    */
     QMLH_TIMER_CM = 7,        /* Coulomb matrices                           molecules */
-    QMLH_TIMER_CLASSES = 8
+    QMLH_TIMER_LOCAL = 8,     /* local kernels                  atom-pair features    */
+    QMLH_TIMER_CLASSES = 9
 };
 int qmlh_timer_enable(int32_t on);
 int qmlh_timer_read(int32_t timer_class, double *total_ms_h, int64_t *launches_h,

@@ -19,6 +19,8 @@ QMLH_OK = 0
 PAIR_LAPLACIAN, PAIR_GAUSSIAN, PAIR_MANHATTAN, PAIR_L2 = 0, 1, 2, 3
 SYMMETRIC = 1
 CM_MAX_SIZE = 256
+ACM_MAX_ATOMS = 256
+ACM_ROW_NORM, ACM_DISTANCE = 0, 1
 
 _lock = threading.Lock()
 _lib = None
@@ -50,6 +52,12 @@ _SIGNATURES = {
     "qmlh_last_error": (ctypes.c_char_p, []),
     "qmlh_init": (ctypes.c_int, []),
     "qmlh_cm_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _i64, _vp, _vp]),
+    "qmlh_atomic_cm_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32,
+                                            _f64, _f64, _f64, _f64, _vp, _i64, _vp, _vp]),
+    "qmlh_local_kernels": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _i64,
+                                          _i64, _vp, _vp, _vp, _i64, _i64, _vp, _i32, _vp, _i64,
+                                          _i64, _i32, _vp]),
+    "qmlh_local_panels": (_i64, [_vp, _i64, _vp]),
     "qmlh_pairwise": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _f64, _vp,
                                      _i64, _i32, _vp]),
     "qmlh_row_sqnorms": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp]),
@@ -97,7 +105,7 @@ _SIGNATURES = {
 }
 
 TIMER_CLASSES = {"syrk": 0, "gemm": 1, "gauss": 2, "pair": 3, "predict": 4, "potrf": 5,
-                 "potrs": 6, "cm": 7}
+                 "potrs": 6, "cm": 7, "local": 8}
 
 
 def timer_enable(on: bool = True) -> None:

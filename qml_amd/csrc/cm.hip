@@ -197,3 +197,276 @@ extern "C" int qmlh_cm_batch(const double* Z, const double* R, const int64_t* of
     QMLH_CHECK_LAUNCH("cm_batch launch");
     return QMLH_OK;
 }
+
+// ------------------------------------------------------------------------------------------
+// Atomic (local) Coulomb matrices: one 64-lane wavefront per output row (central atom).
+//
+// Reproduces fgenerate_local_coulomb_matrix (qml/frepresentations.f90:207-413, row-norm
+// ordering) and fgenerate_atomic_coulomb_matrix (qml/frepresentations.f90:415-598, distance
+// ordering) for one central atom k of one molecule:
+//  * distances |R_b - R_a| for a < b summed x, y, z in order (lines 279-285), so the matrix
+//    of distances is exactly symmetric;
+//  * cutoff_count(k) = 1 + #{x != k : d(x, k) < central cut-off} rows, an error beyond `size`;
+//  * row-norm ordering: M_ij(k) for i < j is ((f_ik Z_i) Z_j / d_ij) times the interaction
+//    decay and then the central decay of j, each a separate rounding (lines 311-351); the
+//    diagonal is ((f_ik f_ik) 0.5) Z_i^2.4; row norms are sums of squares in ascending j (the
+//    one-thread order of the reduction at lines 311-351); the central atom is picked first,
+//    then repeated maxloc with the picked entry set to zero (lines 355-373);
+//  * distance ordering: repeated minloc over d(., k), picked entries set to huge (lines
+//    535-546); entry (m, c) = (f_{s_m k} P(s_m, s_c)) times the central decay of s_c, with P
+//    the interaction-cut pair matrix (lines 510-529 and 556-586);
+//  * decay factor 0.5 (cos(pi ((r - c) + w) / w) + 1) with the device cos (libm's and the
+//    device's cos may differ in the last place; every other operation is IEEE-exact and
+//    matches the Fortran bit for bit).
+// Atoms of the molecule are staged in LDS; a lane owns atoms lane, lane + 64, ...
+// ------------------------------------------------------------------------------------------
+namespace qmlh {
+
+static constexpr int ACM_MAX_ATOMS = QMLH_ACM_MAX_ATOMS;
+static constexpr int ACM_PER_LANE = ACM_MAX_ATOMS / CM_WAVE;
+
+struct AcmParams {
+    double cc, cd, ic, id;  // normalised central / interaction cut-off and decay
+};
+
+__device__ __forceinline__ double acm_decay(double r, double c, double w) {
+    const double pi = 3.141592653589793;  // 4 atan(1) in FP64
+    return 0.5 * (cos(pi * (r - c + w) / w) + 1.0);
+}
+
+// |R_b - R_a| with the smaller index first.
+__device__ __forceinline__ double acm_dist(const double* sR, int a, int b) {
+    if (a == b) return 0.0;
+    int lo = a < b ? a : b, hi = a < b ? b : a;
+    double dx = sR[3 * hi + 0] - sR[3 * lo + 0];
+    double dy = sR[3 * hi + 1] - sR[3 * lo + 1];
+    double dz = sR[3 * hi + 2] - sR[3 * lo + 2];
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return sqrt(s);
+}
+
+// Row-norm ordering's matrix entry M(a, b) for central atom with distances sDk.
+__device__ __forceinline__ double acm_rn_entry(const double* sR, const double* sZ,
+                                               const double* sDk, int a, int b,
+                                               const AcmParams& p) {
+    int i = a < b ? a : b, j = a < b ? b : a;
+    double di = sDk[i];
+    if (di > p.cc) return 0.0;
+    double pref = 1.0;
+    if (di > p.cc - p.cd) pref = acm_decay(di, p.cc, p.cd);
+    if (i == j) return pref * pref * 0.5 * (2.0 * cm_diag(sZ[i]));
+    double dj = sDk[j];
+    if (dj > p.cc) return 0.0;
+    double dij = acm_dist(sR, i, j);
+    if (dij > p.ic) return 0.0;
+    double v = pref * sZ[i] * sZ[j] / dij;
+    if (dij > p.ic - p.id) v = v * 0.5 * (cos(3.141592653589793 * (dij - p.ic + p.id) / p.id) + 1);
+    if (dj > p.cc - p.cd) v = v * 0.5 * (cos(3.141592653589793 * (dj - p.cc + p.cd) / p.cd) + 1);
+    return v;
+}
+
+// Distance ordering's interaction-cut pair matrix P(a, b).
+__device__ __forceinline__ double acm_pair(const double* sR, const double* sZ, int a, int b,
+                                           const AcmParams& p) {
+    if (a == b) return cm_diag(sZ[a]);
+    int i = a < b ? a : b, j = a < b ? b : a;
+    double dij = acm_dist(sR, i, j);
+    if (dij > p.ic) return 0.0;
+    double v = sZ[i] * sZ[j] / dij;
+    if (dij > p.ic - p.id) v = v * 0.5 * (cos(3.141592653589793 * (dij - p.ic + p.id) / p.id) + 1);
+    return v;
+}
+
+// status bits: 1 a central atom needs more than `size` rows; 2 a molecule has more than
+// QMLH_ACM_MAX_ATOMS atoms; 4 a central atom or molecule index is out of range. The row of
+// such a case is written as zeros.
+template <bool ROWNORM>
+__global__ __launch_bounds__(CM_WAVE) void atomic_cm_kernel(
+    const double* __restrict__ Z, const double* __restrict__ R,
+    const int64_t* __restrict__ offsets, int64_t nmol, const int64_t* __restrict__ central,
+    const int64_t* __restrict__ cmol, int64_t nrows, int size, AcmParams p, int64_t ldo,
+    double* __restrict__ out, int32_t* __restrict__ status) {
+    __shared__ double sR[3 * ACM_MAX_ATOMS];
+    __shared__ double sZ[ACM_MAX_ATOMS];
+    __shared__ double sDk[ACM_MAX_ATOMS];
+    __shared__ int sOrder[ACM_MAX_ATOMS];
+
+    const int lane = threadIdx.x;
+    const int d = size * (size + 1) / 2;
+
+    for (int64_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        double* dst = out + row * ldo;
+        const int64_t mol = cmol[row];
+        int bad = 0;
+        int64_t a0 = 0, n64 = 0, k64 = central[row];
+        if (mol < 0 || mol >= nmol) {
+            bad = 4;
+        } else {
+            a0 = offsets[mol];
+            n64 = offsets[mol + 1] - a0;
+            if (n64 > ACM_MAX_ATOMS || n64 < 0) bad = 2;
+            else if (k64 < 0 || k64 >= n64) bad = 4;
+        }
+        if (bad) {
+            for (int64_t q = lane; q < ldo; q += CM_WAVE) dst[q] = 0.0;
+            if (lane == 0 && status) atomicOr(status, bad);
+            continue;
+        }
+        const int n = (int)n64, k = (int)k64;
+        for (int a = lane; a < n; a += CM_WAVE) {
+            sR[3 * a + 0] = R[3 * (a0 + a) + 0];
+            sR[3 * a + 1] = R[3 * (a0 + a) + 1];
+            sR[3 * a + 2] = R[3 * (a0 + a) + 2];
+            sZ[a] = Z[a0 + a];
+        }
+        __syncthreads();
+        int cnt_lane = 0;
+        for (int a = lane; a < n; a += CM_WAVE) {
+            double dk = acm_dist(sR, a, k);
+            sDk[a] = dk;
+            if (a != k && dk < p.cc) ++cnt_lane;
+        }
+#pragma unroll
+        for (int off = CM_WAVE / 2; off > 0; off >>= 1) cnt_lane += __shfl_xor(cnt_lane, off, CM_WAVE);
+        const int cnt = 1 + cnt_lane;
+        __syncthreads();
+        if (cnt > size) {
+            for (int64_t q = lane; q < ldo; q += CM_WAVE) dst[q] = 0.0;
+            if (lane == 0 && status) atomicOr(status, 1);
+            __syncthreads();
+            continue;
+        }
+
+        // Ordering key per owned atom: row norm (picked -> 0, maximum first) or distance
+        // (picked -> huge, minimum first).
+        double key[ACM_PER_LANE];
+#pragma unroll
+        for (int q = 0; q < ACM_PER_LANE; ++q) {
+            const int x = lane + q * CM_WAVE;
+            double v = 0.0;
+            if (x < n) {
+                if (ROWNORM) {
+                    if (x == k) {
+                        v = DBL_MAX;
+                    } else {
+                        double acc = 0.0;
+                        for (int j = 0; j < n; ++j) {
+                            double e = acm_rn_entry(sR, sZ, sDk, x, j, p);
+                            double e2 = e * e;
+                            acc = acc + e2;
+                        }
+                        v = acc;
+                    }
+                } else {
+                    v = sDk[x];
+                }
+            }
+            key[q] = v;
+        }
+        for (int it = 0; it < cnt; ++it) {
+            double bv = ROWNORM ? -1.0 : DBL_MAX;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int q = 0; q < ACM_PER_LANE; ++q) {
+                const int x = lane + q * CM_WAVE;
+                if (x < n) {
+                    bool better = ROWNORM ? (key[q] > bv || (key[q] == bv && x < bi))
+                                          : (key[q] < bv || (key[q] == bv && x < bi));
+                    if (better) {
+                        bv = key[q];
+                        bi = x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = CM_WAVE / 2; off > 0; off >>= 1) {
+                double ov = __shfl_xor(bv, off, CM_WAVE);
+                int oi = __shfl_xor(bi, off, CM_WAVE);
+                bool better = ROWNORM ? (ov > bv || (ov == bv && oi < bi))
+                                      : (ov < bv || (ov == bv && oi < bi));
+                if (better) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (bi >= n) bi = 0;  // only with NaN keys (NaN coordinates): keep indices valid
+            if (lane == 0) sOrder[it] = bi;
+#pragma unroll
+            for (int q = 0; q < ACM_PER_LANE; ++q)
+                if (lane + q * CM_WAVE == bi) key[q] = ROWNORM ? 0.0 : DBL_MAX;
+        }
+        __syncthreads();
+
+        for (int64_t q = lane; q < ldo; q += CM_WAVE) {
+            double v = 0.0;
+            if (q < d) {
+                int m, c;
+                cm_unpack((int)q, m, c);
+                if (m < cnt) {
+                    const int i = sOrder[m], j = sOrder[c];
+                    if (ROWNORM) {
+                        v = acm_rn_entry(sR, sZ, sDk, i, j, p);
+                    } else {
+                        const double di = sDk[i];
+                        if (!(di > p.cc)) {
+                            double pref = 1.0;
+                            if (di > p.cc - p.cd) pref = acm_decay(di, p.cc, p.cd);
+                            v = pref * acm_pair(sR, sZ, i, j, p);
+                            const double dj = sDk[j];
+                            if (dj > p.cc - p.cd)
+                                v = v * 0.5 * (cos(3.141592653589793 * (dj - p.cc + p.cd) / p.cd) + 1);
+                        }
+                    }
+                }
+            }
+            dst[q] = v;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace qmlh
+
+extern "C" int qmlh_atomic_cm_batch(const double* Z, const double* R, const int64_t* offsets,
+                                    int64_t nmol, const int64_t* central, const int64_t* cmol,
+                                    int64_t nrows, int32_t size, int32_t sorting,
+                                    double central_cutoff, double central_decay,
+                                    double interaction_cutoff, double interaction_decay,
+                                    double* out, int64_t ldo, int32_t* status,
+                                    qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (nrows < 0 || nmol < 0) return set_error(QMLH_ERR_ARG, "atomic_cm_batch: negative count");
+    if (size < 1 || size > QMLH_CM_MAX_SIZE)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_batch: size %d outside [1, %d]", size,
+                         QMLH_CM_MAX_SIZE);
+    if (sorting != QMLH_ACM_ROW_NORM && sorting != QMLH_ACM_DISTANCE)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_batch: unknown sorting %d", sorting);
+    int64_t d = (int64_t)size * (size + 1) / 2;
+    if (ldo < d) return set_error(QMLH_ERR_ARG, "atomic_cm_batch: ldo %lld < %lld",
+                                  (long long)ldo, (long long)d);
+    if (nrows == 0) return QMLH_OK;
+    if (!Z || !R || !offsets || !central || !cmol || !out)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_batch: null pointer");
+    // Cut-off normalisation of qml/frepresentations.f90:254-273 (identical at 460-479).
+    AcmParams p{central_cutoff, central_decay, interaction_cutoff, interaction_decay};
+    if (p.cc < 0) p.cc = DBL_MAX;
+    if (p.ic < 0 || p.ic > 2 * p.cc) p.ic = 2 * p.cc;
+    if (p.cd < 0) p.cd = 0.0;
+    else if (p.cd > p.cc) p.cd = p.cc;
+    if (p.id < 0) p.id = 0.0;
+    else if (p.id > p.ic) p.id = p.ic;
+    int64_t grid = nrows < 262144 ? nrows : 262144;
+    hipStream_t s = (hipStream_t)stream;
+    TimerScope ts(QMLH_TIMER_CM, (double)nrows, s);
+    if (sorting == QMLH_ACM_ROW_NORM)
+        hipLaunchKernelGGL(atomic_cm_kernel<true>, dim3((unsigned)grid), dim3(CM_WAVE), 0, s, Z,
+                           R, offsets, nmol, central, cmol, nrows, (int)size, p, ldo, out, status);
+    else
+        hipLaunchKernelGGL(atomic_cm_kernel<false>, dim3((unsigned)grid), dim3(CM_WAVE), 0, s,
+                           Z, R, offsets, nmol, central, cmol, nrows, (int)size, p, ldo, out,
+                           status);
+    QMLH_CHECK_LAUNCH("atomic_cm_batch launch");
+    return QMLH_OK;
+}

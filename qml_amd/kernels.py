@@ -111,3 +111,127 @@ def gaussian_kernel(A, B, sigma):
     else:
         Kd = pairwise_device(_lib.PAIR_GAUSSIAN, Ad, na, Bd, nb, d, sigma, symmetric)
     return _to_host_fortran(Kd)
+
+
+# --------------------------------------------------------------------------------------------
+# Local (atom-summed) kernels: qml/kernels.py:218-298
+# --------------------------------------------------------------------------------------------
+
+def local_panels(offsets) -> np.ndarray:
+    """Host panel partition of a ragged molecule list (see qmlh_local_panels)."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nmol = off.shape[0] - 1
+    panels = np.zeros(nmol + 1, dtype=np.int64)
+    np_ = _lib.load_library().qmlh_local_panels(off.ctypes.data, nmol, panels.ctypes.data)
+    if np_ < 0:
+        _lib.check(int(np_))
+    return panels[:np_ + 1].copy()
+
+
+class LocalSet:
+    """Atom representations of a molecule list resident in HBM, with what the local-kernel
+    launch needs: rows (atoms, ld) FP64, atom offsets and panels on the device, and (for the
+    Gaussian expanded form) the squared row norms."""
+
+    def __init__(self, X, n_atoms, ld: int | None = None):
+        import torch
+
+        counts = np.asarray(n_atoms, dtype=np.int64).reshape(-1)
+        self.nmol = counts.shape[0]
+        self.offsets = np.zeros(self.nmol + 1, dtype=np.int64)
+        np.cumsum(counts, out=self.offsets[1:])
+        self.atoms = int(self.offsets[-1])
+        if isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float64:
+            # Resident rows (e.g. generate_atomic_coulomb_matrices): zero pad columns included.
+            self.X, self.d = X, X.shape[1]
+        else:
+            Xh = np.asarray(X, dtype=np.float64).reshape(self.atoms, -1)
+            self.d = Xh.shape[1]
+            self.X = rows_to_device(Xh, ld)
+        self.panels = local_panels(self.offsets)
+        dev = self.X.device
+        self.offsets_d = torch.from_numpy(self.offsets).to(dev)
+        self.panels_d = torch.from_numpy(self.panels).to(dev)
+        self._norms = None
+        self._maxnorm = None
+
+    @property
+    def npanels(self) -> int:
+        return self.panels.shape[0] - 1
+
+    def norms(self):
+        if self._norms is None:
+            self._norms = row_sqnorms_device(self.X, self.atoms, self.d) if self.atoms else \
+                self.X.new_zeros(1)
+        return self._norms
+
+    def maxnorm(self) -> float:
+        if self._maxnorm is None:
+            self._maxnorm = float(self.norms().max().item()) if self.atoms else 0.0
+        return self._maxnorm
+
+
+def local_kernels_device(kind: int, SA: LocalSet, SB: LocalSet, sigmas, symmetric=False,
+                         expanded: bool | None = None, out=None, stream=None):
+    """Local kernels between two resident molecule sets; returns the (nsig, nmB, nmA) device
+    tensor, i.e. K_s(a, b) at [s, b, a] (column-major per sigma).
+
+    kind: _lib.PAIR_GAUSSIAN or _lib.PAIR_LAPLACIAN. ``expanded`` (Gaussian only): FP64 MFMA
+    expanded distance (True), direct differences (False), or chosen by the error bound of
+    ``expansion_ok`` (None)."""
+    import torch
+
+    sig = np.ascontiguousarray(np.asarray(sigmas, dtype=np.float64).reshape(-1))
+    if np.any(~(sig > 0)):
+        raise ValueError("sigmas must be positive")
+    if SA.d != SB.d:
+        raise ValueError("expected representations of the same size")
+    K = out if out is not None else torch.empty((sig.size, SB.nmol, SA.nmol),
+                                                dtype=torch.float64, device=SA.X.device)
+    nA = nB = None
+    if kind == _lib.PAIR_GAUSSIAN:
+        if expanded is None:
+            expanded = all(expansion_ok(SA.maxnorm(), SB.maxnorm(), SA.d, s) for s in sig)
+        if expanded:
+            nA, nB = SA.norms(), SB.norms()
+    _lib.check(_lib.lib().qmlh_local_kernels(
+        kind, _lib.ptr(SA.X), SA.atoms, SA.X.shape[1], _lib.ptr(nA) if nA is not None else None,
+        _lib.ptr(SA.offsets_d), _lib.ptr(SA.panels_d), SA.npanels,
+        _lib.ptr(SB.X), SB.atoms, SB.X.shape[1], _lib.ptr(nB) if nB is not None else None,
+        _lib.ptr(SB.offsets_d), _lib.ptr(SB.panels_d), SB.npanels, SA.d,
+        sig.ctypes.data, sig.size, _lib.ptr(K), SA.nmol, SA.nmol * SB.nmol,
+        _lib.SYMMETRIC if symmetric else 0, _lib.stream_handle(stream)))
+    return K
+
+
+def _local_kernels(kind, A, B, na, nb, sigmas):
+    assert np.sum(na) == A.shape[0], "Error in A input"
+    assert np.sum(nb) == B.shape[0], "Error in B input"
+    assert A.shape[1] == B.shape[1], "Error in representation sizes"
+    sig = np.asarray(sigmas, dtype=np.float64).reshape(-1)
+    symmetric = A is B and np.array_equal(np.asarray(na), np.asarray(nb))
+    nma, nmb = len(na), len(nb)
+    if nma == 0 or nmb == 0 or sig.size == 0:
+        return np.zeros((sig.size, nma, nmb), order="F")
+    SA = LocalSet(A, na)
+    SB = SA if symmetric else LocalSet(B, nb)
+    Kd = local_kernels_device(kind, SA, SB, sig, symmetric)
+    # (nsig, nmB, nmA) row-major == (nsig, nmA, nmB) with each K_s column-major.
+    return np.transpose(Kd.cpu().numpy(), (0, 2, 1))
+
+
+def get_local_kernels_gaussian(A, B, na, nb, sigmas):
+    """K_s(i, j) = sum_{a in i} sum_{b in j} exp(-|A_a - B_b|_2^2 / (2 sigma_s^2)).
+
+    Same signature, assertions and output shape (nsigmas, N, M) as qml/kernels.py:218-256;
+    A and B are all atoms' rows concatenated, na and nb the atom counts per molecule. The
+    distances come from the FP64 MFMA expanded form where its error bound allows, else from
+    direct differences."""
+    return _local_kernels(_lib.PAIR_GAUSSIAN, A, B, na, nb, sigmas)
+
+
+def get_local_kernels_laplacian(A, B, na, nb, sigmas):
+    """K_s(i, j) = sum_{a in i} sum_{b in j} exp(-|A_a - B_b|_1 / sigma_s).
+
+    Same signature, assertions and output shape (nsigmas, N, M) as qml/kernels.py:258-298."""
+    return _local_kernels(_lib.PAIR_LAPLACIAN, A, B, na, nb, sigmas)
