@@ -64,6 +64,12 @@ def parse():
                    help="run a device copy of this many MiB first (HBM counter calibration)")
     p.add_argument("--kernels", type=int, default=1,
                    help="time the kernel-matrix builds at the large shapes (0 = skip)")
+    p.add_argument("--local", type=int, default=20000,
+                   help="molecules of the local-kernel section (0: skip)")
+    p.add_argument("--local-only", action="store_true",
+                   help="run only the local-kernel section and print its JSON (profiling)")
+    p.add_argument("--local-cpu-sample", type=int, default=2000,
+                   help="molecules of the local-kernel CPU baseline sample (0: skip)")
     p.add_argument("--sweep", type=int, default=64,
                    help="problems in the batched learning-curve sweep (0 = skip)")
     return p.parse_args()
@@ -183,6 +189,96 @@ def run_kernels(log):
     return out
 
 
+def run_local(n_mol, cpu_sample, log):
+    """Local (atom-summed) kernels at scale: n_mol synthetic molecules (5-23 atoms, the QM7-like
+    generator), atomic Coulomb matrices (row-norm, size 23, d = 276) generated on the GPU, then
+    the symmetric n_mol x n_mol local Gaussian (sigma = 724, FP64 MFMA expanded distances) and
+    Laplacian (sigma = 10^3.6, FP64 VALU) kernels. Algorithmic work counts each atom pair of
+    the symmetric problem once: pairs = atoms (atoms + 1) / 2, times d features (2 flops per
+    pair-feature for the Gaussian cross term). The CPU baseline runs the reference Fortran
+    (oracle/_ref, else the C restatement) on a bounded sample of cpu_sample molecules; the
+    reference computes every molecule pair, so its time scales as n_mol^2."""
+    import numpy as np
+    import torch
+
+    from qml_amd import _lib, synthetic
+    from qml_amd.kernels import LocalSet, local_kernels_device
+    from qml_amd.representations import generate_atomic_coulomb_matrices
+
+    mols = synthetic.generate(n_mol, 2024, 5, 23)
+    Z, R, off = mols.ragged()
+    atoms = int(off[-1])
+    d = 276
+    out = {"workload": f"local_krr_{n_mol // 1000}k: {n_mol} molecules ({atoms} atoms, 5-23 per "
+                       "molecule), atomic Coulomb matrices row-norm size 23 (d=276), symmetric "
+                       "local kernels, Gaussian sigma=724 and Laplacian sigma=10^3.6"}
+    generate_atomic_coulomb_matrices(Z, R, off, 23, "row-norm")
+    torch.cuda.synchronize()
+    _lib.timer_enable(True)
+    X = generate_atomic_coulomb_matrices(Z, R, off, 23, "row-norm")
+    torch.cuda.synchronize()
+    ms, n, _ = _lib.timer_read("cm")
+    _lib.timer_enable(False)
+    out["atomic_cm"] = {"rows": atoms, "ms": ms / n, "rows_per_s": atoms / (ms / n * 1e-3)}
+    S = LocalSet(X, np.diff(off))
+    K = torch.empty((1, n_mol, n_mol), dtype=torch.float64, device=X.device)
+    pairs = atoms * (atoms + 1) / 2.0
+    for kind, name, sigma in ((_lib.PAIR_GAUSSIAN, "gaussian", 724.0),
+                              (_lib.PAIR_LAPLACIAN, "laplacian", 10 ** 3.6)):
+        local_kernels_device(kind, S, S, [sigma], symmetric=True, out=K)
+        torch.cuda.synchronize()
+        _lib.timer_enable(True)
+        reps = 2
+        for _ in range(reps):
+            local_kernels_device(kind, S, S, [sigma], symmetric=True, out=K)
+        torch.cuda.synchronize()
+        ms, n, work = _lib.timer_read("local")
+        _lib.timer_enable(False)
+        ms /= n
+        pf = pairs * d / (ms * 1e-3)
+        if kind == _lib.PAIR_GAUSSIAN:
+            tf = 2.0 * pf / 1e12
+            out[name] = {"ms": ms, "tflops": tf, "bound": "mfma", "peak_tflops": PEAK_TFLOPS["fp64"],
+                         "frac": tf / PEAK_TFLOPS["fp64"],
+                         "kernel": "local_kernels_kernel<LK_GAUSS_MFMA>"}
+        else:
+            out[name] = {"ms": ms, "pair_features_per_s": pf, "bound": "valu",
+                         "frac": pf / 19.65e12, "kernel": "local_kernels_kernel<LK_LAPLACE>"}
+        log(f"local {name}: {ms:.1f} ms, {out[name]['frac']:.2f} of peak")
+    out["panels"] = S.npanels
+    out["tile_fill"] = atoms / (S.npanels * 128.0)
+    del K, S, X
+    torch.cuda.empty_cache()
+    if cpu_sample > 0:
+        import oracle
+
+        use_ref = oracle.ref_available()
+        cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+        sub = mols.subset(0, cpu_sample)
+        Zs, Rs, offs = sub.ragged()
+        acm = oracle.ref_atomic_coulomb_matrix if use_ref else oracle.atomic_coulomb_matrix
+        t0 = time.perf_counter()
+        Xs = np.concatenate([acm(*sub.molecule(m), 23, "row-norm") for m in range(sub.nmol)])
+        t1 = time.perf_counter()
+        lk = oracle.ref_local_kernels if use_ref else oracle.local_kernels
+        ns = np.diff(offs)
+        lk("gaussian", Xs, Xs, ns, ns, [724.0])
+        t2 = time.perf_counter()
+        lk("laplacian", Xs, Xs, ns, ns, [10 ** 3.6])
+        t3 = time.perf_counter()
+        r = n_mol / cpu_sample
+        out["cpu_baseline"] = {
+            "value": (t1 - t0) * r + (t2 - t1) * r * r, "unit": "s", "cores": cores,
+            "kind": "reference" if use_ref else "port",
+            "laplacian_s": (t3 - t2) * r * r,
+            "sample": (f"{cpu_sample} molecules on {cores} host threads: atomic CM "
+                       f"{t1 - t0:.2f}s (one molecule per call), Gaussian local kernel "
+                       f"{t2 - t1:.2f}s, Laplacian {t3 - t2:.2f}s (all molecule pairs, as the "
+                       f"reference computes them); extrapolated to {n_mol} molecules as n, n^2")}
+        log(f"local cpu baseline: gaussian {out['cpu_baseline']['value']:.0f}s extrapolated")
+    return out
+
+
 def run_sweep(count, log):
     """64 Laplacian KRR fits, n log-uniform in [500, 8000], (sigma, lambda) grid, size-23
     Coulomb matrices: one graph launch vs the same fits solved one after another."""
@@ -273,6 +369,10 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: 3 x {args.copy_mib} MiB elementwise read + write")
         del src, dst
+    if args.local_only:
+        print(json.dumps({"local": run_local(args.local, args.local_cpu_sample, log)}),
+              flush=True)
+        return
     N = args.n_train
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)
     d = cm_length(size)
@@ -372,6 +472,7 @@ def main():
 
     sweep_out = run_sweep(args.sweep, log) if args.sweep > 0 else None
     kernels_out = run_kernels(log) if args.kernels else None
+    local_out = run_local(args.local, args.local_cpu_sample, log) if args.local > 0 else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -406,6 +507,7 @@ def main():
         "serve": serve_out or None,
         "sweep": sweep_out,
         "kernels": kernels_out,
+        "local": local_out,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),

@@ -302,7 +302,7 @@ void qmlh_krr_batched_destroy(qmlh_batched_plan *plan);
  * symmetric.
  * ---------------------------------------------------------------------------------------- */
 #define QMLH_LOCAL_PANEL_ATOMS 128
-#define QMLH_LOCAL_PANEL_MOLS 32
+#define QMLH_LOCAL_PANEL_MOLS 16
 int qmlh_local_kernels(int32_t kind, const double *A, int64_t atomsA, int64_t lda,
                        const double *normsA, const int64_t *offsA, const int64_t *panelsA,
                        int64_t npA, const double *B, int64_t atomsB, int64_t ldb,

@@ -27,8 +27,9 @@
 namespace qmlh {
 
 static constexpr int LK_TILE = 128;
-static constexpr int LK_MOLS = QMLH_LOCAL_PANEL_MOLS;  // molecules per panel (two blocks of 16)
-static constexpr int LK_MB = LK_MOLS / 16;              // molecule blocks per panel side
+static constexpr int LK_MOLS = QMLH_LOCAL_PANEL_MOLS;  // molecules per panel: one MFMA block
+static constexpr int LK_NS = 4;                         // sigmas per launch
+static_assert(LK_MOLS == 16, "one 16-wide MFMA block of molecules per panel side");
 
 // Workgroups per launch; larger panel-pair counts are covered by a grid-stride loop.
 static constexpr int64_t LK_MAX_GRID = 1 << 20;
@@ -46,9 +47,16 @@ struct LkSide {
 
 struct LkSigmas {
     double inv[4];  // exp argument scale per sigma of this launch
-    int n;          // sigmas in this launch (<= NS)
+    int n;          // sigmas in this launch (<= LK_NS)
     int64_t first;  // index of inv[0] in the caller's list
 };
+
+__device__ __forceinline__ uint32_t lk_byte(uint32_t p, int i) { return (p >> (8 * i)) & 0xffu; }
+
+// sg.inv[s] without a dynamic index into the by-value argument (which would go to scratch).
+__device__ __forceinline__ double lk_inv(const LkSigmas& sg, int s) {
+    return s == 0 ? sg.inv[0] : s == 1 ? sg.inv[1] : s == 2 ? sg.inv[2] : sg.inv[3];
+}
 
 // Panel-local molecule index of tile atom `t` (row0 + t), -1 past the panel's last atom.
 __device__ __forceinline__ int lk_local_mol(const int64_t* __restrict__ off, int64_t m0,
@@ -60,87 +68,52 @@ __device__ __forceinline__ int lk_local_mol(const int64_t* __restrict__ off, int
     return lo;
 }
 
-// Reduce this wave's values to molecule-pair sums S (added to S).
+// Add this wave's molecule-pair sums of one sigma to S.
 // Lane layout shared by both tile kinds: value val(r, c) belongs to tile row rowof(r, lane & 15)
-// and tile column colof(c, lane >> 4); rmol[r][e] is the panel molecule of tile row
-// rowof(r, (lane >> 4) + 4e), cmol[c] that of tile column colof(c, lane >> 4) (-1: padding).
-// Stage 1, per row set r: T(m, n) = sum_c val(r, c) [cmol = 16 nb + n] (16 rows x 16
-// molecules); stage 2: S(ma, n) += sum_e [rmol[r][e] = 16 mb + ma] T((lane >> 4) + 4e, n).
+// and tile column colof(c, lane >> 4); byte e of rpack[r] is the panel molecule of tile row
+// rowof(r, (lane >> 4) + 4e), byte c % 4 of cpack[c / 4] that of tile column colof(c, lane >> 4)
+// (0xff: padding, matching no molecule).
+// Stage 1, per row set r: T(m, n) = sum_c val(r, c) [cmol[c] = n] (16 rows x 16 molecules);
+// stage 2: S(ma, n) += sum_e [rmol[r][e] = ma] T((lane >> 4) + 4e, n).
 template <int NR, int NC, typename Val>
-__device__ __forceinline__ void lk_reduce(Val val, const int (&rmol)[NR][4], const int (&cmol)[NC],
-                                          int nblk_a, int nblk_b,
-                                          double4_t (&S)[LK_MB][LK_MB]) {
+__device__ __forceinline__ void lk_reduce(Val val, const uint32_t (&rpack)[NR],
+                                          const uint32_t (&cpack)[NC / 4], double4_t& S) {
     typedef MfmaTraits<double> M;
-    const int me = threadIdx.x & 15;
+    const uint32_t me = threadIdx.x & 15;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double vr[NC];
+        double4_t T = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vr[c] = val(r, c);
+        for (int c = 0; c < NC; ++c)
+            T = M::mma(val(r, c), lk_byte(cpack[c / 4], c % 4) == me ? 1.0 : 0.0, T);
 #pragma unroll
-        for (int nb = 0; nb < LK_MB; ++nb) {
-            if (nb >= nblk_b) break;
-            double4_t T = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const double ind = cmol[c] == 16 * nb + me ? 1.0 : 0.0;
-                T = M::mma(vr[c], ind, T);
-            }
-#pragma unroll
-            for (int mb = 0; mb < LK_MB; ++mb) {
-                if (mb >= nblk_a) break;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const double ind = rmol[r][e] == 16 * mb + me ? 1.0 : 0.0;
-                    S[mb][nb] = M::mma(ind, T[e], S[mb][nb]);
-                }
-            }
-        }
+        for (int e = 0; e < 4; ++e) S = M::mma(lk_byte(rpack[r], e) == me ? 1.0 : 0.0, T[e], S);
     }
 }
 
 // acc[ma][n] += S in wave order (one barrier per wave); lane element e holds
-// S(16 mb + (lane >> 4) + 4e, 16 nb + (lane & 15)).
-__device__ __forceinline__ void lk_accumulate(double* __restrict__ acc,
-                                              const double4_t (&S)[LK_MB][LK_MB], int nblk_a,
-                                              int nblk_b) {
+// S((lane >> 4) + 4e, lane & 15).
+__device__ __forceinline__ void lk_accumulate(double* __restrict__ acc, const double4_t& S) {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     for (int w2 = 0; w2 < 4; ++w2) {
         if (w == w2) {
 #pragma unroll
-            for (int mb = 0; mb < LK_MB; ++mb)
-#pragma unroll
-                for (int nb = 0; nb < LK_MB; ++nb) {
-                    if (mb >= nblk_a || nb >= nblk_b) continue;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int ma = 16 * mb + (lane >> 4) + 4 * e;
-                        const int n = 16 * nb + (lane & 15);
-                        acc[ma * LK_MOLS + n] += S[mb][nb][e];
-                    }
-                }
+            for (int e = 0; e < 4; ++e) acc[((lane >> 4) + 4 * e) * LK_MOLS + (lane & 15)] += S[e];
         }
         __syncthreads();
     }
 }
 
-__device__ __forceinline__ void lk_zero(double4_t (&S)[LK_MB][LK_MB]) {
-#pragma unroll
-    for (int x = 0; x < LK_MB; ++x)
-#pragma unroll
-        for (int y = 0; y < LK_MB; ++y) S[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
-}
-
-template <int KIND, int NS>
-__global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide B, int64_t d,
+template <int KIND>
+__global__ __launch_bounds__(256, 2) void local_kernels_kernel(LkSide A, LkSide B, int64_t d,
                                                                LkSigmas sg, double* __restrict__ K,
                                                                int64_t ldk, int64_t sstride,
                                                                int symmetric, int64_t tiles_a,
                                                                int64_t ntiles) {
     __shared__ __attribute__((aligned(16))) double lds[KIND == LK_GAUSS_MFMA ? WG_LDS_ELEMS
                                                                               : 2 * PT_BK * PT_LDS];
-    __shared__ double acc[NS][LK_MOLS * LK_MOLS];
+    __shared__ double acc[LK_NS][LK_MOLS * LK_MOLS];
     __shared__ int smolA[LK_TILE], smolB[LK_TILE];
 
     const int t = threadIdx.x;
@@ -163,12 +136,11 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
         const int nma = (int)(A.panels[bi + 1] - ma0), nmb = (int)(B.panels[bj + 1] - mb0);
         const int64_t ra0 = A.offsets[ma0], ra1 = A.offsets[ma0 + nma];
         const int64_t rb0 = B.offsets[mb0], rb1 = B.offsets[mb0 + nmb];
-        const int nblk_a = (nma + 15) / 16, nblk_b = (nmb + 15) / 16;
         // Panels come from qmlh_local_panels; one with more molecules than the LDS sums hold
         // is skipped rather than overrunning them.
         if (nma > LK_MOLS || nmb > LK_MOLS || nma < 0 || nmb < 0) continue;
 
-        for (int q = t; q < NS * LK_MOLS * LK_MOLS; q += 256) (&acc[0][0])[q] = 0.0;
+        for (int q = t; q < LK_NS * LK_MOLS * LK_MOLS; q += 256) (&acc[0][0])[q] = 0.0;
 
         for (int64_t r0 = ra0; r0 < ra1; r0 += LK_TILE) {
             for (int64_t c0 = rb0; c0 < rb1; c0 += LK_TILE) {
@@ -184,25 +156,30 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
                     // Lane value (a, b, e): A row r0 + 64 (w >> 1) + 16 b + (lane & 15),
                     // B column c0 + 64 (w & 1) + 16 a + (lane >> 4) + 4 e.
                     const int rbase = 64 * (w >> 1), cbase = 64 * (w & 1);
-                    int rmol[4][4], cmol[16];
+                    uint32_t rpack[4], cpack[4];
                     double ni[4], nj[16];
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
                         const int64_t i = r0 + rbase + 16 * b + (lane & 15);
                         ni[b] = A.norms[i < A.atoms ? i : A.atoms - 1];
+                        uint32_t p = 0;
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            rmol[b][e] = smolA[rbase + 16 * b + (lane >> 4) + 4 * e];
+                            p |= (uint32_t)(smolA[rbase + 16 * b + (lane >> 4) + 4 * e] & 0xff) << (8 * e);
+                        rpack[b] = p;
                     }
 #pragma unroll
-                    for (int a = 0; a < 4; ++a)
+                    for (int a = 0; a < 4; ++a) {
+                        uint32_t p = 0;
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             const int cl = cbase + 16 * a + (lane >> 4) + 4 * e;
                             const int64_t j = c0 + cl;
                             nj[4 * a + e] = B.norms[j < B.atoms ? j : B.atoms - 1];
-                            cmol[4 * a + e] = smolB[cl];
+                            p |= (uint32_t)(smolB[cl] & 0xff) << (8 * e);
                         }
+                        cpack[a] = p;
+                    }
                     double d2[4][16];
 #pragma unroll
                     for (int b = 0; b < 4; ++b)
@@ -213,15 +190,13 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
                                 const double x = ni[b] + nj[4 * a + e] - 2.0 * tile.acc[a][b][e];
                                 d2[b][4 * a + e] = x > 0.0 ? x : 0.0;
                             }
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (s >= sg.n) break;
-                        const double inv = sg.inv[s];
-                        double4_t S[LK_MB][LK_MB];
-                        lk_zero(S);
-                        lk_reduce<4, 16>([&](int r, int c) { return exp(inv * d2[r][c]); }, rmol,
-                                         cmol, nblk_a, nblk_b, S);
-                        lk_accumulate(acc[s], S, nblk_a, nblk_b);
+#pragma unroll 1
+                    for (int s = 0; s < sg.n; ++s) {
+                        const double inv = lk_inv(sg, s);
+                        double4_t S = {0.0, 0.0, 0.0, 0.0};
+                        lk_reduce<4, 16>([&](int r, int c) { return exp(inv * d2[r][c]); },
+                                         rpack, cpack, S);
+                        lk_accumulate(acc[s], S);
                     }
                 } else {
                     double (*sA)[PT_LDS] = reinterpret_cast<double (*)[PT_LDS]>(lds);
@@ -235,24 +210,34 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
                     else
                         pair_tile_accumulate<OP_SQ>(dist, sA, sB, A.X, A.ld, A.atoms, B.X, B.ld,
                                                     B.atoms, d, r0, c0);
-                    int rmol[8][4], cmol[8];
+                    uint32_t rpack[8], cpack[2];
 #pragma unroll
-                    for (int r = 0; r < 8; ++r)
+                    for (int r = 0; r < 8; ++r) {
+                        uint32_t p = 0;
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            rmol[r][e] = smolA[2 * ((lane >> 4) + 4 * e) + 32 * (r / 2) + r % 2];
+                            p |= (uint32_t)(smolA[2 * ((lane >> 4) + 4 * e) + 32 * (r / 2) + r % 2] & 0xff)
+                                 << (8 * e);
+                        rpack[r] = p;
+                    }
 #pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        cmol[c] = smolB[2 * (4 * w + (lane >> 4)) + 32 * (c / 2) + c % 2];
+                    for (int q = 0; q < 2; ++q) {
+                        uint32_t p = 0;
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (s >= sg.n) break;
-                        const double inv = sg.inv[s];
-                        double4_t S[LK_MB][LK_MB];
-                        lk_zero(S);
-                        lk_reduce<8, 8>([&](int r, int c) { return exp(dist[r][c] * inv); }, rmol,
-                                        cmol, nblk_a, nblk_b, S);
-                        lk_accumulate(acc[s], S, nblk_a, nblk_b);
+                        for (int u = 0; u < 4; ++u) {
+                            const int c = 4 * q + u;
+                            p |= (uint32_t)(smolB[2 * (4 * w + (lane >> 4)) + 32 * (c / 2) + c % 2] & 0xff)
+                                 << (8 * u);
+                        }
+                        cpack[q] = p;
+                    }
+#pragma unroll 1
+                    for (int s = 0; s < sg.n; ++s) {
+                        const double inv = lk_inv(sg, s);
+                        double4_t S = {0.0, 0.0, 0.0, 0.0};
+                        lk_reduce<8, 8>([&](int r, int c) { return exp(dist[r][c] * inv); },
+                                        rpack, cpack, S);
+                        lk_accumulate(acc[s], S);
                     }
                 }
             }
@@ -261,8 +246,7 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
 
         // Write the panel pair's molecule sums; mirror for a symmetric call.
         const bool diag = symmetric && bi == bj;
-        for (int s = 0; s < NS; ++s) {
-            if (s >= sg.n) break;
+        for (int s = 0; s < sg.n; ++s) {
             double* Ks = K + (sg.first + s) * sstride;
             for (int q = t; q < LK_MOLS * LK_MOLS; q += 256) {
                 const int ma = q % LK_MOLS, mb = q / LK_MOLS;
@@ -278,12 +262,12 @@ __global__ __launch_bounds__(256, 1) void local_kernels_kernel(LkSide A, LkSide 
     }
 }
 
-template <int KIND, int NS>
+template <int KIND>
 static void lk_launch(int64_t ntiles, hipStream_t s, const LkSide& A, const LkSide& B,
                       int64_t d, const LkSigmas& sg, double* K, int64_t ldk, int64_t sstride,
                       int symmetric, int64_t tiles_a) {
     const int64_t grid = ntiles < LK_MAX_GRID ? ntiles : LK_MAX_GRID;
-    hipLaunchKernelGGL((local_kernels_kernel<KIND, NS>), dim3((unsigned)grid), dim3(256), 0, s, A,
+    hipLaunchKernelGGL((local_kernels_kernel<KIND>), dim3((unsigned)grid), dim3(256), 0, s, A,
                        B, d, sg, K, ldk, sstride, symmetric, tiles_a, ntiles);
 }
 
@@ -351,25 +335,19 @@ extern "C" int qmlh_local_kernels(int32_t kind, const double* A, int64_t atomsA,
     const double pairs = symmetric ? 0.5 * (double)atomsA * (double)(atomsA + 1)
                                    : (double)atomsA * (double)atomsB;
     TimerScope ts(QMLH_TIMER_LOCAL, pairs * (double)d, s);
-    for (int32_t s0 = 0; s0 < nsig; s0 += 4) {
+    for (int32_t s0 = 0; s0 < nsig; s0 += LK_NS) {
         LkSigmas sg;
-        sg.n = nsig - s0 < 4 ? (int)(nsig - s0) : 4;
+        sg.n = nsig - s0 < LK_NS ? (int)(nsig - s0) : LK_NS;
         sg.first = s0;
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < LK_NS; ++q) {
             const double sig = sigmas_h[s0 + (q < sg.n ? q : 0)];
             sg.inv[q] = kind == QMLH_PAIR_GAUSSIAN ? -0.5 / (sig * sig) : -1.0 / sig;
         }
         const int sym = symmetric ? 1 : 0;
-        if (mfma) {
-            if (sg.n == 1) lk_launch<LK_GAUSS_MFMA, 1>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-            else lk_launch<LK_GAUSS_MFMA, 4>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-        } else if (kind == QMLH_PAIR_GAUSSIAN) {
-            if (sg.n == 1) lk_launch<LK_GAUSS_VALU, 1>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-            else lk_launch<LK_GAUSS_VALU, 4>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-        } else {
-            if (sg.n == 1) lk_launch<LK_LAPLACE, 1>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-            else lk_launch<LK_LAPLACE, 4>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
-        }
+        if (mfma) lk_launch<LK_GAUSS_MFMA>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
+        else if (kind == QMLH_PAIR_GAUSSIAN)
+            lk_launch<LK_GAUSS_VALU>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
+        else lk_launch<LK_LAPLACE>(grid, s, sa, sb, d, sg, K, ldk, sstride, sym, npA);
         QMLH_CHECK_LAUNCH("local_kernels launch");
     }
     return QMLH_OK;

@@ -178,7 +178,7 @@ def test_local_kernels_paths_and_sigmas(kind, expanded, nsig):
 
 @pytest.mark.parametrize("kind", ["gaussian", "laplacian"])
 def test_local_kernels_ragged_edges(kind):
-    # Empty molecules, runs of one-atom molecules (the 32-molecule panel cap), and molecules
+    # Empty molecules, runs of one-atom molecules (the 16-molecule panel cap), and molecules
     # larger than a 128-atom tile (a panel alone, reduced over several tiles).
     from qml_amd.kernels import get_local_kernels_gaussian, get_local_kernels_laplacian
 
