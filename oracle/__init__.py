@@ -7,8 +7,8 @@ path (``qml_amd``) never imports it and has no CPU fallback.
 Contents
 --------
 * ``liboracle.so`` (oracle.c): C restatement of the reference Fortran loops
-  (qml/frepresentations.f90:52-205, qml/fkernels.f90:340-400, qml/fdistance.f90:23-80),
-  built with ``make -C oracle``.
+  (qml/frepresentations.f90:52-205 and 207-598, qml/fkernels.f90:23-196 and 340-400,
+  qml/fdistance.f90:23-80), built with ``make -C oracle``.
 * ``cho_solve``: qml/math.py:60-97 around qml/fcho_solve.f90:23-52, which is exactly LAPACK
   ``dpotrf('U')`` + ``dpotrs('U', nrhs=1)``. LAPACK is a third-party dependency of the
   reference (setup.py:24, unpinned ``-llapack``) and is absent from the reference tree; this
@@ -17,8 +17,9 @@ Contents
   into ``oracle/_ref/libqmlref.so`` (present only where the reference tree was available at
   build time). Used to pin the C restatement.
 
-Pinning (see tests/test_oracle.py): the reference's golden Coulomb-matrix fixtures
-(tests/test_representations.py:74-90 data), the water known answer
+Pinning (see tests/test_oracle.py, tests/test_oracle_local.py): the reference's golden
+Coulomb-matrix and atomic-Coulomb-matrix fixtures (tests/test_representations.py:74-132 data),
+the local-kernel fixtures (tests/test_energy_krr_atomic_cmat.py data), the water known answer
 (docs/source/examples.rst:37-56), the cho_solve fixture pair (tests/test_math.py:32-51),
 closed forms and brute force on small inputs, and bit-for-bit agreement with ``ref``.
 """
