@@ -45,6 +45,33 @@ def ref_local_vectors():
 
 
 @pytest.fixture(scope="session")
+def xyz_texts():
+    z = np.load(os.path.join(GOLDEN, "xyz_texts.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def write_xyz_files(xyz_texts, directory):
+    """Write the committed QM7 XYZ texts back to files; returns {name: path}."""
+    paths = {}
+    for name, text in zip(xyz_texts["names"], xyz_texts["texts"]):
+        path = os.path.join(str(directory), str(name))
+        with open(path, "wb") as f:
+            f.write(bytes(text))
+        paths[str(name)] = path
+    return paths
+
+
+def get_energies(text):
+    """tests/test_energy_krr_cmat.py:36-53: {xyz name: heat of formation} from hof_qm7.txt."""
+    energies = dict()
+    for line in text.splitlines():
+        tokens = line.split()
+        if tokens:
+            energies[tokens[0]] = float(tokens[1])
+    return energies
+
+
+@pytest.fixture(scope="session")
 def cho_pair():
     z = np.load(os.path.join(GOLDEN, "cho_solve_pair.npz"), allow_pickle=False)
     return z["K"], z["y"]

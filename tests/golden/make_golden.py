@@ -22,6 +22,10 @@ Outputs (data only):
                         Ks_local_laplacian.txt (tests/test_energy_krr_atomic_cmat.py:59-217;
                         K_local_gaussian.txt is in cho_solve_pair.npz), and the molecule order
                         that test's np.random.seed(666) shuffle of the first 1000 QM7 names gives.
+  xyz_texts.npz         the raw text of the QM7 files in qm7_subset.npz (tests/qm7/*.xyz), of
+                        tests/data/compound_test.xyz / .exyz (tests/test_compound.py) and of
+                        tests/data/hof_qm7.txt, for the XYZ reader and the end-to-end test on the
+                        reference's on-disk format.
   ref_local_vectors.npz outputs of the reference Fortran (oracle/_ref) on seeded synthetic
                         molecules: atomic Coulomb matrices (both orderings, with and without
                         cut-offs) and local kernels (two sigmas).
@@ -133,6 +137,23 @@ def ref_local_vectors():
     np.savez_compressed(os.path.join(HERE, "ref_local_vectors.npz"), **out)
 
 
+def xyz_texts():
+    q = np.load(os.path.join(HERE, "qm7_subset.npz"))
+    names = [str(n) for n in q["names"]]
+    texts = []
+    for name in names:
+        with open(os.path.join(REF, "tests/qm7", name)) as f:
+            texts.append(f.read())
+    extra = {}
+    for key, rel in (("compound_test_xyz", "tests/data/compound_test.xyz"),
+                     ("compound_test_exyz", "tests/data/compound_test.exyz"),
+                     ("hof_qm7", "tests/data/hof_qm7.txt")):
+        with open(os.path.join(REF, rel), "rb") as f:
+            extra[key] = np.array(f.read())
+    np.savez_compressed(os.path.join(HERE, "xyz_texts.npz"), names=np.array(names),
+                        texts=np.array([t.encode() for t in texts]), **extra)
+
+
 def ref_vectors():
     os.environ["OMP_NUM_THREADS"] = "1"
     sys.path.insert(0, ROOT)
@@ -163,4 +184,5 @@ if __name__ == "__main__":
     ref_vectors()
     local_fixtures()
     ref_local_vectors()
+    xyz_texts()
     print("golden fixtures written to", HERE)
