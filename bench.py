@@ -70,6 +70,8 @@ def parse():
                    help="run only the local-kernel section and print its JSON (profiling)")
     p.add_argument("--local-cpu-sample", type=int, default=2000,
                    help="molecules of the local-kernel CPU baseline sample (0: skip)")
+    p.add_argument("--qm7", type=int, default=1,
+                   help="run the QM7 end-to-end test from XYZ files (0 = skip)")
     p.add_argument("--sweep", type=int, default=64,
                    help="problems in the batched learning-curve sweep (0 = skip)")
     return p.parse_args()
@@ -279,6 +281,118 @@ def run_local(n_mol, cpu_sample, log):
     return out
 
 
+def run_qm7(log):
+    """The reference's QM7 test on its own file format (tests/test_energy_krr_cmat.py:54-113):
+    1000 XYZ files (committed as text in tests/golden/xyz_texts.npz, written to a temporary
+    directory) read into Compound objects, sorted Coulomb matrices (size 23), Laplacian KRR with
+    sigma = 10^4.2, lambda = 1e-10 on 700 training / 300 test molecules after the seed-666
+    shuffle. Timed end to end (files on disk in, predictions on the host out) on the GPU path,
+    and the same steps on the reference path (its Fortran from oracle/_ref, LAPACK via scipy)."""
+    import tempfile
+
+    import numpy as np
+
+    import oracle
+    from qml_amd.compound import Compound
+    from qml_amd.kernels import laplacian_kernel
+    from qml_amd.math import cho_solve
+
+    z = np.load(os.path.join(ROOT, "tests", "golden", "xyz_texts.npz"), allow_pickle=False)
+    hof = {}
+    for line in bytes(z["hof_qm7"]).decode().splitlines():
+        tok = line.split()
+        if tok:
+            hof[tok[0]] = float(tok[1])
+    out = {"workload": "qm7_krr_cmat: 1000 QM7 XYZ files, sorted CM size 23, Laplacian "
+                       "sigma=10^4.2, lambda=1e-10, 700 train / 300 test"}
+    with tempfile.TemporaryDirectory() as tmp:
+        paths = {}
+        for name, text in zip(z["names"], z["texts"]):
+            paths[str(name)] = os.path.join(tmp, str(name))
+            with open(paths[str(name)], "wb") as f:
+                f.write(bytes(text))
+        names = sorted(hof)[:1000]
+
+        def path_gpu():
+            mols = []
+            for n in names:
+                m = Compound(xyz=paths[n])
+                m.properties = hof[n]
+                m.generate_coulomb_matrix(size=23, sorting="row-norm")
+                mols.append(m)
+            np.random.seed(666)
+            np.random.shuffle(mols)
+            X = np.array([m.representation for m in mols[:700]])
+            Xs = np.array([m.representation for m in mols[-300:]])
+            Y = np.array([m.properties for m in mols[:700]])
+            Ys = np.array([m.properties for m in mols[-300:]])
+            K = laplacian_kernel(X, X, 10 ** 4.2)
+            K[np.diag_indices_from(K)] += 1e-10
+            alpha = cho_solve(K, Y)
+            return np.mean(np.abs(Ys - laplacian_kernel(X, Xs, 10 ** 4.2).T.dot(alpha)))
+
+        def path_ref():
+            use_ref = oracle.ref_available()
+            cm = oracle.ref_coulomb_matrix if use_ref else oracle.coulomb_matrix
+            mols = []
+            for n in names:
+                m = Compound(xyz=paths[n])
+                m.properties = hof[n]
+                m.representation = cm(m.nuclear_charges, m.coordinates, 23, "row-norm")
+                mols.append(m)
+            np.random.seed(666)
+            np.random.shuffle(mols)
+            X = np.array([m.representation for m in mols[:700]])
+            Xs = np.array([m.representation for m in mols[-300:]])
+            Y = np.array([m.properties for m in mols[:700]])
+            Ys = np.array([m.properties for m in mols[-300:]])
+            kern = (lambda A, B: oracle.ref_kernel("laplacian", A, B, 10 ** 4.2)) if use_ref \
+                else (lambda A, B: oracle.laplacian_kernel(A, B, 10 ** 4.2))
+            K = kern(X, X)
+            K[np.diag_indices_from(K)] += 1e-10
+            alpha = oracle.cho_solve(K, Y)
+            return np.mean(np.abs(Ys - kern(X, Xs).T.dot(alpha))), use_ref
+
+        def path_batched():
+            # The framework's own path: one reader pass per split, one CM launch per split,
+            # the fit and the predictions on the device.
+            from qml_amd.compound import read_xyz_batch
+            from qml_amd.krr import KRR
+
+            order = list(range(1000))
+            np.random.seed(666)
+            np.random.shuffle(order)
+            tr = [names[i] for i in order[:700]]
+            te = [names[i] for i in order[-300:]]
+            Z, R, off = read_xyz_batch([paths[n] for n in tr])
+            Zq, Rq, offq = read_xyz_batch([paths[n] for n in te])
+            model = KRR("laplacian", 10 ** 4.2, 1e-10, "fp64")
+            model.fit_molecules(Z, R, off, 23, np.array([hof[n] for n in tr]))
+            pred = model.predict_molecules(Zq, Rq, offq, 23)
+            return np.mean(np.abs(np.array([hof[n] for n in te]) - pred))
+
+        path_gpu()  # warm-up (library init, first launches)
+        path_batched()
+        t0 = time.perf_counter()
+        mae = path_gpu()
+        t1 = time.perf_counter()
+        mae_ref, use_ref = path_ref()
+        t2 = time.perf_counter()
+        mae_b = path_batched()
+        t3 = time.perf_counter()
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    out.update({"gpu_s": t1 - t0, "mae": float(mae), "gpu_batched_s": t3 - t2,
+                "batched_mae": float(mae_b), "reference_s": t2 - t1,
+                "reference_mae": float(mae_ref), "reference_kind": "reference" if use_ref else "port",
+                "reference_cores": cores,
+                "note": "gpu_s and reference_s: per-molecule Compound calls as in the reference "
+                        "test; gpu_batched_s: read_xyz_batch + one CM launch per split + "
+                        "KRR fit/predict on the device; file reads included everywhere"})
+    log(f"qm7 end to end: GPU {t1 - t0:.3f}s (MAE {mae:.3f}), batched {t3 - t2:.3f}s, "
+        f"reference {t2 - t1:.3f}s")
+    return out
+
+
 def run_sweep(count, log):
     """64 Laplacian KRR fits, n log-uniform in [500, 8000], (sigma, lambda) grid, size-23
     Coulomb matrices: one graph launch vs the same fits solved one after another."""
@@ -370,8 +484,8 @@ def main():
         log(f"copy calibration: 3 x {args.copy_mib} MiB elementwise read + write")
         del src, dst
     if args.local_only:
-        print(json.dumps({"local": run_local(args.local, args.local_cpu_sample, log)}),
-              flush=True)
+        print(json.dumps({"local": run_local(args.local, args.local_cpu_sample, log),
+                          "qm7": run_qm7(log) if args.qm7 else None}), flush=True)
         return
     N = args.n_train
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)
@@ -473,6 +587,7 @@ def main():
     sweep_out = run_sweep(args.sweep, log) if args.sweep > 0 else None
     kernels_out = run_kernels(log) if args.kernels else None
     local_out = run_local(args.local, args.local_cpu_sample, log) if args.local > 0 else None
+    qm7_out = run_qm7(log) if args.qm7 else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -508,6 +623,7 @@ def main():
         "sweep": sweep_out,
         "kernels": kernels_out,
         "local": local_out,
+        "qm7": qm7_out,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),

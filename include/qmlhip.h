@@ -147,6 +147,17 @@ int qmlh_potrs(const double *L, int64_t n, int64_t lda, double *x, void *ws, siz
 /* A(i,i) += value for i < n (the K[diag] += lambda step of tests/test_energy_krr_cmat.py:104). */
 int qmlh_add_diagonal(double *A, int64_t n, int64_t lda, double value, qmlh_stream_t stream);
 
+/* Inverse of a symmetric positive definite matrix.
+ * Replaces fcho_invert (qml/fcho_solve.f90:54-73: LAPACK dpotrf("L") + dpotri("L")) behind
+ * qml/math.py:33-58 cho_invert. On entry the lower triangle of the column-major A holds the
+ * matrix; on exit A holds the whole symmetric inverse (both triangles, as cho_invert returns it
+ * after its lower-to-upper copy). info: dpotrf's (0, or the order of the first non-positive
+ * leading minor; the inverse is then not meaningful, as in the reference). ws: device
+ * workspace of qmlh_cho_invert_workspace(n) bytes, 16-byte aligned. */
+size_t qmlh_cho_invert_workspace(int64_t n);
+int qmlh_cho_invert(double *A, int64_t n, int64_t lda, int32_t *info, void *ws, size_t ws_bytes,
+                    qmlh_stream_t stream);
+
 /* ---------------------------------------------------------------------------------------- *
  * Mixed precision (no single reference counterpart: together these replace the FP64
  * dpotrf/dpotrs pair of qml/fcho_solve.f90:35,46 in the KRR fit at FP64 accuracy).

@@ -325,6 +325,24 @@ def cho_solve(A, y):
     return x.reshape(n)
 
 
+def cho_invert(A):
+    """qml/math.py:33-58 around qml/fcho_solve.f90:54-73: LAPACK dpotrf('L') + dpotri('L') on
+    the Fortran-ordered array (A itself when it already is one), then lower -> upper copy."""
+    from scipy.linalg import lapack
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("expected square matrix")
+    I = np.asfortranarray(A)
+    c, info = lapack.dpotrf(I, lower=1, clean=0, overwrite_a=1)
+    if info > 0:
+        print(" WARNING: Cholesky decomposition DPOTRF() exited with error code:", info)
+    inv, info2 = lapack.dpotri(c, lower=1, overwrite_c=1)
+    I[...] = inv
+    i_lower = np.tril_indices_from(I)
+    I.T[i_lower] = I[i_lower]
+    return I
+
+
 def krr_fit_predict(X, y, Xs, kernel, sigma, lam):
     """KRR composition of tests/test_energy_krr_cmat.py:101-111: returns (alpha, y_pred)."""
     kern = laplacian_kernel if kernel == "laplacian" else gaussian_kernel

@@ -69,9 +69,10 @@ template <typename T>
 int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int32_t* sync,
                    hipStream_t s);
 
-// V := L^-1 V for an (n, nrhs) column-major right-hand side (storage float), L lower float;
+// V := L^-1 V for an (n, nrhs) column-major right-hand side, L lower (both float or double);
 // dinv: the FP64 inverses of L's 64 x 64 diagonal blocks (trsv_diag_inverse).
-int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
-                         int64_t nrhs, const double* dinv, hipStream_t s);
+template <typename T>
+int trsm_lower_fwd_multi(const T* L, int64_t n, int64_t ldl, T* V, int64_t ldv, int64_t nrhs,
+                         const double* dinv, hipStream_t s);
 
 }  // namespace qmlh

@@ -483,15 +483,16 @@ int potrs_impl(const T* L, int64_t n, int64_t lda, double* x, double* tmp, int32
 }
 
 // ------------------------------------------------------------------------------------------
-// Multiple right-hand sides: V := L^-1 V (V FP32, n x nrhs column-major), right-looking in
+// Multiple right-hand sides: V := L^-1 V (V n x nrhs column-major, FP32 or FP64), right-looking in
 // 256-row blocks. Within a block, each 64-row step multiplies by the FP64 inverse of the 64 x 64
 // diagonal block (trsv_diag_inverse) and updates the rest of the block; then one FP32 MFMA
 // product updates every row below the 256-row block, so V streams through HBM once per 256
 // rows instead of once per 64.
 // ------------------------------------------------------------------------------------------
 // V[0:w, j0 .. j0+63] := D V[0:w, j0 ..] with D the 64 x 64 FP64 inverse (zero outside w).
+template <typename T>
 __global__ __launch_bounds__(256) void dinv_apply_kernel(const double* __restrict__ D,
-                                                         float* __restrict__ V, int64_t ldv,
+                                                         T* __restrict__ V, int64_t ldv,
                                                          int w, int64_t nrhs) {
     __shared__ double sD[PF_IB][PF_IB + 1];  // sD[i][c] = D(i, c)
     __shared__ double sV[PF_IB][PF_IB + 1];  // sV[c][j] = V(c, j0 + j)
@@ -526,13 +527,14 @@ __global__ __launch_bounds__(256) void dinv_apply_kernel(const double* __restric
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) {
             const int64_t j = j0 + 16 * q + jj;
-            if (j < nrhs) V[i + j * ldv] = (float)o[jj];
+            if (j < nrhs) V[i + j * ldv] = (T)o[jj];
         }
     }
 }
 
-int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64_t ldv,
-                         int64_t nrhs, const double* dinv, hipStream_t s) {
+template <typename T>
+int trsm_lower_fwd_multi(const T* L, int64_t n, int64_t ldl, T* V, int64_t ldv, int64_t nrhs,
+                         const double* dinv, hipStream_t s) {
     if (n <= 0 || nrhs <= 0) return QMLH_OK;
     constexpr int64_t NBM = 256;
     for (int64_t k = 0; k < n; k += NBM) {
@@ -540,18 +542,18 @@ int trsm_lower_fwd_multi(const float* L, int64_t n, int64_t ldl, float* V, int64
         for (int64_t jj = 0; jj < kb; jj += PF_IB) {
             const int64_t c = k + jj;
             const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
-            hipLaunchKernelGGL(dinv_apply_kernel, dim3((unsigned)ceil_div(nrhs, PF_IB)), dim3(256),
+            hipLaunchKernelGGL(dinv_apply_kernel<T>, dim3((unsigned)ceil_div(nrhs, PF_IB)), dim3(256),
                                0, s, dinv + (c / PF_IB) * (PF_IB * PF_IB), V + c, ldv, w, nrhs);
             QMLH_CHECK_LAUNCH("trsm multi diagonal launch");
             // rest of the 256-row block: V[c+w : k+kb] -= L[c+w : k+kb, c : c+w] V[c : c+w];
             // B(j, kk) = V[(c + kk) + j ldv] (row layout).
             const int64_t rest_in = kb - jj - w;
-            QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (c + w) + c * ldl, ldl, V + c,
+            QMLH_TRY((launch_update<T, LAYOUT_ROW>(UPD_FULL, L + (c + w) + c * ldl, ldl, V + c,
                                                        ldv, rest_in, nrhs, w, V + (c + w), ldv,
                                                        nullptr, s)));
         }
         const int64_t rest = n - k - kb;
-        QMLH_TRY((launch_update<float, LAYOUT_ROW>(UPD_FULL, L + (k + kb) + k * ldl, ldl, V + k,
+        QMLH_TRY((launch_update<T, LAYOUT_ROW>(UPD_FULL, L + (k + kb) + k * ldl, ldl, V + k,
                                                    ldv, rest, nrhs, kb, V + (k + kb), ldv, nullptr,
                                                    s)));
     }
@@ -588,6 +590,78 @@ static int transpose_f32(const float* in, int64_t ldi, int64_t R, int64_t Cc, fl
                        R, Cc, out, ldo, tr);
     QMLH_CHECK_LAUNCH("transpose launch");
     return QMLH_OK;
+}
+
+template int trsm_lower_fwd_multi<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t,
+                                         const double*, hipStream_t);
+
+// ------------------------------------------------------------------------------------------
+// Inverse of an SPD matrix from its Cholesky factor (the drop-in for dpotrf + dpotri):
+//   W := L^-1 by the multi-right-hand-side forward solve of the identity (the columns' entries
+//   above the diagonal stay exact zeros), A := W^T (transpose), W := -(A A^T) on the lower
+//   tiles (FP64 MFMA rank-n update from zero), then A(i, j) = A(j, i) = -W(i, j), i >= j.
+// ------------------------------------------------------------------------------------------
+// W := diag * I (n x n): the identity, or zero.
+__global__ __launch_bounds__(256) void identity_f64_kernel(double* __restrict__ W, int64_t ldw,
+                                                           int64_t n, double diag) {
+    const int64_t total = n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx % n, j = idx / n;
+        W[i + j * ldw] = i == j ? diag : 0.0;
+    }
+}
+
+// out(j, i) = in(i, j), 64 x 64 tiles through LDS (tiles_r tiles down the rows of `in`).
+__global__ __launch_bounds__(256) void transpose_f64_kernel(const double* __restrict__ in,
+                                                            int64_t ldi, int64_t n,
+                                                            double* __restrict__ out, int64_t ldo,
+                                                            int64_t tiles_r) {
+    __shared__ double t[64][65];
+    const int64_t r0 = (blockIdx.x % tiles_r) * 64, c0 = (blockIdx.x / tiles_r) * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int q = ty; q < 64; q += 4) {
+        const int64_t i = r0 + tx, j = c0 + q;
+        if (i < n && j < n) t[q][tx] = in[i + j * ldi];
+    }
+    __syncthreads();
+    for (int q = ty; q < 64; q += 4) {
+        const int64_t i = r0 + q, j = c0 + tx;
+        if (i < n && j < n) out[j + i * ldo] = t[tx][q];
+    }
+}
+
+// A(i, j) = A(j, i) = -W(i, j) for i >= j, tile by tile: tile (bi, bj), bi >= bj, reads W's
+// lower tile and writes A's tile (bi, bj) and its mirror (bj, bi) through LDS.
+__global__ __launch_bounds__(256) void negate_symmetrize_kernel(const double* __restrict__ W,
+                                                                int64_t ldw, int64_t n,
+                                                                double* __restrict__ A,
+                                                                int64_t lda) {
+    __shared__ double t[64][65];
+    const int64_t bid = blockIdx.x;
+    int64_t bj = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while (bj * (bj + 1) / 2 > bid) --bj;
+    while ((bj + 1) * (bj + 2) / 2 <= bid) ++bj;
+    const int64_t bi = bid - bj * (bj + 1) / 2;  // bi <= bj: tile rows bj, columns bi of W
+    const int64_t r0 = bj * 64, c0 = bi * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int q = ty; q < 64; q += 4) {
+        const int64_t i = r0 + tx, j = c0 + q;
+        double v = 0.0;
+        if (i < n && j < n) {
+            // the diagonal tile's upper half mirrors its lower half
+            v = (i >= j) ? -W[i + j * ldw] : -W[j + i * ldw];
+            A[i + j * lda] = v;
+        }
+        t[q][tx] = v;
+    }
+    __syncthreads();
+    if (bi == bj) return;
+    for (int q = ty; q < 64; q += 4) {
+        // mirror: A(c0 + tx, r0 + q) = value at (r0 + q, c0 + tx) = t[tx][q]
+        const int64_t i = c0 + tx, j = r0 + q;
+        if (i < n && j < n) A[i + j * lda] = t[tx][q];
+    }
 }
 
 template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*,
@@ -666,4 +740,43 @@ extern "C" int qmlh_potrs(const double* L, int64_t n, int64_t lda, double* x, vo
     double* tmp = static_cast<double*>(ws);
     int32_t* sync = n > 0 ? reinterpret_cast<int32_t*>(tmp + n) : nullptr;
     return potrs_impl<double>(L, n, lda, x, tmp, sync, (hipStream_t)stream);
+}
+
+extern "C" size_t qmlh_cho_invert_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    return (size_t)(n * n + (int64_t)qmlh::trsv_dinv_doubles(n)) * sizeof(double);
+}
+
+extern "C" int qmlh_cho_invert(double* A, int64_t n, int64_t lda, int32_t* info, void* ws,
+                               size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || lda < (n > 1 ? n : 1)) return set_error(QMLH_ERR_ARG, "cho_invert: n / lda");
+    if (n == 0) return QMLH_OK;
+    if (!A || !info) return set_error(QMLH_ERR_ARG, "cho_invert: null pointer");
+    if (!ws || ws_bytes < qmlh_cho_invert_workspace(n))
+        return set_error(QMLH_ERR_WORKSPACE, "cho_invert: workspace too small");
+    if (reinterpret_cast<uintptr_t>(ws) & 15)
+        return set_error(QMLH_ERR_ARG, "cho_invert: workspace must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    double* W = static_cast<double*>(ws);
+    double* dinv = W + n * n;
+    const int64_t ldw = n;
+    QMLH_TRY(potrf_impl<double>(A, n, lda, info, s));
+    const int64_t grid = ceil_div(n * n, 256) < 65536 ? ceil_div(n * n, 256) : 65536;
+    hipLaunchKernelGGL(identity_f64_kernel, dim3((unsigned)grid), dim3(256), 0, s, W, ldw, n, 1.0);
+    QMLH_CHECK_LAUNCH("cho_invert identity launch");
+    QMLH_TRY(trsv_diag_inverse<double>(A, n, lda, dinv, s));
+    QMLH_TRY(trsm_lower_fwd_multi<double>(A, n, lda, W, ldw, n, dinv, s));  // W = L^-1
+    const int64_t tr = ceil_div(n, 64);
+    hipLaunchKernelGGL(transpose_f64_kernel, dim3((unsigned)(tr * tr)), dim3(256), 0, s, W, ldw,
+                       n, A, lda, tr);  // A = L^-T
+    QMLH_CHECK_LAUNCH("cho_invert transpose launch");
+    hipLaunchKernelGGL(identity_f64_kernel, dim3((unsigned)grid), dim3(256), 0, s, W, ldw, n, 0.0);
+    QMLH_CHECK_LAUNCH("cho_invert clear launch");
+    // W := 0 - A A^T = -(L^-T L^-1) on the lower tiles (FP64 MFMA).
+    QMLH_TRY((launch_update<double>(UPD_SYRK, A, lda, A, lda, n, n, n, W, ldw, nullptr, s)));
+    hipLaunchKernelGGL(negate_symmetrize_kernel, dim3((unsigned)(tr * (tr + 1) / 2)), dim3(256),
+                       0, s, W, ldw, n, A, lda);
+    QMLH_CHECK_LAUNCH("cho_invert symmetrize launch");
+    return QMLH_OK;
 }

@@ -1,8 +1,13 @@
-"""Cholesky solve on the GPU: the drop-in for qml.math.cho_solve (qml/math.py:60-97).
+"""Cholesky solve and inverse on the GPU: the drop-ins for qml.math.cho_solve
+(qml/math.py:60-97) and qml.math.cho_invert (qml/math.py:33-58).
 
-The reference calls LAPACK dpotrf("U") and dpotrs("U", n, 1) (qml/fcho_solve.f90:35,46) on the
-upper triangle (numpy indices i <= j), prints a WARNING on a non-zero info and still returns x,
-then restores the diagonal of A and copies its lower triangle onto the upper one.
+cho_solve: the reference calls LAPACK dpotrf("U") and dpotrs("U", n, 1)
+(qml/fcho_solve.f90:35,46) on the upper triangle (numpy indices i <= j), prints a WARNING on a
+non-zero info and still returns x, then restores the diagonal of A and copies its lower triangle
+onto the upper one.
+cho_invert: the reference makes a Fortran-ordered array I of A (a copy unless A already is one,
+in which case A itself is overwritten), runs dpotrf("L") + dpotri("L") on it
+(qml/fcho_solve.f90:54-73) and copies I's lower triangle onto the upper one.
 """
 
 from __future__ import annotations
@@ -86,3 +91,41 @@ def cho_solve(A, y):
     print_potrf_warning(int(info.item()))
     _copy_lower_to_upper(A)
     return x
+
+
+def cho_invert_device(Ad, n: int, stream=None):
+    """In place: the column-major (n, n) device matrix (lower triangle read) becomes its whole
+    symmetric inverse. Returns the device info tensor (dpotrf's)."""
+    import torch
+
+    L = _lib.lib()
+    info = torch.zeros(1, dtype=torch.int32, device=Ad.device)
+    nbytes = L.qmlh_cho_invert_workspace(n)
+    ws = torch.empty(max(1, (nbytes + 7) // 8), dtype=torch.float64, device=Ad.device)
+    _lib.check(L.qmlh_cho_invert(_lib.ptr(Ad), n, n, _lib.ptr(info), _lib.ptr(ws),
+                                 ws.numel() * 8, _lib.stream_handle(stream)))
+    return info
+
+
+def cho_invert(A):
+    """Inverse of a symmetric positive definite matrix on the GPU; same signature, result
+    layout (Fortran order) and aliasing as qml.math.cho_invert (qml/math.py:33-58)."""
+    import torch
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError('expected square matrix')
+    I = np.asfortranarray(A)
+    if I.dtype != np.float64:
+        raise TypeError(f"cho_invert: expected a float64 array (got {I.dtype}), as the "
+                        "reference's in-place Fortran call requires")
+    n = I.shape[0]
+    if n == 0:
+        return I
+    # I.T is a C-ordered view of I's bytes: its rows are I's columns (column-major I).
+    Id = torch.from_numpy(np.ascontiguousarray(I.T)).to(device())
+    info = cho_invert_device(Id, n)
+    I.T[...] = Id.cpu().numpy()
+    code = int(info.item())
+    if code > 0:
+        print(f" WARNING: Cholesky decomposition DPOTRF() exited with error code:{code:12d}")
+    return I
