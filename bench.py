@@ -67,11 +67,14 @@ def parse():
     p.add_argument("--local", type=int, default=20000,
                    help="molecules of the local-kernel section (0: skip)")
     p.add_argument("--local-only", action="store_true",
-                   help="run only the local-kernel section and print its JSON (profiling)")
+                   help="run only the sections of the rows beyond the fit (local kernels, QM7 "
+                        "from XYZ files, cho_invert) and print their JSON (profiling)")
     p.add_argument("--local-cpu-sample", type=int, default=2000,
                    help="molecules of the local-kernel CPU baseline sample (0: skip)")
     p.add_argument("--qm7", type=int, default=1,
                    help="run the QM7 end-to-end test from XYZ files (0 = skip)")
+    p.add_argument("--invert", type=int, default=20000,
+                   help="order of the cho_invert measurement (0 = skip)")
     p.add_argument("--sweep", type=int, default=64,
                    help="problems in the batched learning-curve sweep (0 = skip)")
     return p.parse_args()
@@ -393,6 +396,72 @@ def run_qm7(log):
     return out
 
 
+def run_invert(n, cpu_n, log):
+    """cho_invert at n (device-resident, qmlh_cho_invert: FP64 Cholesky, inverse of the factor
+    from the identity, MFMA rank-n product) on a synthetic SPD Gaussian kernel matrix plus
+    lambda = 1e-6. Algorithmic work is LAPACK's dpotrf + dpotri: n^3 flops. CPU baseline:
+    scipy's LAPACK dpotrf + dpotri (what the reference's fcho_invert calls) at cpu_n,
+    extrapolated as n^3."""
+    import numpy as np
+    import torch
+
+    from qml_amd import _lib, synthetic
+    from qml_amd.kernels import gaussian_mfma_device, row_sqnorms_device
+    from qml_amd.math import add_diagonal_device, cho_invert_device
+    from qml_amd.representations import cm_length, generate_coulomb_matrices
+
+    tr, _, size = synthetic.workload("krr_serve_50k", n_train=n, n_test=0)
+    d = cm_length(size)
+    X = generate_coulomb_matrices(*tr.ragged(), size=size)
+    nr = row_sqnorms_device(X, n, d)
+    K0 = gaussian_mfma_device(X, nr, n, X, nr, n, d, 4000.0, True)
+    add_diagonal_device(K0, n, 1e-6)
+    K = torch.empty_like(K0)
+    K.copy_(K0)
+    cho_invert_device(K, n)
+    torch.cuda.synchronize()
+    reps, ms = 2, 0.0
+    for _ in range(reps):
+        K.copy_(K0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        info = cho_invert_device(K, n)
+        e1.record()
+        e1.synchronize()
+        ms += e0.elapsed_time(e1) / reps
+    # residual of the inverse on a few columns: |K0 Ki e_j - e_j|
+    cols = torch.arange(0, n, max(1, n // 8), device=K.device)
+    # K is symmetric, so its rows `cols` are the columns; (n, n) row-major = column-major.
+    R = torch.matmul(K0, K[cols].T.contiguous())
+    E = torch.zeros_like(R)
+    E[cols, torch.arange(cols.numel(), device=K.device)] = 1.0
+    resid = float((R - E).abs().max())
+    flops = float(n) ** 3
+    out = {"n": n, "ms": ms, "tflops_lapack_equiv": flops / (ms * 1e-3) / 1e12,
+           "frac": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"], "info": int(info.item()),
+           "max_abs_residual_sampled_columns": resid,
+           "workload": f"cho_invert of a {n} x {n} Gaussian kernel matrix (size 29 Coulomb "
+                       "matrices, sigma=4000) + 1e-6 I, FP64"}
+    del K, K0, X
+    torch.cuda.empty_cache()
+    if cpu_n > 0:
+        from scipy.linalg import lapack
+
+        A = np.asfortranarray(np.eye(cpu_n) * 2.0 + np.random.default_rng(0).random((cpu_n, cpu_n)) * 1e-3)
+        A = np.asfortranarray(A + A.T)
+        t0 = time.perf_counter()
+        c, _ = lapack.dpotrf(A, lower=1, overwrite_a=1)
+        lapack.dpotri(c, lower=1, overwrite_c=1)
+        t1 = time.perf_counter()
+        cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+        out["cpu_baseline"] = {"value": (t1 - t0) * (n / cpu_n) ** 3, "unit": "s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"LAPACK dpotrf + dpotri (scipy) at n={cpu_n}: "
+                                         f"{t1 - t0:.2f}s on {cores} threads; extrapolated as n^3"}
+    log(f"cho_invert n={n}: {ms:.1f} ms ({out['tflops_lapack_equiv']:.1f} TF/s LAPACK-equivalent)")
+    return out
+
+
 def run_sweep(count, log):
     """64 Laplacian KRR fits, n log-uniform in [500, 8000], (sigma, lambda) grid, size-23
     Coulomb matrices: one graph launch vs the same fits solved one after another."""
@@ -485,7 +554,10 @@ def main():
         del src, dst
     if args.local_only:
         print(json.dumps({"local": run_local(args.local, args.local_cpu_sample, log),
-                          "qm7": run_qm7(log) if args.qm7 else None}), flush=True)
+                          "qm7": run_qm7(log) if args.qm7 else None,
+                          "cho_invert": run_invert(args.invert, 10000 if args.local_cpu_sample
+                                                   else 0, log) if args.invert > 0 else None}),
+              flush=True)
         return
     N = args.n_train
     tr, te, size = synthetic.workload("krr_serve_50k", n_train=N, n_test=4096)
@@ -588,6 +660,7 @@ def main():
     kernels_out = run_kernels(log) if args.kernels else None
     local_out = run_local(args.local, args.local_cpu_sample, log) if args.local > 0 else None
     qm7_out = run_qm7(log) if args.qm7 else None
+    invert_out = run_invert(args.invert, 10000, log) if args.invert > 0 else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -624,6 +697,7 @@ def main():
         "kernels": kernels_out,
         "local": local_out,
         "qm7": qm7_out,
+        "cho_invert": invert_out,
         "phases_fp64": phases,
         "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),

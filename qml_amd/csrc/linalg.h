@@ -73,6 +73,6 @@ int trsv_lower_bwd(const T* L, int64_t n, int64_t lda, double* z, double* x, int
 // dinv: the FP64 inverses of L's 64 x 64 diagonal blocks (trsv_diag_inverse).
 template <typename T>
 int trsm_lower_fwd_multi(const T* L, int64_t n, int64_t ldl, T* V, int64_t ldv, int64_t nrhs,
-                         const double* dinv, hipStream_t s);
+                         const double* dinv, hipStream_t s, bool lower_rhs = false);
 
 }  // namespace qmlh

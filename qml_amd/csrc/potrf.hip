@@ -534,28 +534,32 @@ __global__ __launch_bounds__(256) void dinv_apply_kernel(const double* __restric
 
 template <typename T>
 int trsm_lower_fwd_multi(const T* L, int64_t n, int64_t ldl, T* V, int64_t ldv, int64_t nrhs,
-                         const double* dinv, hipStream_t s) {
+                         const double* dinv, hipStream_t s, bool lower_rhs) {
     if (n <= 0 || nrhs <= 0) return QMLH_OK;
     constexpr int64_t NBM = 256;
+    // With a lower-triangular right-hand side (column j zero above row j, e.g. the identity),
+    // rows r.. only involve columns j < r + width: the others are still zero there.
+    auto cols = [&](int64_t row_end) { return lower_rhs && row_end < nrhs ? row_end : nrhs; };
     for (int64_t k = 0; k < n; k += NBM) {
         const int64_t kb = n - k < NBM ? n - k : NBM;
         for (int64_t jj = 0; jj < kb; jj += PF_IB) {
             const int64_t c = k + jj;
             const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
-            hipLaunchKernelGGL(dinv_apply_kernel<T>, dim3((unsigned)ceil_div(nrhs, PF_IB)), dim3(256),
-                               0, s, dinv + (c / PF_IB) * (PF_IB * PF_IB), V + c, ldv, w, nrhs);
+            const int64_t nc = cols(c + w);
+            hipLaunchKernelGGL(dinv_apply_kernel<T>, dim3((unsigned)ceil_div(nc, PF_IB)), dim3(256),
+                               0, s, dinv + (c / PF_IB) * (PF_IB * PF_IB), V + c, ldv, w, nc);
             QMLH_CHECK_LAUNCH("trsm multi diagonal launch");
             // rest of the 256-row block: V[c+w : k+kb] -= L[c+w : k+kb, c : c+w] V[c : c+w];
             // B(j, kk) = V[(c + kk) + j ldv] (row layout).
             const int64_t rest_in = kb - jj - w;
             QMLH_TRY((launch_update<T, LAYOUT_ROW>(UPD_FULL, L + (c + w) + c * ldl, ldl, V + c,
-                                                       ldv, rest_in, nrhs, w, V + (c + w), ldv,
+                                                       ldv, rest_in, nc, w, V + (c + w), ldv,
                                                        nullptr, s)));
         }
         const int64_t rest = n - k - kb;
         QMLH_TRY((launch_update<T, LAYOUT_ROW>(UPD_FULL, L + (k + kb) + k * ldl, ldl, V + k,
-                                                   ldv, rest, nrhs, kb, V + (k + kb), ldv, nullptr,
-                                                   s)));
+                                                   ldv, rest, cols(k + kb), kb, V + (k + kb), ldv,
+                                                   nullptr, s)));
     }
     return QMLH_OK;
 }
@@ -593,7 +597,7 @@ static int transpose_f32(const float* in, int64_t ldi, int64_t R, int64_t Cc, fl
 }
 
 template int trsm_lower_fwd_multi<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t,
-                                         const double*, hipStream_t);
+                                         const double*, hipStream_t, bool);
 
 // ------------------------------------------------------------------------------------------
 // Inverse of an SPD matrix from its Cholesky factor (the drop-in for dpotrf + dpotri):
@@ -766,15 +770,25 @@ extern "C" int qmlh_cho_invert(double* A, int64_t n, int64_t lda, int32_t* info,
     hipLaunchKernelGGL(identity_f64_kernel, dim3((unsigned)grid), dim3(256), 0, s, W, ldw, n, 1.0);
     QMLH_CHECK_LAUNCH("cho_invert identity launch");
     QMLH_TRY(trsv_diag_inverse<double>(A, n, lda, dinv, s));
-    QMLH_TRY(trsm_lower_fwd_multi<double>(A, n, lda, W, ldw, n, dinv, s));  // W = L^-1
+    QMLH_TRY(trsm_lower_fwd_multi<double>(A, n, lda, W, ldw, n, dinv, s, true));  // W = L^-1
     const int64_t tr = ceil_div(n, 64);
     hipLaunchKernelGGL(transpose_f64_kernel, dim3((unsigned)(tr * tr)), dim3(256), 0, s, W, ldw,
                        n, A, lda, tr);  // A = L^-T
     QMLH_CHECK_LAUNCH("cho_invert transpose launch");
     hipLaunchKernelGGL(identity_f64_kernel, dim3((unsigned)grid), dim3(256), 0, s, W, ldw, n, 0.0);
     QMLH_CHECK_LAUNCH("cho_invert clear launch");
-    // W := 0 - A A^T = -(L^-T L^-1) on the lower tiles (FP64 MFMA).
-    QMLH_TRY((launch_update<double>(UPD_SYRK, A, lda, A, lda, n, n, n, W, ldw, nullptr, s)));
+    // W := 0 - A A^T = -(L^-T L^-1) on the lower triangle (FP64 MFMA), by row blocks: A = L^-T
+    // is upper triangular, so row block [r0, r1) only needs the products over k >= r0 --
+    // a rectangular update left of the diagonal block and a lower-tile one on it.
+    constexpr int64_t INV_NB = 512;
+    for (int64_t r0 = 0; r0 < n; r0 += INV_NB) {
+        const int64_t rb = n - r0 < INV_NB ? n - r0 : INV_NB;
+        const double* Ar = A + r0 + r0 * lda;  // rows r0.., k from r0
+        QMLH_TRY((launch_update<double>(UPD_FULL, Ar, lda, A + r0 * lda, lda, rb, r0, n - r0,
+                                        W + r0, ldw, nullptr, s)));
+        QMLH_TRY((launch_update<double>(UPD_SYRK, Ar, lda, Ar, lda, rb, rb, n - r0,
+                                        W + r0 + r0 * ldw, ldw, nullptr, s)));
+    }
     hipLaunchKernelGGL(negate_symmetrize_kernel, dim3((unsigned)(tr * (tr + 1) / 2)), dim3(256),
                        0, s, W, ldw, n, A, lda);
     QMLH_CHECK_LAUNCH("cho_invert symmetrize launch");
