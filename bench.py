@@ -146,7 +146,9 @@ def roofline(timers, steps, factor):
 def run_kernels(log):
     """Kernel-matrix builds at the large-configuration shapes: the Laplacian (FP64 VALU) on a
     row block of 8192 queries against 100,000 training rows (size 29, d = 435, sigma = 724),
-    and the symmetric Gaussian (FP64 MFMA expanded form) at N = 20,000 (sigma = 4000)."""
+    the symmetric Gaussian (FP64 MFMA expanded form) at N = 20,000 (sigma = 4000), the linear
+    kernel on the same tiles, and the Sargan / Matern (L2) kernels and p-distances (p = 3 by
+    products, p = 2.5 by pow on a 1024-row block) at the Laplacian's shape."""
     import torch
 
     from qml_amd import _lib, synthetic
@@ -189,8 +191,49 @@ def run_kernels(log):
     out["gaussian_mfma_20k"] = {"shape": f"{n} x {n} symmetric, d={d}", "ms": ms,
                                 "tflops": flops / (ms * 1e-3) / 1e12,
                                 "frac_of_fp64_mfma": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
+    # Linear kernel on the same cross-term tiles (qmlh_linear_mfma), symmetric 20k.
+    from qml_amd.distance import p_distance  # noqa: F401  (drop-in; device form used below)
+    from qml_amd.kernels import pairwise_ext_device
+
+    lin = torch.empty((n, n), dtype=torch.float64, device=X.device)
+    L = _lib.lib()
+
+    def linear():
+        _lib.check(L.qmlh_linear_mfma(_lib.ptr(X2), n, X2.shape[1], _lib.ptr(X2), n, X2.shape[1],
+                                      d, _lib.ptr(lin), n, _lib.SYMMETRIC,
+                                      _lib.stream_handle(None)))
+
+    linear()
+    e0.record()
+    for _ in range(reps):
+        linear()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    out["linear_mfma_20k"] = {"shape": f"{n} x {n} symmetric, d={d}", "ms": ms,
+                              "tflops": flops / (ms * 1e-3) / 1e12,
+                              "frac_of_fp64_mfma": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
+    del lin, K2
+    # Parameterised VALU kernels on a row block against the 100k set (the Laplacian's shape).
+    K = torch.empty((100000, rows), dtype=torch.float64, device=X.device)
+    for name, kind, params, nrows in (
+            ("sargan_2gammas", _lib.PAIR_SARGAN, [724.0, 0.5, 0.25], rows),
+            ("matern_l2_order2", _lib.PAIR_MATERN_L2, [724.0, 2.0], rows),
+            ("p_distance_3", _lib.PAIR_P_INT, [3.0], rows),
+            ("p_distance_2.5", _lib.PAIR_P_REAL, [2.5], 1024)):
+        Kv = K.view(-1)[:100000 * nrows].view(100000, nrows)  # (nrows, 100000) column-major
+        pairwise_ext_device(kind, X[:nrows], nrows, X, 100000, d, params, False, Kv)
+        e0.record()
+        pairwise_ext_device(kind, X[:nrows], nrows, X, 100000, d, params, False, Kv)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        pf = nrows * 100000.0 * d
+        out[name] = {"shape": f"{nrows} x 100000, d={d}", "ms": ms,
+                     "pair_features_per_s": pf / (ms * 1e-3)}
     log(f"kernels: laplacian {out['laplacian_rowblock']['frac_of_fp64_valu']:.2f} of FP64 VALU, "
-        f"gaussian {out['gaussian_mfma_20k']['tflops']:.1f} TF/s")
+        f"gaussian {out['gaussian_mfma_20k']['tflops']:.1f} TF/s, linear "
+        f"{out['linear_mfma_20k']['tflops']:.1f} TF/s")
     return out
 
 

@@ -111,6 +111,32 @@ int qmlh_pairwise(int32_t kind, const double *A, int64_t na, int64_t lda, const 
                   int64_t nb, int64_t ldb, int64_t d, double sigma, double *K, int64_t ldk,
                   int32_t flags, qmlh_stream_t stream);
 
+/* Pairwise kernels with parameters (VALU tile kernels, direct differences), the epilogue
+ * variants of the same feature sums. params_h: host array.
+ * kind QMLH_PAIR_SARGAN: params = [sigma, gamma_1 .. gamma_ng] (ng <= 16):
+ *      K(i,j) = exp(-d/sigma) (1 + sum_m gamma_m (d/sigma)^m), d = sum_k |A_ik - B_jk|;
+ *      replaces fsargan_kernel (qml/fkernels.f90:491-531) behind qml/kernels.py:117-150 (and
+ *      matern_kernel's L1 form, qml/kernels.py:152-216).
+ * kind QMLH_PAIR_MATERN_L2: params = [sigma, order (0, 1, 2)], d = sqrt(sum_k (A_ik - B_jk)^2):
+ *      exp(-d/s), exp(-sqrt3 d/s)(1 + sqrt3 d/s), exp(-sqrt5 d/s)(1 + sqrt5 d/s + 5 d^2/(3 s^2));
+ *      replaces fmatern_kernel_l2 (qml/fkernels.f90:427-488).
+ * kind QMLH_PAIR_P_INT: params = [p], integer p >= 1: D(i,j) = (sum_k |A_ik - B_jk|^p)^(1/p)
+ *      (p = 1, 3, 4 by products, other p by pow); replaces fp_distance_integer
+ *      (qml/fdistance.f90:121-155).
+ * kind QMLH_PAIR_P_REAL: params = [p], p > 0, real powers; replaces fp_distance_double
+ *      (qml/fdistance.f90:83-118). Both behind qml/distance.py:94-147 p_distance.
+ * flags QMLH_SYMMETRIC as for qmlh_pairwise. */
+enum {
+    QMLH_PAIR_SARGAN = 4,
+    QMLH_PAIR_MATERN_L2 = 5,
+    QMLH_PAIR_P_INT = 6,
+    QMLH_PAIR_P_REAL = 7
+};
+int qmlh_pairwise_ext(int32_t kind, const double *A, int64_t na, int64_t lda, const double *B,
+                      int64_t nb, int64_t ldb, int64_t d, const double *params_h,
+                      int32_t nparams, double *K, int64_t ldk, int32_t flags,
+                      qmlh_stream_t stream);
+
 /* Squared row norms norms[i] = sum_k X_ik^2 in FP64 (k in order). */
 int qmlh_row_sqnorms(const double *X, int64_t n, int64_t ldx, int64_t d, double *norms,
                      qmlh_stream_t stream);
@@ -126,6 +152,13 @@ int qmlh_gaussian_mfma(const double *A, int64_t na, int64_t lda, const double *n
                        const double *B, int64_t nb, int64_t ldb, const double *normsB,
                        int64_t d, double sigma, double *K, int64_t ldk, int32_t flags,
                        qmlh_stream_t stream);
+
+/* Linear kernel K(i,j) = sum_k A_ik B_jk on FP64 MFMA (the same cross-term tiles as the
+ * Gaussian); replaces flinear_kernel (qml/fkernels.f90:403-424) behind qml/kernels.py:94-115.
+ * Same layout rules as qmlh_gaussian_mfma; QMLH_SYMMETRIC computes upper tiles and mirrors. */
+int qmlh_linear_mfma(const double *A, int64_t na, int64_t lda, const double *B, int64_t nb,
+                     int64_t ldb, int64_t d, double *K, int64_t ldk, int32_t flags,
+                     qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Cholesky factorisation and solve.

@@ -76,6 +76,17 @@ def lib():
                                            _c_double_p, _c_int64_p, i64, i64, _c_double_p, i64,
                                            _c_double_p]
         L.oracle_local_kernels.restype = None
+        L.oracle_linear.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, _c_double_p]
+        L.oracle_linear.restype = None
+        L.oracle_sargan.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, dbl, _c_double_p,
+                                    i64, _c_double_p]
+        L.oracle_sargan.restype = None
+        L.oracle_matern_l2.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, dbl, ctypes.c_int,
+                                       _c_double_p]
+        L.oracle_matern_l2.restype = None
+        L.oracle_p_distance.argtypes = [_c_double_p, i64, _c_double_p, i64, i64, dbl, ctypes.c_int,
+                                        _c_double_p]
+        L.oracle_p_distance.restype = None
         _lib = L
     return _lib
 
@@ -106,6 +117,9 @@ def ref():
         L.ref_local_kernels.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_int_p, ci,
                                         _c_int_p, ci, _c_double_p, ci, _c_double_p]
         L.ref_local_kernels.restype = None
+        L.ref_kernel_ext.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p, dbl,
+                                     _c_double_p, ci, ci, dbl, ci]
+        L.ref_kernel_ext.restype = None
         _ref = L
     return _ref
 
@@ -286,6 +300,82 @@ def ref_kernel(kind, A, B, sigma=1.0):
     A, B, K = _prep(A, B)
     ref().ref_kernel(REF_KIND[kind], _ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1],
                      _ptr(K), float(sigma))
+    return K
+
+
+def linear_kernel(A, B):
+    A, B, K = _prep(A, B)
+    lib().oracle_linear(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], _ptr(K))
+    return K
+
+
+def sargan_kernel(A, B, sigma, gammas):
+    """fsargan_kernel; qml/kernels.py:117-150 returns the Laplacian kernel for no gammas."""
+    g = np.ascontiguousarray(np.asarray(gammas, dtype=np.float64).reshape(-1))
+    if g.size == 0:
+        return laplacian_kernel(A, B, sigma)
+    A, B, K = _prep(A, B)
+    lib().oracle_sargan(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], float(sigma),
+                        _ptr(g), g.size, _ptr(K))
+    return K
+
+
+def matern_kernel(A, B, sigma, order=0, metric="l1"):
+    """qml/kernels.py:152-216: the L1 form through sargan_kernel, the L2 form fmatern_kernel_l2."""
+    if metric == "l1":
+        if order == 0:
+            gammas = []
+        elif order == 1:
+            gammas = [1]
+            sigma /= np.sqrt(3)
+        elif order == 2:
+            gammas = [1, 1 / 3.0]
+            sigma /= np.sqrt(5)
+        else:
+            raise SystemExit
+        return sargan_kernel(A, B, sigma, gammas)
+    if metric != "l2":
+        raise SystemExit
+    A, B, K = _prep(A, B)
+    lib().oracle_matern_l2(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], float(sigma),
+                           int(order), _ptr(K))
+    return K
+
+
+def p_distance(A, B, p=2):
+    """qml/distance.py:94-147 dispatch (int / integer-valued float / float p)."""
+    if type(p) == type(1.0) and p.is_integer():
+        p = int(p)
+    if type(p) == type(1):
+        if p == 2:
+            return l2_distance(A, B)
+        A, B, D = _prep(A, B)
+        lib().oracle_p_distance(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], float(p),
+                                1, _ptr(D))
+        return D
+    if type(p) == type(1.0):
+        A, B, D = _prep(A, B)
+        lib().oracle_p_distance(_ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1], p, 0,
+                                _ptr(D))
+        return D
+    raise ValueError("expected exponent of integer or float type")
+
+
+REF_EXT_KIND = {"linear": 0, "sargan": 1, "matern_l2": 2, "p_int": 3, "p_real": 4}
+
+
+def ref_kernel_ext(kind, A, B, sigma=1.0, gammas=(), order=0, p=2.0):
+    """The reference's flinear_kernel / fsargan_kernel / fmatern_kernel_l2 / fp_distance_*."""
+    A, B, K = _prep(A, B)
+    g = np.ascontiguousarray(np.asarray(gammas, dtype=np.float64).reshape(-1))
+    if g.size == 0:
+        g = np.zeros(1)
+        ng = 0
+    else:
+        ng = g.size
+    ref().ref_kernel_ext(REF_EXT_KIND[kind], _ptr(A), A.shape[0], _ptr(B), B.shape[0],
+                         A.shape[1], _ptr(K), float(sigma), _ptr(g), ng, int(order), float(p),
+                         int(p))
     return K
 
 

@@ -383,3 +383,87 @@ void oracle_local_kernels(int kind, const double *A, const int64_t *offA, int64_
     }
     free(inv);
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Other global kernels and the p-distance: qml/fkernels.f90:403-531, qml/fdistance.f90:83-155.
+ * Same conventions as above (rows of A, B; K column-major (na, nb)).
+ * ------------------------------------------------------------------------------------------ */
+
+/* flinear_kernel (403-424): K(j, i) = dot_product(a_j, b_i), a serial sum. */
+void oracle_linear(const double *A, int64_t na, const double *B, int64_t nb, int64_t d,
+                   double *K) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nb; ++i)
+        for (int64_t j = 0; j < na; ++j) {
+            double s = 0.0;
+            for (int64_t k = 0; k < d; ++k) s = s + A[j * d + k] * B[i * d + k];
+            K[j + i * na] = s;
+        }
+}
+
+/* Fortran x**m for an integer m >= 1 by binary powering. */
+static double powi_f(double x, int m) {
+    double r = (m & 1) ? x : 1.0;
+    while (m >>= 1) {
+        x = x * x;
+        if (m & 1) r = r * x;
+    }
+    return r;
+}
+
+/* fsargan_kernel (491-531): d = sum |a - b|; prefactor(m) = gamma_m (-inv d)**m;
+ * K = exp(inv d) (1 + sum(prefactor)), inv = -1/sigma. */
+void oracle_sargan(const double *A, int64_t na, const double *B, int64_t nb, int64_t d,
+                   double sigma, const double *gammas, int64_t ng, double *K) {
+    double inv = -1.0 / sigma;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nb; ++i)
+        for (int64_t j = 0; j < na; ++j) {
+            double s = 0.0;
+            for (int64_t k = 0; k < d; ++k) s = s + fabs(A[j * d + k] - B[i * d + k]);
+            double pre = 0.0;
+            for (int64_t m = 0; m < ng; ++m) pre = pre + gammas[m] * powi_f(-inv * s, (int)(m + 1));
+            K[j + i * na] = exp(inv * s) * (1 + pre);
+        }
+}
+
+/* fmatern_kernel_l2 (427-488): d2 = sum (a - b)^2, d = sqrt(d2); order 0: exp(inv d),
+ * inv = -1/sigma; 1: exp(inv d)(1 - inv d), inv = -sqrt(3)/sigma; otherwise
+ * exp(inv d)(1 - inv d + inv2 d2), inv = -sqrt(5)/sigma, inv2 = 5/(3 sigma^2). */
+void oracle_matern_l2(const double *A, int64_t na, const double *B, int64_t nb, int64_t d,
+                      double sigma, int order, double *K) {
+    double inv = order == 0 ? -1.0 / sigma : order == 1 ? -sqrt(3.0) / sigma : -sqrt(5.0) / sigma;
+    double inv2 = 5.0 / (3.0 * sigma * sigma);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nb; ++i)
+        for (int64_t j = 0; j < na; ++j) {
+            double s = 0.0;
+            for (int64_t k = 0; k < d; ++k) {
+                double t = A[j * d + k] - B[i * d + k];
+                s = s + t * t;
+            }
+            double dd = sqrt(s);
+            double v;
+            if (order == 0) v = exp(inv * dd);
+            else if (order == 1) v = exp(inv * dd) * (1.0 - inv * dd);
+            else v = exp(inv * dd) * (1.0 - inv * dd + inv2 * s);
+            K[j + i * na] = v;
+        }
+}
+
+/* fp_distance_integer (121-155) when p_is_int, else fp_distance_double (83-118):
+ * D = (sum |a - b|**p)**(1/p). */
+void oracle_p_distance(const double *A, int64_t na, const double *B, int64_t nb, int64_t d,
+                       double p, int p_is_int, double *D) {
+    double inv_p = 1.0 / p;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nb; ++i)
+        for (int64_t j = 0; j < na; ++j) {
+            double s = 0.0;
+            for (int64_t k = 0; k < d; ++k) {
+                double t = fabs(A[j * d + k] - B[i * d + k]);
+                s = s + (p_is_int ? powi_f(t, (int)p) : pow(t, p));
+            }
+            D[j + i * na] = pow(s, inv_p);
+        }
+}

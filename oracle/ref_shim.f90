@@ -189,3 +189,67 @@ subroutine ref_local_kernels(kind, q1, na, q2, nb, d, n1, nm1, n2, nm2, sigmas, 
   deallocate(m1)
   deallocate(m2)
 end subroutine
+
+subroutine ref_kernel_ext(kind, a, na, b, nb, d, k, sigma, gammas, ng, order, p, ip) &
+    bind(C, name="ref_kernel_ext")
+  ! kind: 0 linear (flinear_kernel), 1 Sargan (fsargan_kernel), 2 Matern L2
+  ! (fmatern_kernel_l2), 3 integer p-distance (fp_distance_integer), 4 real p-distance
+  ! (fp_distance_double). a, b are C-ordered (na, d) / (nb, d); k is (na, nb) column-major.
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: kind, na, nb, d, ng, order, ip
+  real(c_double), intent(in) :: a(d, na), b(d, nb), gammas(*)
+  real(c_double), intent(inout) :: k(na, nb)
+  real(c_double), value :: sigma, p
+  interface
+    subroutine flinear_kernel(a, na, b, nb, k)
+      double precision, dimension(:,:), intent(in) :: a, b
+      integer, intent(in) :: na, nb
+      double precision, dimension(:,:), intent(inout) :: k
+    end subroutine
+    subroutine fsargan_kernel(a, na, b, nb, k, sigma, gammas, ng)
+      double precision, dimension(:,:), intent(in) :: a, b
+      double precision, dimension(:), intent(in) :: gammas
+      integer, intent(in) :: na, nb, ng
+      double precision, dimension(:,:), intent(inout) :: k
+      double precision, intent(in) :: sigma
+    end subroutine
+    subroutine fmatern_kernel_l2(a, na, b, nb, k, sigma, order)
+      double precision, dimension(:,:), intent(in) :: a, b
+      integer, intent(in) :: na, nb
+      double precision, dimension(:,:), intent(inout) :: k
+      double precision, intent(in) :: sigma
+      integer, intent(in) :: order
+    end subroutine
+    subroutine fp_distance_integer(a, b, d, p)
+      double precision, dimension(:,:), intent(in) :: a, b
+      double precision, dimension(:,:), intent(inout) :: d
+      integer, intent(in) :: p
+    end subroutine
+    subroutine fp_distance_double(a, b, d, p)
+      double precision, dimension(:,:), intent(in) :: a, b
+      double precision, dimension(:,:), intent(inout) :: d
+      double precision, intent(in) :: p
+    end subroutine
+  end interface
+  integer :: n1, n2, o, pi
+  double precision :: s, pp
+  n1 = na
+  n2 = nb
+  s = sigma
+  o = order
+  pi = ip
+  pp = p
+  select case (kind)
+  case (0)
+    call flinear_kernel(a, n1, b, n2, k)
+  case (1)
+    call fsargan_kernel(a, n1, b, n2, k, s, gammas(1:ng), ng)
+  case (2)
+    call fmatern_kernel_l2(a, n1, b, n2, k, s, o)
+  case (3)
+    call fp_distance_integer(a, b, k, pi)
+  case (4)
+    call fp_distance_double(a, b, k, pp)
+  end select
+end subroutine

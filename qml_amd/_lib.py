@@ -17,6 +17,7 @@ LIB_PATH = os.environ.get("QMLH_LIBRARY", os.path.join(_HERE, "libqmlhip.so"))
 
 QMLH_OK = 0
 PAIR_LAPLACIAN, PAIR_GAUSSIAN, PAIR_MANHATTAN, PAIR_L2 = 0, 1, 2, 3
+PAIR_SARGAN, PAIR_MATERN_L2, PAIR_P_INT, PAIR_P_REAL = 4, 5, 6, 7
 SYMMETRIC = 1
 CM_MAX_SIZE = 256
 ACM_MAX_ATOMS = 256
@@ -61,6 +62,10 @@ _SIGNATURES = {
     "qmlh_pairwise": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _f64, _vp,
                                      _i64, _i32, _vp]),
     "qmlh_row_sqnorms": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp]),
+    "qmlh_pairwise_ext": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i32,
+                                         _vp, _i64, _i32, _vp]),
+    "qmlh_linear_mfma": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i32,
+                                        _vp]),
     "qmlh_gaussian_mfma": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64,
                                           _f64, _vp, _i64, _i32, _vp]),
     "qmlh_potrf": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),

@@ -115,6 +115,46 @@ __global__ __launch_bounds__(256, 2) void gaussian_mfma_kernel(
     }
 }
 
+// Linear kernel K(i, j) = a_i . b_j (qml/fkernels.f90:403-424) from the same cross-term tiles;
+// symmetric mode computes the upper tiles and mirrors.
+__global__ __launch_bounds__(256, 2) void linear_mfma_kernel(
+    const double* __restrict__ A, int64_t lda, int64_t na, const double* __restrict__ B,
+    int64_t ldb, int64_t nb, int64_t d, double* __restrict__ K, int64_t ldk, int symmetric,
+    int64_t tiles_m) {
+    __shared__ __attribute__((aligned(16))) double lds[WG_LDS_ELEMS];
+    int64_t bi, bj;
+    tile_coords(blockIdx.x, symmetric, tiles_m, bi, bj);
+    Operand<double, LAYOUT_ROW> opA{A, lda, na, d};
+    Operand<double, LAYOUT_ROW> opB{B, ldb, nb, d};
+    WaveTile<double> t;
+    wave_tile_zero(t);
+    wg_tile_mma_lds(t, lds, opB, opA, bj * GK_TILE, bi * GK_TILE, d);
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t col0 = bj * GK_TILE + 64 * (w & 1);
+    const int64_t row0 = bi * GK_TILE + 64 * (w >> 1);
+    if (row0 >= na || col0 >= nb) return;
+    const bool diag_tile = symmetric && bi == bj;
+    const bool mirror = symmetric && bi != bj;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + (lane & 15);
+            if (i >= na) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + tile_row<double>(a, e);
+                if (j >= nb) continue;
+                if (diag_tile && i > j) continue;
+                const double v = t.acc[a][b][e];
+                K[i + j * ldk] = v;
+                if (mirror || (diag_tile && i < j)) K[j + i * ldk] = v;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256, 2) void gaussian_deflated_kernel(
     const double* __restrict__ X, int64_t ldx, int64_t n, const double* __restrict__ nrm,
     const double* __restrict__ u, int64_t d, double scale, double lambda, double shift,
@@ -211,6 +251,32 @@ extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, cons
                        (hipStream_t)stream, A, lda, na, normsA, B, ldb, nb, normsB, d,
                        -0.5 / (sigma * sigma), K, ldk, sym ? 1 : 0, tm);
     QMLH_CHECK_LAUNCH("gaussian_mfma launch");
+    return QMLH_OK;
+}
+
+extern "C" int qmlh_linear_mfma(const double* A, int64_t na, int64_t lda, const double* B,
+                                int64_t nb, int64_t ldb, int64_t d, double* K, int64_t ldk,
+                                int32_t flags, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (na < 0 || nb < 0 || d < 0) return set_error(QMLH_ERR_ARG, "linear_mfma: negative size");
+    if (lda % 4 || ldb % 4) return set_error(QMLH_ERR_ARG, "linear_mfma: lda, ldb must be multiples of 4");
+    if (lda < d || ldb < d) return set_error(QMLH_ERR_ARG, "linear_mfma: ld < d");
+    if (ldk < na) return set_error(QMLH_ERR_ARG, "linear_mfma: ldk < na");
+    if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15)
+        return set_error(QMLH_ERR_ARG, "linear_mfma: A and B must be 16-byte aligned");
+    const bool sym = (flags & QMLH_SYMMETRIC) != 0;
+    if (sym && na != nb) return set_error(QMLH_ERR_ARG, "linear_mfma: symmetric needs na == nb");
+    if (na == 0 || nb == 0) return QMLH_OK;
+    if (!A || !B || !K) return set_error(QMLH_ERR_ARG, "linear_mfma: null pointer");
+    const int64_t tm = ceil_div(na, GK_TILE), tn = ceil_div(nb, GK_TILE);
+    const int64_t ntiles = sym ? tm * (tm + 1) / 2 : tm * tn;
+    if (ntiles > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "linear_mfma: grid too large");
+    const double work = 2.0 * (double)d * (sym ? (double)na * (double)(na + 1) / 2.0
+                                               : (double)na * (double)nb);
+    TimerScope ts(QMLH_TIMER_GAUSS, work, (hipStream_t)stream);
+    hipLaunchKernelGGL(linear_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0,
+                       (hipStream_t)stream, A, lda, na, B, ldb, nb, d, K, ldk, sym ? 1 : 0, tm);
+    QMLH_CHECK_LAUNCH("linear_mfma launch");
     return QMLH_OK;
 }
 

@@ -18,23 +18,70 @@ static constexpr int PT_BK = 16;
 static constexpr int PT_LDS = PT_BM + 2;  // padded k-major row (doubles), keeps 16 B alignment
 static constexpr int PT_THREADS = 256;
 
-enum { OP_L1 = 0, OP_SQ = 1 };
-enum { EPI_IDENTITY = 0, EPI_SQRT = 1, EPI_EXP = 2 };
+// Per-feature terms: |a - b|, (a - b)^2, |a - b|^3 and |a - b|^4 by the products binary
+// powering forms (x (x x), (x x)(x x)), |a - b|^p for any other p (pow).
+enum { OP_L1 = 0, OP_SQ = 1, OP_POW3 = 2, OP_POW4 = 3, OP_POWR = 4 };
+// Epilogues of the feature sum s: s, sqrt(s), exp(s0 s), the Sargan polynomial-times-exp in
+// the L1 distance, the Matern forms in the L2 distance, s^s0 (the p-norm root).
+enum { EPI_IDENTITY = 0, EPI_SQRT = 1, EPI_EXP = 2, EPI_SARGAN = 3, EPI_MATERN = 4, EPI_ROOT = 5 };
+
+static constexpr int PAIR_MAX_GAMMAS = 16;
+
+struct PairEpi {
+    double s0, s1;                 // scales (see the epilogues)
+    double g[PAIR_MAX_GAMMAS];     // Sargan coefficients gamma_1 ..
+    int ng;                        // their count
+    int order;                     // Matern order 0, 1, 2
+};
+
+// x^n for n >= 0 by binary powering.
+__device__ __forceinline__ double pair_powi(double x, int n) {
+    double r = (n & 1) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1) r = r * x;
+    }
+    return r;
+}
 
 template <int OP>
-__device__ __forceinline__ double pair_term(double acc, double a, double b) {
+__device__ __forceinline__ double pair_term(double acc, double a, double b, double p = 0.0) {
     if (OP == OP_L1) {
         return acc + fabs(a - b);
-    } else {
+    } else if (OP == OP_SQ) {
         double t = a - b;
         return fma(t, t, acc);
+    } else if (OP == OP_POW3) {
+        const double x = fabs(a - b);
+        return acc + x * (x * x);
+    } else if (OP == OP_POW4) {
+        const double x = fabs(a - b);
+        const double x2 = x * x;
+        return acc + x2 * x2;
+    } else {
+        return acc + pow(fabs(a - b), p);
     }
 }
 
 template <int EPI>
-__device__ __forceinline__ double pair_epilogue(double acc, double scale) {
-    if (EPI == EPI_EXP) return exp(scale * acc);
+__device__ __forceinline__ double pair_epilogue(double acc, const PairEpi& e) {
+    if (EPI == EPI_EXP) return exp(e.s0 * acc);
     if (EPI == EPI_SQRT) return sqrt(acc);
+    if (EPI == EPI_ROOT) return pow(acc, e.s0);
+    if (EPI == EPI_SARGAN) {
+        // qml/fkernels.f90:513-518: sum_m gamma_m (-inv d)^m, then exp(inv d) (1 + sum).
+        const double x = -e.s0 * acc;
+        double sum = 0.0;
+        for (int m = 0; m < e.ng; ++m) sum = sum + e.g[m] * pair_powi(x, m + 1);
+        return exp(e.s0 * acc) * (1 + sum);
+    }
+    if (EPI == EPI_MATERN) {
+        // qml/fkernels.f90:442-482 with acc the squared distance.
+        const double d = sqrt(acc);
+        if (e.order == 0) return exp(e.s0 * d);
+        if (e.order == 1) return exp(e.s0 * d) * (1.0 - e.s0 * d);
+        return exp(e.s0 * d) * (1.0 - e.s0 * d + e.s1 * acc);
+    }
     return acc;
 }
 
@@ -64,7 +111,7 @@ __device__ __forceinline__ void pair_tile_accumulate(double (&acc)[8][8], double
                                                      const double* __restrict__ A, int64_t lda,
                                                      int64_t na, const double* __restrict__ B,
                                                      int64_t ldb, int64_t nb, int64_t d,
-                                                     int64_t row0, int64_t col0) {
+                                                     int64_t row0, int64_t col0, double p = 0.0) {
     const int t = threadIdx.x;
     const int tx = t % 16;
     const int ty = t / 16;
@@ -92,7 +139,8 @@ __device__ __forceinline__ void pair_tile_accumulate(double (&acc)[8][8], double
 #pragma unroll
             for (int r = 0; r < 8; ++r)
 #pragma unroll
-                for (int c = 0; c < 8; ++c) acc[r][c] = pair_term<OP>(acc[r][c], a[r], b[c]);
+                for (int c = 0; c < 8; ++c)
+                    acc[r][c] = pair_term<OP>(acc[r][c], a[r], b[c], p);
         }
         __syncthreads();
     }

@@ -1,8 +1,9 @@
-"""Laplacian and Gaussian kernel matrices on the GPU.
+"""Kernel matrices on the GPU.
 
-``laplacian_kernel`` and ``gaussian_kernel`` keep the signatures and results of
-qml.kernels (qml/kernels.py:36-92): inputs (na, d) and (nb, d), output the Fortran-ordered
-(na, nb) FP64 array. The ``*_device`` forms take and return device tensors.
+``laplacian_kernel``, ``gaussian_kernel``, ``linear_kernel``, ``sargan_kernel`` and
+``matern_kernel`` keep the signatures and results of qml.kernels (qml/kernels.py:36-216):
+inputs (na, d) and (nb, d), output the Fortran-ordered (na, nb) FP64 array. The local
+(atom-summed) kernels are below. The ``*_device`` forms take and return device tensors.
 """
 
 from __future__ import annotations
@@ -111,6 +112,93 @@ def gaussian_kernel(A, B, sigma):
     else:
         Kd = pairwise_device(_lib.PAIR_GAUSSIAN, Ad, na, Bd, nb, d, sigma, symmetric)
     return _to_host_fortran(Kd)
+
+
+def pairwise_ext_device(kind: int, Ad, na: int, Bd, nb: int, d: int, params, symmetric=False,
+                        out=None, stream=None):
+    """Parameterised VALU pairwise kernels (Sargan, Matern L2, p-distances); returns the
+    (nb, na) tensor = the column-major (na, nb) matrix."""
+    prm = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+    K = _empty_kernel_device(na, nb) if out is None else out
+    _lib.check(_lib.lib().qmlh_pairwise_ext(
+        kind, _lib.ptr(Ad), na, Ad.shape[1], _lib.ptr(Bd), nb, Bd.shape[1], d, prm.ctypes.data,
+        prm.size, _lib.ptr(K), na, _lib.SYMMETRIC if symmetric else 0,
+        _lib.stream_handle(stream)))
+    return K
+
+
+def _rows_pair(A, B):
+    symmetric = A is B
+    A = np.asarray(A, dtype=np.float64)
+    B = A if symmetric else np.asarray(B, dtype=np.float64)
+    return A, B, symmetric
+
+
+def linear_kernel(A, B):
+    """K_ij = A_i . B_j on FP64 MFMA; (na, nb), Fortran order (qml/kernels.py:94-115).
+    Divergence: representations of different sizes raise ValueError (the reference does not
+    check them)."""
+    _check_pair(A, B)
+    A, B, symmetric = _rows_pair(A, B)
+    na, nb, d = A.shape[0], B.shape[0], A.shape[1]
+    if na == 0 or nb == 0:
+        return np.empty((na, nb), order="F")
+    Ad = rows_to_device(A)
+    Bd = Ad if symmetric else rows_to_device(B)
+    K = _empty_kernel_device(na, nb)
+    _lib.check(_lib.lib().qmlh_linear_mfma(
+        _lib.ptr(Ad), na, Ad.shape[1], _lib.ptr(Bd), nb, Bd.shape[1], d, _lib.ptr(K), na,
+        _lib.SYMMETRIC if symmetric else 0, _lib.stream_handle(None)))
+    return _to_host_fortran(K)
+
+
+def _ext_kernel(kind, A, B, params):
+    _check_pair(A, B)
+    A, B, symmetric = _rows_pair(A, B)
+    na, nb, d = A.shape[0], B.shape[0], A.shape[1]
+    if na == 0 or nb == 0:
+        return np.empty((na, nb), order="F")
+    Ad = rows_to_device(A)
+    Bd = Ad if symmetric else rows_to_device(B)
+    return _to_host_fortran(pairwise_ext_device(kind, Ad, na, Bd, nb, d, params, symmetric))
+
+
+def sargan_kernel(A, B, sigma, gammas):
+    """K_ij = exp(-|A_i - B_j|_1 / sigma) (1 + sum_k gamma_k |A_i - B_j|_1^k / sigma^k)
+    (qml/kernels.py:117-150); no gammas gives the Laplacian kernel."""
+    ng = len(gammas)
+    if ng == 0:
+        return laplacian_kernel(A, B, sigma)
+    return _ext_kernel(_lib.PAIR_SARGAN, A, B,
+                       [float(sigma)] + [float(g) for g in np.asarray(gammas).reshape(-1)])
+
+
+def matern_kernel(A, B, sigma, order=0, metric="l1"):
+    """Matern kernel of order 0, 1, 2 in the L1 or L2 distance (qml/kernels.py:152-216): the L1
+    form is a Sargan kernel with gammas (), (1), (1, 1/3) and sigma / sqrt(3), sigma / sqrt(5);
+    the L2 form follows fmatern_kernel_l2 (orders other than 0 and 1 take the order-2 form,
+    as its Fortran does). Unknown orders (L1) and metrics print the reference's message and
+    raise SystemExit."""
+    if metric == "l1":
+        if order == 0:
+            gammas = []
+        elif order == 1:
+            gammas = [1]
+            sigma /= np.sqrt(3)
+        elif order == 2:
+            gammas = [1, 1 / 3.0]
+            sigma /= np.sqrt(5)
+        else:
+            print("Order:%d not implemented in Matern Kernel" % order)
+            raise SystemExit
+        return sargan_kernel(A, B, sigma, gammas)
+    elif metric == "l2":
+        pass
+    else:
+        print("Error: Unknown distance metric %s in Matern kernel" % str(metric))
+        raise SystemExit
+    return _ext_kernel(_lib.PAIR_MATERN_L2, A, B,
+                       [float(sigma), float(order if order in (0, 1) else 2)])
 
 
 # --------------------------------------------------------------------------------------------

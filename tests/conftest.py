@@ -72,6 +72,12 @@ def get_energies(text):
 
 
 @pytest.fixture(scope="session")
+def ref_kernels_ext():
+    z = np.load(os.path.join(GOLDEN, "ref_kernels_ext.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
 def cho_pair():
     z = np.load(os.path.join(GOLDEN, "cho_solve_pair.npz"), allow_pickle=False)
     return z["K"], z["y"]

@@ -26,6 +26,8 @@ Outputs (data only):
                         tests/data/compound_test.xyz / .exyz (tests/test_compound.py) and of
                         tests/data/hof_qm7.txt, for the XYZ reader and the end-to-end test on the
                         reference's on-disk format.
+  ref_kernels_ext.npz   outputs of the reference Fortran (oracle/_ref) for the linear, Sargan,
+                        Matern (L2) kernels and the integer / real p-distances on random rows.
   ref_local_vectors.npz outputs of the reference Fortran (oracle/_ref) on seeded synthetic
                         molecules: atomic Coulomb matrices (both orderings, with and without
                         cut-offs) and local kernels (two sigmas).
@@ -154,6 +156,27 @@ def xyz_texts():
                         texts=np.array([t.encode() for t in texts]), **extra)
 
 
+def ref_kernels_ext():
+    os.environ["OMP_NUM_THREADS"] = "1"
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    rng = np.random.default_rng(77)
+    A = rng.random((37, 61)) * 4.0
+    B = rng.random((29, 61)) * 4.0
+    out = dict(A=A, B=B, sigma=13.0, gammas=np.array([0.4, 1.1, 0.25]))
+    out["linear"] = oracle.ref_kernel_ext("linear", A, B)
+    for ng in (1, 2, 3):
+        out[f"sargan_{ng}"] = oracle.ref_kernel_ext("sargan", A, B, 13.0, out["gammas"][:ng])
+    for order in (0, 1, 2):
+        out[f"matern_l2_{order}"] = oracle.ref_kernel_ext("matern_l2", A, B, 13.0, order=order)
+    for p in (1, 3, 5):
+        out[f"p_int_{p}"] = oracle.ref_kernel_ext("p_int", A, B, p=p)
+    for p in (0.5, 2.5):
+        out[f"p_real_{p}"] = oracle.ref_kernel_ext("p_real", A, B, p=p)
+    np.savez_compressed(os.path.join(HERE, "ref_kernels_ext.npz"), **out)
+
+
 def ref_vectors():
     os.environ["OMP_NUM_THREADS"] = "1"
     sys.path.insert(0, ROOT)
@@ -185,4 +208,5 @@ if __name__ == "__main__":
     local_fixtures()
     ref_local_vectors()
     xyz_texts()
+    ref_kernels_ext()
     print("golden fixtures written to", HERE)
