@@ -502,6 +502,49 @@ def run_invert(n, cpu_n, log):
                                "sample": f"LAPACK dpotrf + dpotri (scipy) at n={cpu_n}: "
                                          f"{t1 - t0:.2f}s on {cores} threads; extrapolated as n^3"}
     log(f"cho_invert n={n}: {ms:.1f} ms ({out['tflops_lapack_equiv']:.1f} TF/s LAPACK-equivalent)")
+    # Bunch-Kaufman (bkf_solve / bkf_invert): one-workgroup factorisation, per-column solves, on
+    # a symmetric indefinite matrix of order 2000, against scipy's dsytrf + dsytrs at the same n.
+    from qml_amd.math import bkf_factor_device, bkf_solve_device
+
+    nb = 2000
+    rng = np.random.default_rng(1)
+    M = rng.standard_normal((nb, nb))
+    Ah = M + M.T
+    A0 = torch.from_numpy(np.ascontiguousarray(Ah.T)).cuda()
+    Ab = A0.clone()
+    bkf_factor_device(Ab, nb)
+    torch.cuda.synchronize()
+    Ab.copy_(A0)
+    e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+    e0.record()
+    ipiv, _ = bkf_factor_device(Ab, nb)
+    e1.record()
+    yb = torch.from_numpy(rng.standard_normal(nb)).cuda()
+    e2.record()
+    bkf_solve_device(Ab, nb, ipiv, yb, 1)
+    Xb = torch.eye(nb, dtype=torch.float64, device=Ab.device)
+    e3.record()
+    bkf_solve_device(Ab, nb, ipiv, Xb, nb)
+    e4 = torch.cuda.Event(enable_timing=True)
+    e4.record()
+    e4.synchronize()
+    bk = {"n": nb, "factor_ms": e0.elapsed_time(e1), "solve_ms": e2.elapsed_time(e3),
+          "inverse_solves_ms": e3.elapsed_time(e4),
+          "factor_gflops": nb ** 3 / 3.0 / (e0.elapsed_time(e1) * 1e-3) / 1e9,
+          "note": "one workgroup walks the columns (pivot search depends on every update)"}
+    from scipy.linalg import lapack
+
+    t0 = time.perf_counter()
+    lu, piv, _ = lapack.dsytrf(np.asfortranarray(Ah), lower=1)[:3]
+    t1 = time.perf_counter()
+    lapack.dsytrs(lu, piv, np.eye(nb), lower=1)
+    t2 = time.perf_counter()
+    bk["cpu_lapack_ms"] = {"dsytrf": (t1 - t0) * 1e3, "dsytrs_identity": (t2 - t1) * 1e3,
+                           "cores": int(os.environ.get("OMP_NUM_THREADS",
+                                                       len(os.sched_getaffinity(0))))}
+    out["bunch_kaufman"] = bk
+    log(f"bunch-kaufman n={nb}: factor {bk['factor_ms']:.1f} ms (LAPACK "
+        f"{bk['cpu_lapack_ms']['dsytrf']:.1f} ms), inverse solves {bk['inverse_solves_ms']:.1f} ms")
     return out
 
 

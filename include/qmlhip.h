@@ -191,6 +191,21 @@ size_t qmlh_cho_invert_workspace(int64_t n);
 int qmlh_cho_invert(double *A, int64_t n, int64_t lda, int32_t *info, void *ws, size_t ws_bytes,
                     qmlh_stream_t stream);
 
+/* Symmetric indefinite factorisation with Bunch-Kaufman pivoting, A = L D L^T (lower), and
+ * solves with it. Replace LAPACK dsytrf("L") / dsytrs("L") in fbkf_solve and fbkf_invert
+ * (qml/fcho_solve.f90:76-150) behind qml/math.py:100-164 bkf_invert / bkf_solve (whose inverse
+ * here is the solve with the identity). qmlh_bkf_factor overwrites the lower triangle of the
+ * column-major A with L and D as dsytrf does, writes the 1-based pivots to ipiv (device int32,
+ * n; negative for both columns of a 2 x 2 block) and dsytrf's info (first zero pivot, or 0);
+ * one workgroup walks the columns, so it suits orders up to a few thousand. ws: device
+ * workspace of qmlh_bkf_workspace(n) bytes. qmlh_bkf_solve: B (n x nrhs column-major, ldb)
+ * := A^-1 B with those factors, one workgroup per column. */
+size_t qmlh_bkf_workspace(int64_t n);
+int qmlh_bkf_factor(double *A, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info, void *ws,
+                    size_t ws_bytes, qmlh_stream_t stream);
+int qmlh_bkf_solve(const double *A, int64_t n, int64_t lda, const int32_t *ipiv, double *B,
+                   int64_t ldb, int64_t nrhs, qmlh_stream_t stream);
+
 /* ---------------------------------------------------------------------------------------- *
  * Mixed precision (no single reference counterpart: together these replace the FP64
  * dpotrf/dpotrs pair of qml/fcho_solve.f90:35,46 in the KRR fit at FP64 accuracy).

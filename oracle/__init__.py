@@ -433,6 +433,47 @@ def cho_invert(A):
     return I
 
 
+def bkf_solve(A, y):
+    """qml/math.py:131-164 around qml/fcho_solve.f90:115-150: LAPACK dsytrf('L') + dsytrs."""
+    from scipy.linalg import lapack
+
+    n = A.shape[0]
+    A_diag = A[np.diag_indices_from(A)]
+    lu, piv, info = lapack.dsytrf(np.asfortranarray(A, dtype=np.float64), lower=1)[:3]
+    if info > 0:
+        print(" WARNING: Bunch-Kaufman factorization DSYTRI() exited with error code:", info)
+    x, _ = lapack.dsytrs(lu, piv, np.array(y, dtype=np.float64), lower=1)
+    A[np.diag_indices_from(A)] = A_diag
+    i_lower = np.tril_indices_from(A)
+    A.T[i_lower] = A[i_lower]
+    return x.reshape(n)
+
+
+def bkf_invert(A):
+    """qml/math.py:100-128 around qml/fcho_solve.f90:76-112: dsytrf('L') + dsytri('L') on the
+    Fortran-ordered array, then lower -> upper copy. scipy does not expose dsytri; the inverse
+    here is dsytrs with the identity on the same factors (equal to dsytri's to rounding), of
+    which the lower triangle is kept, as the reference keeps dsytri's."""
+    from scipy.linalg import lapack
+
+    I = np.asfortranarray(A)
+    n = I.shape[0]
+    lu, piv, info = lapack.dsytrf(np.array(I, dtype=np.float64, order="F"), lower=1)[:3]
+    if info > 0:
+        print(" WARNING: Bunch-Kaufman factorization DSYTRI() exited with error code:", info)
+    X, _ = lapack.dsytrs(lu, piv, np.eye(n), lower=1)
+    I[...] = np.tril(X) + np.tril(X, -1).T
+    return I
+
+
+def bkf_factor(A):
+    """LAPACK dsytrf('L') of A (a copy): (lower factors, 1-based pivots, info)."""
+    from scipy.linalg import lapack
+
+    lu, piv, info = lapack.dsytrf(np.asfortranarray(A, dtype=np.float64), lower=1)[:3]
+    return np.tril(lu), piv, info
+
+
 def krr_fit_predict(X, y, Xs, kernel, sigma, lam):
     """KRR composition of tests/test_energy_krr_cmat.py:101-111: returns (alpha, y_pred)."""
     kern = laplacian_kernel if kernel == "laplacian" else gaussian_kernel

@@ -73,6 +73,9 @@ _SIGNATURES = {
     "qmlh_potrs": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "qmlh_cho_invert_workspace": (_sz, [_i64]),
     "qmlh_cho_invert": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "qmlh_bkf_workspace": (_sz, [_i64]),
+    "qmlh_bkf_factor": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _sz, _vp]),
+    "qmlh_bkf_solve": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp]),
     "qmlh_spotrf": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
     "qmlh_spotrf_f16x3_workspace": (_sz, [_i64]),
     "qmlh_spotrf_f16x3": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _sz, _vp]),

@@ -8,6 +8,8 @@ onto the upper one.
 cho_invert: the reference makes a Fortran-ordered array I of A (a copy unless A already is one,
 in which case A itself is overwritten), runs dpotrf("L") + dpotri("L") on it
 (qml/fcho_solve.f90:54-73) and copies I's lower triangle onto the upper one.
+bkf_solve / bkf_invert: the same with LAPACK's Bunch-Kaufman dsytrf("L") + dsytrs / dsytri
+(qml/fcho_solve.f90:76-150), for symmetric indefinite matrices.
 """
 
 from __future__ import annotations
@@ -128,4 +130,89 @@ def cho_invert(A):
     code = int(info.item())
     if code > 0:
         print(f" WARNING: Cholesky decomposition DPOTRF() exited with error code:{code:12d}")
+    return I
+
+
+def bkf_factor_device(Ad, n: int, stream=None):
+    """In place: Bunch-Kaufman L D L^T of the column-major (n, n) device matrix (lower triangle
+    read). Returns (ipiv, info) device tensors (1-based pivots, as LAPACK)."""
+    import torch
+
+    L = _lib.lib()
+    ipiv = torch.zeros(max(1, n), dtype=torch.int32, device=Ad.device)
+    info = torch.zeros(1, dtype=torch.int32, device=Ad.device)
+    nbytes = L.qmlh_bkf_workspace(n)
+    ws = torch.empty(max(1, (nbytes + 7) // 8), dtype=torch.float64, device=Ad.device)
+    _lib.check(L.qmlh_bkf_factor(_lib.ptr(Ad), n, n, _lib.ptr(ipiv), _lib.ptr(info),
+                                 _lib.ptr(ws), ws.numel() * 8, _lib.stream_handle(stream)))
+    return ipiv, info
+
+
+def bkf_solve_device(Ad, n: int, ipiv, Bd, nrhs: int, stream=None):
+    """Bd (nrhs rows of n: column-major n x nrhs) := A^-1 Bd with the factors of
+    bkf_factor_device."""
+    _lib.check(_lib.lib().qmlh_bkf_solve(_lib.ptr(Ad), n, n, _lib.ptr(ipiv), _lib.ptr(Bd), n,
+                                         nrhs, _lib.stream_handle(stream)))
+    return Bd
+
+
+def _print_bkf_warning(info: int, inverse: bool) -> None:
+    # qml/fcho_solve.f90:100-108 and 137-147 (Fortran list-directed output).
+    if info > 0:
+        print(f" WARNING: Bunch-Kaufman factorization DSYTRI() exited with error code:{info:12d}")
+        if inverse:
+            print(f" WARNING: BKF inversion DPOTRI() exited with error code:{info:12d}")
+
+
+def bkf_solve(A, y):
+    """Solve A x = y for a symmetric (possibly indefinite) A by Bunch-Kaufman on the GPU; same
+    signature, errors and side effects as qml.math.bkf_solve (qml/math.py:131-164): A's lower
+    triangle is read, its diagonal restored and its lower triangle copied onto the upper one."""
+    import torch
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError('expected square matrix')
+    if len(y.shape) != 1 or y.shape[0] != A.shape[1]:
+        raise ValueError('expected matrix and vector of same stride size')
+    n = A.shape[0]
+    A_diag = A[np.diag_indices_from(A)]
+    x = np.zeros(n)
+    if n == 0:
+        return x
+    # Column-major bytes of A's Fortran view: A.T in C order.
+    Ad = torch.from_numpy(np.ascontiguousarray(np.asarray(A, dtype=np.float64).T)).to(device())
+    ipiv, info = bkf_factor_device(Ad, n)
+    xd = torch.from_numpy(np.array(y, dtype=np.float64)).to(device())
+    bkf_solve_device(Ad, n, ipiv, xd, 1)
+    x[:n] = xd.cpu().numpy()
+    _print_bkf_warning(int(info.item()), False)
+    A[np.diag_indices_from(A)] = A_diag
+    _copy_lower_to_upper(A)
+    return x
+
+
+def bkf_invert(A):
+    """Inverse of a symmetric (possibly indefinite) matrix by Bunch-Kaufman on the GPU; same
+    signature, result layout (Fortran order) and aliasing as qml.math.bkf_invert
+    (qml/math.py:100-128). The inverse is the factorisation's solve with the identity; its
+    lower triangle is mirrored, so the result is exactly symmetric as the reference's is."""
+    import torch
+
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError('expected square matrix')
+    I = np.asfortranarray(A)
+    if I.dtype != np.float64:
+        raise TypeError(f"bkf_invert: expected a float64 array (got {I.dtype}), as the "
+                        "reference's in-place Fortran call requires")
+    n = I.shape[0]
+    if n == 0:
+        return I
+    Ad = torch.from_numpy(np.ascontiguousarray(I.T)).to(device())
+    ipiv, info = bkf_factor_device(Ad, n)
+    X = torch.eye(n, dtype=torch.float64, device=Ad.device)
+    bkf_solve_device(Ad, n, ipiv, X, n)
+    Xh = X.cpu().numpy().T  # column-major solve result -> (n, n) with X[i, j]
+    low = np.tril(Xh)
+    I[...] = low + np.tril(Xh, -1).T
+    _print_bkf_warning(int(info.item()), True)
     return I

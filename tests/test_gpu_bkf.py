@@ -1,0 +1,99 @@
+"""Bunch-Kaufman on the GPU (qml/math.py:100-164) against LAPACK dsytrf / dsytrs (the oracle).
+
+Bars: the reference tests' own checks on its fixture (tests/test_math.py:73-110); the same
+pivots as LAPACK's unblocked factorisation (orders below its 64-column block) and factors within
+1e-10; solutions and inverses within 1e-8 relative of LAPACK's on well-conditioned indefinite
+matrices (1e-5 at the fixture's condition number 4.4e9); singular matrices report LAPACK's
+info."""
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+    from qml_amd import _lib
+
+    _lib.lib()
+
+
+def _factor(A):
+    from qml_amd.math import bkf_factor_device
+
+    n = A.shape[0]
+    Ad = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    ipiv, info = bkf_factor_device(Ad, n)
+    return np.tril(Ad.cpu().numpy().T), ipiv.cpu().numpy()[:n], int(info.item())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 40, 63])
+def test_bkf_factor_matches_lapack_pivots(n):
+    rng = np.random.default_rng(n)
+    for trial in range(6):
+        M = rng.standard_normal((n, n))
+        A = M + M.T
+        if trial % 2:
+            A[np.diag_indices(n)] *= 0.01  # small diagonal: 2 x 2 pivots
+        L, piv, info = _factor(A)
+        Lr, pivr, infor = oracle.bkf_factor(A)
+        assert np.array_equal(piv, pivr) and info == infor
+        assert np.max(np.abs(L - Lr)) <= 1e-10 * max(1.0, np.max(np.abs(Lr)))
+
+
+def test_bkf_singular_reports_info():
+    A = np.diag([1.0, 2.0, 0.0, 3.0])
+    _, piv, info = _factor(A)
+    assert info == oracle.bkf_factor(A)[2] == 3
+
+
+def test_bkf_reference_fixture(cho_pair):
+    from qml_amd.math import bkf_invert, bkf_solve
+
+    K, y = cho_pair
+    A = K.copy()
+    Ai = bkf_invert(A)
+    assert np.array_equal(A, K)
+    assert np.allclose(np.matmul(A, Ai), np.eye(A.shape[0]), atol=1e-7)
+    assert np.array_equal(Ai, Ai.T)
+    A, yy = K.copy(), y.copy()
+    x = bkf_solve(A, yy)
+    assert np.allclose(A, K) and np.array_equal(yy, y)
+    assert np.allclose(x, np.linalg.solve(K, y))
+    xr = oracle.bkf_solve(K.copy(), y.copy())
+    assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) <= 1e-5
+
+
+@pytest.mark.parametrize("n", [5, 64, 65, 300, 1000])
+def test_bkf_random_indefinite(n):
+    from qml_amd.math import bkf_invert, bkf_solve
+
+    rng = np.random.default_rng(100 + n)
+    M = rng.standard_normal((n, n))
+    A = M + M.T + np.diag(rng.choice([-1.0, 1.0], n) * 2 * np.sqrt(n))  # indefinite, conditioned
+    y = rng.standard_normal(n)
+    x = bkf_solve(A.copy(), y)
+    xr = oracle.bkf_solve(A.copy(), y.copy())
+    assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) <= 1e-8
+    Ai = bkf_invert(A.copy())
+    Air = oracle.bkf_invert(A.copy())
+    assert np.max(np.abs(Ai - Air)) / np.max(np.abs(Air)) <= 1e-8
+    assert np.array_equal(Ai, Ai.T)
+
+
+def test_bkf_invert_fortran_input_is_overwritten():
+    from qml_amd.math import bkf_invert
+
+    P = np.asfortranarray(np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -2.0]]))
+    Ai = bkf_invert(P)
+    assert Ai is P
+    np.testing.assert_allclose(Ai, [[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -0.5]],
+                               atol=1e-15)
