@@ -58,13 +58,14 @@ __device__ __forceinline__ double lk_inv(const LkSigmas& sg, int s) {
     return s == 0 ? sg.inv[0] : s == 1 ? sg.inv[1] : s == 2 ? sg.inv[2] : sg.inv[3];
 }
 
-// Panel-local molecule index of tile atom `t` (row0 + t), -1 past the panel's last atom.
-__device__ __forceinline__ int lk_local_mol(const int64_t* __restrict__ off, int64_t m0,
-                                            int nm, int64_t atom, int64_t atom_end) {
+// Panel-local molecule index of `atom` from the panel's atom offsets staged in LDS (soff[m] =
+// first atom of the panel's molecule m), -1 past the panel's last atom.
+__device__ __forceinline__ int lk_local_mol(const int* __restrict__ soff, int nm, int atom,
+                                            int atom_end) {
     if (atom >= atom_end) return -1;
     int lo = 0;
     for (int m = 1; m < nm; ++m)
-        if (off[m0 + m] <= atom) lo = m;
+        if (soff[m] <= atom) lo = m;
     return lo;
 }
 
@@ -109,43 +110,49 @@ template <int KIND>
 __global__ __launch_bounds__(256, 2) void local_kernels_kernel(LkSide A, LkSide B, int64_t d,
                                                                LkSigmas sg, double* __restrict__ K,
                                                                int64_t ldk, int64_t sstride,
-                                                               int symmetric, int64_t tiles_a,
-                                                               int64_t ntiles) {
+                                                               int symmetric, int tiles_a,
+                                                               int ntiles) {
     __shared__ __attribute__((aligned(16))) double lds[KIND == LK_GAUSS_MFMA ? WG_LDS_ELEMS
                                                                               : 2 * PT_BK * PT_LDS];
     __shared__ double acc[LK_NS][LK_MOLS * LK_MOLS];
     __shared__ int smolA[LK_TILE], smolB[LK_TILE];
+    __shared__ int soffA[LK_MOLS], soffB[LK_MOLS];
 
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
 
-    for (int64_t bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
-        int64_t bi, bj;
+    // Atom, molecule and panel indices fit in 32 bits (checked by the host); fewer 64-bit
+    // values keep the scalar registers from spilling.
+    for (int bid = blockIdx.x; bid < ntiles; bid += gridDim.x) {
+        int bi, bj;
         if (symmetric) {
             int64_t j = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
             while (j * (j + 1) / 2 > bid) --j;
             while ((j + 1) * (j + 2) / 2 <= bid) ++j;
-            bj = j;
-            bi = bid - j * (j + 1) / 2;
+            bj = (int)j;
+            bi = (int)(bid - j * (j + 1) / 2);
         } else {
             bi = bid % tiles_a;
             bj = bid / tiles_a;
         }
-        const int64_t ma0 = A.panels[bi], mb0 = B.panels[bj];
+        const int ma0 = (int)A.panels[bi], mb0 = (int)B.panels[bj];
         const int nma = (int)(A.panels[bi + 1] - ma0), nmb = (int)(B.panels[bj + 1] - mb0);
-        const int64_t ra0 = A.offsets[ma0], ra1 = A.offsets[ma0 + nma];
-        const int64_t rb0 = B.offsets[mb0], rb1 = B.offsets[mb0 + nmb];
+        const int ra0 = (int)A.offsets[ma0], ra1 = (int)A.offsets[ma0 + nma];
+        const int rb0 = (int)B.offsets[mb0], rb1 = (int)B.offsets[mb0 + nmb];
         // Panels come from qmlh_local_panels; one with more molecules than the LDS sums hold
         // is skipped rather than overrunning them.
         if (nma > LK_MOLS || nmb > LK_MOLS || nma < 0 || nmb < 0) continue;
 
         for (int q = t; q < LK_NS * LK_MOLS * LK_MOLS; q += 256) (&acc[0][0])[q] = 0.0;
+        if (t < nma) soffA[t] = (int)A.offsets[ma0 + t];
+        else if (t >= 64 && t - 64 < nmb) soffB[t - 64] = (int)B.offsets[mb0 + t - 64];
+        __syncthreads();
 
-        for (int64_t r0 = ra0; r0 < ra1; r0 += LK_TILE) {
-            for (int64_t c0 = rb0; c0 < rb1; c0 += LK_TILE) {
-                if (t < LK_TILE) smolA[t] = lk_local_mol(A.offsets, ma0, nma, r0 + t, ra1);
-                else smolB[t - LK_TILE] = lk_local_mol(B.offsets, mb0, nmb, c0 + t - LK_TILE, rb1);
+        for (int r0 = ra0; r0 < ra1; r0 += LK_TILE) {
+            for (int c0 = rb0; c0 < rb1; c0 += LK_TILE) {
+                if (t < LK_TILE) smolA[t] = lk_local_mol(soffA, nma, r0 + t, ra1);
+                else smolB[t - LK_TILE] = lk_local_mol(soffB, nmb, c0 + t - LK_TILE, rb1);
                 __syncthreads();
                 if (KIND == LK_GAUSS_MFMA) {
                     Operand<double, LAYOUT_ROW> opA{A.X, A.ld, A.atoms, d};
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(256, 2) void local_kernels_kernel(LkSide A, LkSide 
                     double ni[4], nj[16];
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const int64_t i = r0 + rbase + 16 * b + (lane & 15);
+                        const int i = r0 + rbase + 16 * b + (lane & 15);
                         ni[b] = A.norms[i < A.atoms ? i : A.atoms - 1];
                         uint32_t p = 0;
 #pragma unroll
@@ -174,7 +181,7 @@ __global__ __launch_bounds__(256, 2) void local_kernels_kernel(LkSide A, LkSide 
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             const int cl = cbase + 16 * a + (lane >> 4) + 4 * e;
-                            const int64_t j = c0 + cl;
+                            const int j = c0 + cl;
                             nj[4 * a + e] = B.norms[j < B.atoms ? j : B.atoms - 1];
                             p |= (uint32_t)(smolB[cl] & 0xff) << (8 * e);
                         }
@@ -253,7 +260,7 @@ __global__ __launch_bounds__(256, 2) void local_kernels_kernel(LkSide A, LkSide 
                 if (ma >= nma || mb >= nmb) continue;
                 if (diag && ma > mb) continue;
                 const double val = acc[s][ma * LK_MOLS + mb];
-                const int64_t ga = ma0 + ma, gb = mb0 + mb;
+                const int64_t ga = (int64_t)ma0 + ma, gb = (int64_t)mb0 + mb;
                 Ks[ga + gb * ldk] = val;
                 if (symmetric && ga != gb) Ks[gb + ga * ldk] = val;
             }
@@ -268,7 +275,7 @@ static void lk_launch(int64_t ntiles, hipStream_t s, const LkSide& A, const LkSi
                       int symmetric, int64_t tiles_a) {
     const int64_t grid = ntiles < LK_MAX_GRID ? ntiles : LK_MAX_GRID;
     hipLaunchKernelGGL((local_kernels_kernel<KIND>), dim3((unsigned)grid), dim3(256), 0, s, A,
-                       B, d, sg, K, ldk, sstride, symmetric, tiles_a, ntiles);
+                       B, d, sg, K, ldk, sstride, symmetric, (int)tiles_a, (int)ntiles);
 }
 
 }  // namespace qmlh
@@ -330,6 +337,8 @@ extern "C" int qmlh_local_kernels(int32_t kind, const double* A, int64_t atomsA,
     LkSide sa{A, lda, normsA, offsA, panelsA, atomsA};
     LkSide sb{B, ldb, normsB, offsB, panelsB, atomsB};
     const int64_t grid = symmetric ? npA * (npA + 1) / 2 : npA * npB;
+    if (grid > 0x7fffffffLL || atomsA > 0x7fffffffLL - LK_TILE || atomsB > 0x7fffffffLL - LK_TILE)
+        return set_error(QMLH_ERR_ARG, "local_kernels: more than 2^31 atoms or panel pairs");
     hipStream_t s = (hipStream_t)stream;
     // Work: atom pairs x features (the pairs of a symmetric call counted once).
     const double pairs = symmetric ? 0.5 * (double)atomsA * (double)(atomsA + 1)
