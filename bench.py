@@ -502,8 +502,8 @@ def run_invert(n, cpu_n, log):
                                "sample": f"LAPACK dpotrf + dpotri (scipy) at n={cpu_n}: "
                                          f"{t1 - t0:.2f}s on {cores} threads; extrapolated as n^3"}
     log(f"cho_invert n={n}: {ms:.1f} ms ({out['tflops_lapack_equiv']:.1f} TF/s LAPACK-equivalent)")
-    # Bunch-Kaufman (bkf_solve / bkf_invert): one-workgroup factorisation, per-column solves, on
-    # a symmetric indefinite matrix of order 2000, against scipy's dsytrf + dsytrs at the same n.
+    # Bunch-Kaufman (bkf_solve / bkf_invert): stepped factorisation, per-column solves, on a
+    # symmetric indefinite matrix of order 2000, against scipy's dsytrf + dsytrs at the same n.
     from qml_amd.math import bkf_factor_device, bkf_solve_device
 
     nb = 2000
@@ -531,7 +531,8 @@ def run_invert(n, cpu_n, log):
     bk = {"n": nb, "factor_ms": e0.elapsed_time(e1), "solve_ms": e2.elapsed_time(e3),
           "inverse_solves_ms": e3.elapsed_time(e4),
           "factor_gflops": nb ** 3 / 3.0 / (e0.elapsed_time(e1) * 1e-3) / 1e9,
-          "note": "one workgroup walks the columns (pivot search depends on every update)"}
+          "note": "a step per column: pivot search and interchange in one workgroup, the "
+                  "trailing rank-1 / rank-2 update over many"}
     from scipy.linalg import lapack
 
     t0 = time.perf_counter()

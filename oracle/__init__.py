@@ -467,10 +467,14 @@ def bkf_invert(A):
 
 
 def bkf_factor(A):
-    """LAPACK dsytrf('L') of A (a copy): (lower factors, 1-based pivots, info)."""
+    """LAPACK dsytrf('L') of A (a copy) with a workspace of n, which makes LAPACK run its
+    unblocked Bunch-Kaufman code (dsytf2): (lower factors, 1-based pivots, info). The
+    reference's blocked call (lwork = n nb) makes the same pivot choices in exact arithmetic."""
     from scipy.linalg import lapack
 
-    lu, piv, info = lapack.dsytrf(np.asfortranarray(A, dtype=np.float64), lower=1)[:3]
+    n = A.shape[0]
+    lu, piv, info = lapack.dsytrf(np.asfortranarray(A, dtype=np.float64), lower=1,
+                                  lwork=max(1, n))[:3]
     return np.tril(lu), piv, info
 
 

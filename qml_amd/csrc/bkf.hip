@@ -9,10 +9,11 @@
 //   (k, imax) with imax moved to k + 1. Interchanges act on the trailing matrix only; ipiv holds
 //   1-based pivots, negative for both columns of a 2 x 2 block; info is the first zero pivot.
 //
-// The factorisation is one workgroup of 1024 lanes walking the columns (the pivot search of each
-// column depends on every earlier update); the trailing rank-1 / rank-2 updates are spread over
-// the 16 waves, a column per wave, rows over lanes. The solves run one workgroup per
-// right-hand-side column, so an inverse (n right-hand sides) fills the GPU.
+// Below order 256 the factorisation is one workgroup of 1024 lanes walking the columns (the pivot
+// search of each column depends on every earlier update), the trailing rank-1 / rank-2 updates
+// spread over its 16 waves; from 256 on each column is a step of three launches whose update
+// covers the GPU. The solves run one workgroup per right-hand-side column, so an inverse (n
+// right-hand sides) fills the GPU.
 #include "common.h"
 #include "timer.h"
 
@@ -56,6 +57,108 @@ __device__ __forceinline__ void bk_argmax(double v, int64_t i, double* s_v, int6
     out_i = bi;
 }
 
+// Pivot choice and interchange for column k (one workgroup): returns kstep (1 or 2) and kp
+// (0-based), zero = a zero column (no pivot, no update). Every thread gets the same values.
+__device__ __forceinline__ void bk_pivot(double* __restrict__ A, int64_t n, int64_t lda, int64_t k,
+                                         double* s_v, int64_t* s_i, int& kstep, int64_t& kp,
+                                         bool& zero) {
+    const int t = threadIdx.x;
+    const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+    auto at = [&](int64_t i, int64_t j) -> double& { return A[i + j * lda]; };
+    kstep = 1;
+    kp = k;
+    const double absakk = fabs(at(k, k));
+    double colmax = 0.0;
+    int64_t imax = k;
+    if (k + 1 < n) {
+        double v = -1.0;
+        int64_t vi = n;
+        for (int64_t i = k + 1 + t; i < n; i += BK_THREADS) {
+            const double a = fabs(at(i, k));
+            if (a > v) {
+                v = a;
+                vi = i;
+            }
+        }
+        bk_argmax(v, vi, s_v, s_i, colmax, imax);
+    }
+    zero = (absakk > colmax ? absakk : colmax) == 0.0;
+    if (zero) return;
+    if (absakk < alpha * colmax) {
+        // rowmax: largest |entry| of row imax in columns k .. imax-1, then of column imax below
+        // the diagonal.
+        double v = -1.0;
+        int64_t vi = n;
+        for (int64_t j = k + t; j < imax; j += BK_THREADS) {
+            const double a = fabs(at(imax, j));
+            if (a > v) {
+                v = a;
+                vi = j;
+            }
+        }
+        for (int64_t i = imax + 1 + t; i < n; i += BK_THREADS) {
+            const double a = fabs(at(i, imax));
+            if (a > v) {
+                v = a;
+                vi = i;
+            }
+        }
+        double rowmax;
+        int64_t jmax;
+        bk_argmax(v, vi, s_v, s_i, rowmax, jmax);
+        if (absakk >= alpha * colmax * (colmax / rowmax)) {
+            kp = k;
+        } else if (fabs(at(imax, imax)) >= alpha * rowmax) {
+            kp = imax;
+        } else {
+            kp = imax;
+            kstep = 2;
+        }
+    }
+    const int64_t kk = k + kstep - 1;
+    if (kp != kk) {
+        // Interchange rows and columns kk and kp of the trailing lower triangle.
+        for (int64_t i = kp + 1 + t; i < n; i += BK_THREADS) {
+            const double x = at(i, kk);
+            at(i, kk) = at(i, kp);
+            at(i, kp) = x;
+        }
+        for (int64_t j = kk + 1 + t; j < kp; j += BK_THREADS) {
+            const double x = at(j, kk);
+            at(j, kk) = at(kp, j);
+            at(kp, j) = x;
+        }
+        if (t == 0) {
+            const double x = at(kk, kk);
+            at(kk, kk) = at(kp, kp);
+            at(kp, kp) = x;
+            if (kstep == 2) {
+                const double y = at(k + 1, k);
+                at(k + 1, k) = at(kp, k);
+                at(kp, k) = y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The 2 x 2 pivot's multipliers of rows j >= k + 2 from the old columns k, k + 1 (into wsp).
+__device__ __forceinline__ void bk_multipliers(const double* __restrict__ A, int64_t n,
+                                               int64_t lda, int64_t k, double* __restrict__ wsp) {
+    auto at = [&](int64_t i, int64_t j) -> double { return A[i + j * lda]; };
+    double d21 = at(k + 1, k);
+    const double d11 = at(k + 1, k + 1) / d21;
+    const double d22 = at(k, k) / d21;
+    const double tt = 1.0 / (d11 * d22 - 1.0);
+    d21 = tt / d21;
+    for (int64_t j = k + 2 + threadIdx.x; j < n; j += blockDim.x) {
+        const double ajk = at(j, k), ajk1 = at(j, k + 1);
+        wsp[2 * j] = d21 * (d11 * ajk - ajk1);
+        wsp[2 * j + 1] = d21 * (d22 * ajk1 - ajk);
+    }
+}
+
+// Whole factorisation in one workgroup (small orders).
 __global__ __launch_bounds__(BK_THREADS) void bkf_factor_kernel(double* __restrict__ A, int64_t n,
                                                                  int64_t lda,
                                                                  int32_t* __restrict__ ipiv,
@@ -64,128 +167,43 @@ __global__ __launch_bounds__(BK_THREADS) void bkf_factor_kernel(double* __restri
     __shared__ double s_v[BK_WAVES];
     __shared__ int64_t s_i[BK_WAVES];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const double alpha = (1.0 + sqrt(17.0)) / 8.0;
     int32_t first_zero = 0;
     auto at = [&](int64_t i, int64_t j) -> double& { return A[i + j * lda]; };
 
     int64_t k = 0;
     while (k < n) {
-        int kstep = 1;
-        int64_t kp = k;
-        const double absakk = fabs(at(k, k));
-        double colmax = 0.0;
-        int64_t imax = k;
-        if (k + 1 < n) {
-            double v = -1.0;
-            int64_t vi = n;
-            for (int64_t i = k + 1 + t; i < n; i += BK_THREADS) {
-                const double a = fabs(at(i, k));
-                if (a > v) {
-                    v = a;
-                    vi = i;
-                }
-            }
-            bk_argmax(v, vi, s_v, s_i, colmax, imax);
-        }
-        const bool zero = (absakk > colmax ? absakk : colmax) == 0.0;
+        int kstep;
+        int64_t kp;
+        bool zero;
+        bk_pivot(A, n, lda, k, s_v, s_i, kstep, kp, zero);
         if (zero) {
             if (first_zero == 0) first_zero = (int32_t)(k + 1);
-            kp = k;
-        } else {
-            if (absakk >= alpha * colmax) {
-                kp = k;
-            } else {
-                // rowmax: largest |entry| of row imax in columns k .. imax-1, then of column
-                // imax below the diagonal.
-                double v = -1.0;
-                int64_t vi = n;
-                for (int64_t j = k + t; j < imax; j += BK_THREADS) {
-                    const double a = fabs(at(imax, j));
-                    if (a > v) {
-                        v = a;
-                        vi = j;
-                    }
-                }
-                for (int64_t i = imax + 1 + t; i < n; i += BK_THREADS) {
-                    const double a = fabs(at(i, imax));
-                    if (a > v) {
-                        v = a;
-                        vi = i;
-                    }
-                }
-                double rowmax;
-                int64_t jmax;
-                bk_argmax(v, vi, s_v, s_i, rowmax, jmax);
-                if (absakk >= alpha * colmax * (colmax / rowmax)) {
-                    kp = k;
-                } else if (fabs(at(imax, imax)) >= alpha * rowmax) {
-                    kp = imax;
-                } else {
-                    kp = imax;
-                    kstep = 2;
+        } else if (kstep == 1) {
+            // A22 := A22 - x x^T / d (lower), then column k := x / d (the dsyr + dscal pair).
+            const double r1 = 1.0 / at(k, k);
+            for (int64_t j = k + 1 + w; j < n; j += BK_WAVES) {
+                const double xj = at(j, k);
+                if (xj != 0.0) {
+                    const double temp = -r1 * xj;
+                    for (int64_t i = j + lane; i < n; i += 64) at(i, j) = at(i, j) + at(i, k) * temp;
                 }
             }
-            const int64_t kk = k + kstep - 1;
-            if (kp != kk) {
-                // Interchange rows and columns kk and kp of the trailing lower triangle.
-                for (int64_t i = kp + 1 + t; i < n; i += BK_THREADS) {
-                    const double x = at(i, kk);
-                    at(i, kk) = at(i, kp);
-                    at(i, kp) = x;
-                }
-                for (int64_t j = kk + 1 + t; j < kp; j += BK_THREADS) {
-                    const double x = at(j, kk);
-                    at(j, kk) = at(kp, j);
-                    at(kp, j) = x;
-                }
-                if (t == 0) {
-                    const double x = at(kk, kk);
-                    at(kk, kk) = at(kp, kp);
-                    at(kp, kp) = x;
-                    if (kstep == 2) {
-                        const double y = at(k + 1, k);
-                        at(k + 1, k) = at(kp, k);
-                        at(kp, k) = y;
-                    }
-                }
-                __syncthreads();
+            __syncthreads();
+            for (int64_t i = k + 1 + t; i < n; i += BK_THREADS) at(i, k) = r1 * at(i, k);
+        } else if (k + 2 < n) {
+            // 2 x 2 pivot (k, k+1): multipliers from the old columns, the rank-2 update with
+            // them, then the columns replaced by the multipliers.
+            bk_multipliers(A, n, lda, k, wsp);
+            __syncthreads();
+            for (int64_t j = k + 2 + w; j < n; j += BK_WAVES) {
+                const double wk = wsp[2 * j], wkp1 = wsp[2 * j + 1];
+                for (int64_t i = j + lane; i < n; i += 64)
+                    at(i, j) = at(i, j) - at(i, k) * wk - at(i, k + 1) * wkp1;
             }
-            if (kstep == 1) {
-                // A22 := A22 - x x^T / d (lower), then column k := x / d (the dsyr + dscal pair).
-                const double r1 = 1.0 / at(k, k);
-                for (int64_t j = k + 1 + w; j < n; j += BK_WAVES) {
-                    const double xj = at(j, k);
-                    if (xj != 0.0) {
-                        const double temp = -r1 * xj;
-                        for (int64_t i = j + lane; i < n; i += 64) at(i, j) = at(i, j) + at(i, k) * temp;
-                    }
-                }
-                __syncthreads();
-                for (int64_t i = k + 1 + t; i < n; i += BK_THREADS) at(i, k) = r1 * at(i, k);
-            } else if (k + 2 < n) {
-                // 2 x 2 pivot (k, k+1): multipliers of every later row from the old columns, the
-                // rank-2 update with them, then the columns replaced by the multipliers.
-                double d21 = at(k + 1, k);
-                const double d11 = at(k + 1, k + 1) / d21;
-                const double d22 = at(k, k) / d21;
-                const double tt = 1.0 / (d11 * d22 - 1.0);
-                d21 = tt / d21;
-                for (int64_t j = k + 2 + t; j < n; j += BK_THREADS) {
-                    const double ajk = at(j, k), ajk1 = at(j, k + 1);
-                    wsp[2 * j] = d21 * (d11 * ajk - ajk1);
-                    wsp[2 * j + 1] = d21 * (d22 * ajk1 - ajk);
-                }
-                __syncthreads();
-                for (int64_t j = k + 2 + w; j < n; j += BK_WAVES) {
-                    const double wk = wsp[2 * j], wkp1 = wsp[2 * j + 1];
-                    for (int64_t i = j + lane; i < n; i += 64)
-                        at(i, j) = at(i, j) - at(i, k) * wk - at(i, k + 1) * wkp1;
-                }
-                __syncthreads();
-                for (int64_t j = k + 2 + t; j < n; j += BK_THREADS) {
-                    at(j, k) = wsp[2 * j];
-                    at(j, k + 1) = wsp[2 * j + 1];
-                }
+            __syncthreads();
+            for (int64_t j = k + 2 + t; j < n; j += BK_THREADS) {
+                at(j, k) = wsp[2 * j];
+                at(j, k + 1) = wsp[2 * j + 1];
             }
         }
         if (t == 0) {
@@ -200,6 +218,127 @@ __global__ __launch_bounds__(BK_THREADS) void bkf_factor_kernel(double* __restri
         k += kstep;
     }
     if (t == 0) *info = first_zero;
+}
+
+// ------------------------------------------------------------------------------------------
+// Larger orders: one step per column, the same arithmetic in three launches -- the pivot and
+// interchange (one workgroup), the rank-1 / rank-2 update of the trailing triangle (64 x 64
+// tiles, many workgroups), and the scaled or replaced pivot columns. The current column lives
+// in device memory (a 2 x 2 pivot consumes two steps, so step s is idle when the column
+// counter has passed s); the host enqueues n steps without reading anything back.
+// ------------------------------------------------------------------------------------------
+struct BkState {
+    int64_t k;       // next column to factor
+    int64_t step_k;  // column of the current step (-1: idle step)
+    int32_t kstep;
+    int32_t zero;
+    double r1;       // 1 / D(k, k) of a 1 x 1 pivot
+};
+
+__global__ void bkf_init_kernel(BkState* st, int32_t* info) {
+    st->k = 0;
+    st->step_k = -1;
+    *info = 0;
+}
+
+__global__ __launch_bounds__(BK_THREADS) void bkf_pivot_kernel(double* __restrict__ A, int64_t n,
+                                                                int64_t lda,
+                                                                int32_t* __restrict__ ipiv,
+                                                                int32_t* __restrict__ info,
+                                                                BkState* __restrict__ st,
+                                                                double* __restrict__ wsp,
+                                                                int64_t s) {
+    __shared__ double s_v[BK_WAVES];
+    __shared__ int64_t s_i[BK_WAVES];
+    const int t = threadIdx.x;
+    const int64_t k = st->k;
+    if (k != s) {  // consumed by the previous step's 2 x 2 pivot
+        __syncthreads();
+        if (t == 0) st->step_k = -1;
+        return;
+    }
+    int kstep;
+    int64_t kp;
+    bool zero;
+    bk_pivot(A, n, lda, k, s_v, s_i, kstep, kp, zero);
+    if (!zero && kstep == 2 && k + 2 < n) bk_multipliers(A, n, lda, k, wsp);
+    __syncthreads();
+    if (t == 0) {
+        if (zero && *info == 0) *info = (int32_t)(k + 1);
+        if (kstep == 1) {
+            ipiv[k] = (int32_t)(kp + 1);
+        } else {
+            ipiv[k] = -(int32_t)(kp + 1);
+            ipiv[k + 1] = -(int32_t)(kp + 1);
+        }
+        st->step_k = k;
+        st->kstep = kstep;
+        st->zero = zero ? 1 : 0;
+        st->r1 = zero ? 0.0 : 1.0 / A[k + k * lda];
+        st->k = k + kstep;
+    }
+}
+
+// Trailing update of step s: tile (ti, tj), ti >= tj, of the triangle rows/columns s+1 .. n-1.
+__global__ __launch_bounds__(256) void bkf_update_kernel(double* __restrict__ A, int64_t n,
+                                                          int64_t lda,
+                                                          const BkState* __restrict__ st,
+                                                          const double* __restrict__ wsp,
+                                                          int64_t s) {
+    const int64_t k = st->step_k;
+    if (k < 0 || st->zero) return;
+    const int kstep = st->kstep;
+    const int64_t j0 = k + kstep;  // first trailing column
+    if (j0 >= n) return;
+    const int64_t bid = blockIdx.x;
+    int64_t tj = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while (tj * (tj + 1) / 2 > bid) --tj;
+    while ((tj + 1) * (tj + 2) / 2 <= bid) ++tj;
+    const int64_t ti = bid - tj * (tj + 1) / 2;  // ti <= tj: row tile tj, column tile ti
+    const int64_t r0 = s + 1 + 64 * tj, c0 = s + 1 + 64 * ti;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t i = r0 + lane;
+    auto at = [&](int64_t a, int64_t b) -> double& { return A[a + b * lda]; };
+    if (i >= n) return;
+    if (kstep == 1) {
+        const double r1 = st->r1;
+        const double aik = at(i, k);
+        for (int q = w; q < 64; q += 4) {
+            const int64_t j = c0 + q;
+            if (j < j0 || j > i) continue;
+            const double xj = at(j, k);
+            if (xj != 0.0) {
+                const double temp = -r1 * xj;
+                at(i, j) = at(i, j) + aik * temp;
+            }
+        }
+    } else {
+        const double aik = at(i, k), aik1 = at(i, k + 1);
+        for (int q = w; q < 64; q += 4) {
+            const int64_t j = c0 + q;
+            if (j < j0 || j > i) continue;
+            at(i, j) = at(i, j) - aik * wsp[2 * j] - aik1 * wsp[2 * j + 1];
+        }
+    }
+}
+
+// The pivot columns of step s: scaled by 1 / D (1 x 1) or replaced by the multipliers (2 x 2).
+__global__ __launch_bounds__(256) void bkf_finish_kernel(double* __restrict__ A, int64_t n,
+                                                          int64_t lda,
+                                                          const BkState* __restrict__ st,
+                                                          const double* __restrict__ wsp) {
+    const int64_t k = st->step_k;
+    if (k < 0 || st->zero) return;
+    const int64_t i = k + 1 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (st->kstep == 1) {
+        if (i < n) A[i + k * lda] = st->r1 * A[i + k * lda];
+    } else {
+        const int64_t j = i + 1;  // rows k + 2 ..
+        if (j < n) {
+            A[j + k * lda] = wsp[2 * j];
+            A[j + (k + 1) * lda] = wsp[2 * j + 1];
+        }
+    }
 }
 
 // Block-wide sum of v (fixed order: waves in order), every thread gets the result.
@@ -316,7 +455,12 @@ __global__ __launch_bounds__(BS_THREADS) void bkf_solve_kernel(const double* __r
 
 }  // namespace qmlh
 
-extern "C" size_t qmlh_bkf_workspace(int64_t n) { return n > 0 ? (size_t)(2 * n) * sizeof(double) : 0; }
+// Orders from which the stepped (many-workgroup) factorisation is used.
+static constexpr int64_t BK_STEPPED_MIN = 256;
+
+extern "C" size_t qmlh_bkf_workspace(int64_t n) {
+    return n > 0 ? (size_t)(2 * n) * sizeof(double) + 256 : 0;
+}
 
 extern "C" int qmlh_bkf_factor(double* A, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
                                void* ws, size_t ws_bytes, qmlh_stream_t stream) {
@@ -327,11 +471,33 @@ extern "C" int qmlh_bkf_factor(double* A, int64_t n, int64_t lda, int32_t* ipiv,
     if (!A || !ipiv || !info) return set_error(QMLH_ERR_ARG, "bkf_factor: null pointer");
     if (!ws || ws_bytes < qmlh_bkf_workspace(n))
         return set_error(QMLH_ERR_WORKSPACE, "bkf_factor: workspace too small");
+    if (reinterpret_cast<uintptr_t>(ws) & 15)
+        return set_error(QMLH_ERR_ARG, "bkf_factor: workspace must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     TimerScope ts(QMLH_TIMER_POTRF, (double)n * (double)n * (double)n / 3.0, s);
-    hipLaunchKernelGGL(bkf_factor_kernel, dim3(1), dim3(BK_THREADS), 0, s, A, n, lda, ipiv, info,
-                       static_cast<double*>(ws));
-    QMLH_CHECK_LAUNCH("bkf_factor launch");
+    BkState* st = static_cast<BkState*>(ws);
+    double* wsp = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
+    if (n < BK_STEPPED_MIN) {
+        hipLaunchKernelGGL(bkf_factor_kernel, dim3(1), dim3(BK_THREADS), 0, s, A, n, lda, ipiv,
+                           info, wsp);
+        QMLH_CHECK_LAUNCH("bkf_factor launch");
+        return QMLH_OK;
+    }
+    hipLaunchKernelGGL(bkf_init_kernel, dim3(1), dim3(1), 0, s, st, info);
+    QMLH_CHECK_LAUNCH("bkf_factor init launch");
+    for (int64_t step = 0; step < n; ++step) {
+        hipLaunchKernelGGL(bkf_pivot_kernel, dim3(1), dim3(BK_THREADS), 0, s, A, n, lda, ipiv,
+                           info, st, wsp, step);
+        const int64_t m = n - step - 1;  // trailing order bound of this step
+        if (m > 0) {
+            const int64_t tiles = ceil_div(m, 64);
+            hipLaunchKernelGGL(bkf_update_kernel, dim3((unsigned)(tiles * (tiles + 1) / 2)),
+                               dim3(256), 0, s, A, n, lda, st, wsp, step);
+            hipLaunchKernelGGL(bkf_finish_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0,
+                               s, A, n, lda, st, wsp);
+        }
+    }
+    QMLH_CHECK_LAUNCH("bkf_factor step launches");
     return QMLH_OK;
 }
 
