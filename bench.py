@@ -618,7 +618,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and not built.", file=sys.stderr)
+              "path is not built.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -1,6 +1,6 @@
 """Seeded synthetic molecules for the KRR workloads.
 
-Molecules follow a fixed recipe so every run, host and GPU box sees identical inputs:
+Molecules follow a fixed recipe so every run, on any host or GPU, sees identical inputs:
 
 * atom count uniform in ``[min_atoms, max_atoms]``;
 * elements H, C, N, O, S (Z = 1, 6, 7, 8, 16) with probabilities 0.55, 0.30, 0.07, 0.07, 0.01;
