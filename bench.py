@@ -42,12 +42,33 @@ SYRK_KERNEL = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_krr_serve_50k.json")
 
 
+def train_relres(model, y) -> float:
+    """|y - (K + lambda I) alpha| / |y| of a fitted model, K alpha from the prediction kernel."""
+    import numpy as np
+    import torch
+
+    ka = model.predict_device(model.Xd, model.n)
+    yd = torch.as_tensor(np.asarray(y), dtype=torch.float64).to(ka.device)
+    r = yd - ka - model.lam * model.alpha
+    return float(r.norm() / yd.norm())
+
+
+def _short(x: float) -> str:
+    """1e-06 -> 1e-6: the exponent without its leading zero."""
+    return (f"{x:g}").replace("e-0", "e-").replace("e+0", "e+")
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n-train", type=int, default=50000)
+    p.add_argument("--kernel", choices=["gaussian", "laplacian"], default="gaussian",
+                   help="kernel of the fits (the headline workload is Gaussian)")
+    p.add_argument("--factor", choices=["auto", "fp32", "f16x3"], default="auto",
+                   help="mixed fit's low-precision factor (auto: the model's default, "
+                        "f16x3)")
     p.add_argument("--sigma", type=float, default=4000.0)
     p.add_argument("--lam", type=float, default=1e-6)
     p.add_argument("--requests", type=int, default=300)
@@ -676,7 +697,7 @@ def main():
         _lib.timer_enable(False)
         return (t1 - t0) / args.steps, timers, reps[-1]
 
-    model64 = KRR("gaussian", args.sigma, args.lam, "fp64")
+    model64 = KRR(args.kernel, args.sigma, args.lam, "fp64")
     fit64_s, timers, _ = run_fits(model64, "fp64")
     # Predictions of the FP64 fit on a fixed query set, the reference for the mixed fit's error.
     Zq, Rq, offq = te.subset(0, 2048).ragged()
@@ -685,7 +706,8 @@ def main():
     modelmx = None
     fitmx_s, timers_mx, rep_mx = None, None, None
     if args.precision in ("mixed", "both"):
-        modelmx = KRR("gaussian", args.sigma, args.lam, "mixed", tol=args.tol)
+        modelmx = KRR(args.kernel, args.sigma, args.lam, "mixed", tol=args.tol,
+                      factor=None if args.factor == "auto" else args.factor)
         fitmx_s, timers_mx, rep_mx = run_fits(modelmx, "mixed")
         predmx = modelmx.predict_molecules(Zq, Rq, offq, size)
         std = float(np.std(tr.labels))
@@ -693,7 +715,11 @@ def main():
             "pred_max_abs_over_std": float(np.max(np.abs(predmx - pred64)) / std),
             "alpha_max_rel": float((modelmx.alpha - alpha64).abs().max() / alpha64.abs().max()),
             "refinement_iterations": rep_mx.iterations, "relres": rep_mx.relres,
-            "fp32_shift": rep_mx.shift, "reference": "FP64 GPU fit (same data)"}
+            "fp32_shift": rep_mx.shift, "reference": "FP64 GPU fit (same data)",
+            # Both solutions' residuals, recomputed alike with the prediction kernel over the
+            # training rows: which of the two fits is the more accurate at small lambda.
+            "relres_recomputed": {"fp64_direct": train_relres(model64, tr.labels),
+                                  "mixed": train_relres(modelmx, tr.labels)}}
     fit_s = fitmx_s if fitmx_s is not None else fit64_s
     model = modelmx if modelmx is not None else model64
 
@@ -768,9 +794,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32 (mixed)" if fitmx_s is not None else "f64",
         "data": "synthetic (seeded molecules, 5-29 atoms, H/C/N/O/S)",
-        "config": {"workload": "krr_serve_50k: KRR fit N=50000, Coulomb size 29 (d=435), "
-                               "Gaussian sigma=4000, lambda=1e-6; predict requests 1-512 molecules",
-                   "n_train": N, "d": d, "sigma": args.sigma, "lambda": args.lam,
+        "config": {"workload": (f"krr_serve_{N // 1000}k: KRR fit N={N}, Coulomb size {size} "
+                                f"(d={d}), {args.kernel.capitalize()} sigma={args.sigma:g}, "
+                                f"lambda={_short(args.lam)}; predict requests 1-512 molecules"),
+                   "n_train": N, "d": d, "kernel": args.kernel,
+                   "factor": modelmx.factor if modelmx is not None else None, "sigma": args.sigma,
+                   "lambda": args.lam,
                    "precision": "mixed" if fitmx_s is not None else "fp64"},
         "fit_s": {"mixed": fitmx_s, "fp64": fit64_s,
                   "speedup_mixed_vs_fp64": (fit64_s / fitmx_s) if fitmx_s else None},

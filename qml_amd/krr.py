@@ -44,8 +44,10 @@ class KRR:
     kernel: "gaussian" (FP64 MFMA expanded form) or "laplacian" (FP64 VALU).
     precision:
       "fp64"  -- K in FP64, FP64 MFMA Cholesky, two triangular solves (the drop-in path);
-      "mixed" -- K in FP64 for the residuals, an FP32 Cholesky (FP32 MFMA trailing updates,
-                 FP64 diagonal blocks) as preconditioner, FP64 conjugate-gradient refinement to
+      "mixed" -- K in FP64 for the residuals, a Cholesky factor in FP32 storage (trailing
+                 updates as split-FP16 MFMA products, factor="f16x3", the default, or FP32 MFMA,
+                 factor="fp32"; FP64 diagonal blocks) as preconditioner, FP64
+                 conjugate-gradient refinement to
                  |y - K alpha| <= tol |y|. For the Gaussian kernel the factor is of the
                  deflated matrix B = K - V V^T (V = u [1, x / sigma], u_i =
                  exp(-|x_i|^2 / 2 sigma^2); the constant and linear Taylor terms of the
@@ -66,10 +68,10 @@ class KRR:
         self.precision = precision
         self.tol = float(tol)
         self.max_iter = int(max_iter)
-        # Mixed-precision factor: "f16x3" (split-FP16 trailing updates) needs the deflated
-        # Gaussian remainder to stay accurate relative to lambda; "fp32" everywhere else.
+        # Mixed-precision factor: "f16x3" (split-FP16 trailing updates, the faster) or "fp32"
+        # (FP32 MFMA trailing updates).
         if factor is None:
-            factor = "f16x3" if kernel == "gaussian" else "fp32"
+            factor = "f16x3"
         if factor not in ("fp32", "f16x3"):
             raise ValueError("factor must be 'fp32' or 'f16x3'")
         self.factor = factor
