@@ -661,7 +661,8 @@ def main():
         log(f"copy calibration: 3 x {args.copy_mib} MiB elementwise read + write")
         del src, dst
     if args.local_only:
-        print(json.dumps({"local": run_local(args.local, args.local_cpu_sample, log),
+        print(json.dumps({"local": (run_local(args.local, args.local_cpu_sample, log)
+                                    if args.local > 0 else None),
                           "qm7": run_qm7(log) if args.qm7 else None,
                           "cho_invert": run_invert(args.invert, 10000 if args.local_cpu_sample
                                                    else 0, log) if args.invert > 0 else None}),

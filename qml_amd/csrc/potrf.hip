@@ -40,7 +40,10 @@ static constexpr int UP_TILE = WG_T;
 // the workgroup of tile (0, 0) then factors the w_next x w_next diagonal block at C (the next
 // sub-panel's, which only that workgroup writes), so the next sub-panel needs no launch of its
 // own for it.
-enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2, UPD_TRAP_POTF2 = 3 };
+// UPD_SYRK_UT: the lower triangle of C -= A A^T for an upper-triangular A (row i zero left of
+// column i): tile row bi starts its k loop at its first row, bi * UP_TILE, skipping only
+// products that are exactly zero (the inverse's -L^-T L^-1 in one launch).
+enum { UPD_FULL = 0, UPD_SYRK = 1, UPD_TRAP = 2, UPD_TRAP_POTF2 = 3, UPD_SYRK_UT = 4 };
 
 // ------------------------------------------------------------------------------------------
 // C(i, j) -= sum_k A(i, k) B(j, k); A and C column-major, B column-major (LB = COL: B(j, k) at
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     if (info && *info) return;
     int64_t bi, bj;
     const int64_t bid = blockIdx.x;
-    if (MODE == UPD_SYRK) {
+    if (MODE == UPD_SYRK || MODE == UPD_SYRK_UT) {
         int64_t i = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
         while (i * (i + 1) / 2 > bid) --i;
         while ((i + 1) * (i + 2) / 2 <= bid) ++i;
@@ -71,8 +74,9 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     }
     // First MFMA operand: B rows (output columns); second: A rows (output rows), so the 16
     // lanes of one accumulator element run down 16 consecutive rows of column-major C.
-    Operand<T, LAYOUT_COL> opA{A, lda, M, Kd};
-    Operand<T, LB> opB{B, ldb, N, Kd};
+    const int64_t k0 = MODE == UPD_SYRK_UT ? bi * UP_TILE : 0;
+    Operand<T, LAYOUT_COL> opA{A + k0 * lda, lda, M, Kd - k0};
+    Operand<T, LB> opB{B + (LB == LAYOUT_COL ? k0 * ldb : k0), ldb, N, Kd - k0};
     const int w = threadIdx.x >> 6;
     const int64_t col0 = bj * UP_TILE + 64 * (w & 1);
     const int64_t row0 = bi * UP_TILE + 64 * (w >> 1);
@@ -80,7 +84,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const bool interior = live && row0 + 64 <= M && col0 + 64 <= N && (!MASK || col0 + 63 <= row0);
     WaveTile<T> t;
     wave_tile_load_c<T, MASK>(t, C, ldc, row0, col0, M, N, live, interior);
-    wg_tile_mma_lds<T, LB, LAYOUT_COL, true>(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd);
+    wg_tile_mma_lds<T, LB, LAYOUT_COL, true>(t, lds, opB, opA, bj * UP_TILE, bi * UP_TILE, Kd - k0);
     if (live) wave_tile_store_c<T, MASK>(t, C, ldc, row0, col0, M, N, interior);
     if (MODE == UPD_TRAP_POTF2 && bid == 0 && w_next > 0) {
         // every wave of this workgroup got here; the barrier makes the tile's stores visible
@@ -102,11 +106,17 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
     if (timer_class < 0) timer_class = mode == UPD_SYRK ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
     double flops = 2.0 * (double)M * (double)N * (double)Kd;
     if (mode == UPD_SYRK) flops = (double)M * (double)(M + 1) * (double)Kd;
+    if (mode == UPD_SYRK_UT) flops = (double)M * (double)M * (double)M / 3.0;  // M = N = Kd
     if (mode >= UPD_TRAP) flops = ((double)M * (double)N - (double)N * (double)(N - 1) / 2.0) * 2.0 * (double)Kd;
     TimerScope ts(timer_class, flops, s);
     switch (mode) {
         case UPD_SYRK:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
+                               dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
+                               (int64_t)0);
+            break;
+        case UPD_SYRK_UT:
+            hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK_UT, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
                                (int64_t)0);
             break;
@@ -777,18 +787,10 @@ extern "C" int qmlh_cho_invert(double* A, int64_t n, int64_t lda, int32_t* info,
     QMLH_CHECK_LAUNCH("cho_invert transpose launch");
     hipLaunchKernelGGL(identity_f64_kernel, dim3((unsigned)grid), dim3(256), 0, s, W, ldw, n, 0.0);
     QMLH_CHECK_LAUNCH("cho_invert clear launch");
-    // W := 0 - A A^T = -(L^-T L^-1) on the lower triangle (FP64 MFMA), by row blocks: A = L^-T
-    // is upper triangular, so row block [r0, r1) only needs the products over k >= r0 --
-    // a rectangular update left of the diagonal block and a lower-tile one on it.
-    constexpr int64_t INV_NB = 512;
-    for (int64_t r0 = 0; r0 < n; r0 += INV_NB) {
-        const int64_t rb = n - r0 < INV_NB ? n - r0 : INV_NB;
-        const double* Ar = A + r0 + r0 * lda;  // rows r0.., k from r0
-        QMLH_TRY((launch_update<double>(UPD_FULL, Ar, lda, A + r0 * lda, lda, rb, r0, n - r0,
-                                        W + r0, ldw, nullptr, s)));
-        QMLH_TRY((launch_update<double>(UPD_SYRK, Ar, lda, Ar, lda, rb, rb, n - r0,
-                                        W + r0 + r0 * ldw, ldw, nullptr, s)));
-    }
+    // W := 0 - A A^T = -(L^-T L^-1) on the lower triangle (FP64 MFMA) in one launch over
+    // every lower tile: A = L^-T is upper triangular, so tile row bi only needs the products
+    // over k >= its first row.
+    QMLH_TRY((launch_update<double>(UPD_SYRK_UT, A, lda, A, lda, n, n, n, W, ldw, nullptr, s)));
     hipLaunchKernelGGL(negate_symmetrize_kernel, dim3((unsigned)(tr * (tr + 1) / 2)), dim3(256),
                        0, s, W, ldw, n, A, lda);
     QMLH_CHECK_LAUNCH("cho_invert symmetrize launch");
