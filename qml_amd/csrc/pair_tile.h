@@ -105,6 +105,8 @@ __device__ __forceinline__ void pair_load_chunk(double (*s)[PT_LDS], const doubl
 
 // acc[r][c] = sum over features of term(A(row0 + ri(r)), B(col0 + cj(c))) with
 // ri(r) = 2 tx + 32 (r / 2) + r % 2 and cj(c) = 2 ty + 32 (c / 2) + c % 2. Has barriers.
+// The next chunk's global loads are issued into registers before the current chunk's
+// products and written to LDS after them, so their latency hides behind the arithmetic.
 template <int OP>
 __device__ __forceinline__ void pair_tile_accumulate(double (&acc)[8][8], double (*sA)[PT_LDS],
                                                      double (*sB)[PT_LDS],
@@ -120,11 +122,34 @@ __device__ __forceinline__ void pair_tile_accumulate(double (&acc)[8][8], double
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
 
+    // chunk staging: lane t holds feature t % 16 of rows t / 16 + 16 q (q < 8), as
+    // pair_load_chunk; rows >= n and features >= d read as zero
+    const int kl = t % PT_BK;
+    const int rbase = t / PT_BK;
+    double pa[PT_BM / 16], pb[PT_BN / 16];
+    auto fetch = [&](int64_t k0) {
+        const int64_t kk = k0 + kl;
+#pragma unroll
+        for (int q = 0; q < PT_BM / 16; ++q) {
+            const int64_t ra = row0 + rbase + 16 * q, rb = col0 + rbase + 16 * q;
+            pa[q] = (ra < na && kk < d) ? A[ra * lda + kk] : 0.0;
+            pb[q] = (rb < nb && kk < d) ? B[rb * ldb + kk] : 0.0;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < PT_BM / 16; ++q) {
+            sA[kl][rbase + 16 * q] = pa[q];
+            sB[kl][rbase + 16 * q] = pb[q];
+        }
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
     for (int64_t k0 = 0; k0 < d; k0 += PT_BK) {
-        pair_load_chunk(sA, A, lda, na, d, row0, k0, t);
-        pair_load_chunk(sB, B, ldb, nb, d, col0, k0, t);
-        __syncthreads();
-#pragma unroll 4
+        const bool more = k0 + PT_BK < d;
+        if (more) fetch(k0 + PT_BK);
+#pragma unroll 2
         for (int k = 0; k < PT_BK; ++k) {
             double a[8], b[8];
 #pragma unroll
@@ -142,7 +167,11 @@ __device__ __forceinline__ void pair_tile_accumulate(double (&acc)[8][8], double
                 for (int c = 0; c < 8; ++c)
                     acc[r][c] = pair_term<OP>(acc[r][c], a[r], b[c], p);
         }
-        __syncthreads();
+        __syncthreads();  // every lane has read the chunk
+        if (more) {
+            stash();
+            __syncthreads();
+        }
     }
 }
 
