@@ -167,7 +167,8 @@ def roofline(timers, steps, factor):
 def run_kernels(log):
     """Kernel-matrix builds at the large-configuration shapes: the Laplacian (FP64 VALU) on a
     row block of 8192 queries against 100,000 training rows (size 29, d = 435, sigma = 724),
-    the symmetric Gaussian (FP64 MFMA expanded form) at N = 20,000 (sigma = 4000), the linear
+    the symmetric Gaussian (FP64 MFMA expanded form) and Laplacian (sigma = 724) at N = 20,000
+    (sigma = 4000), the linear
     kernel on the same tiles, and the Sargan / Matern (L2) kernels and p-distances (p = 3 by
     products, p = 2.5 by pow on a 1024-row block) at the Laplacian's shape."""
     import torch
@@ -212,6 +213,19 @@ def run_kernels(log):
     out["gaussian_mfma_20k"] = {"shape": f"{n} x {n} symmetric, d={d}", "ms": ms,
                                 "tflops": flops / (ms * 1e-3) / 1e12,
                                 "frac_of_fp64_mfma": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
+    # Laplacian, symmetric 20k (the training-kernel shape of the drop-in and of the fits):
+    # upper tiles computed, mirrored through LDS.
+    pairwise_device(_lib.PAIR_LAPLACIAN, X2, n, X2, n, d, 724.0, True, K2)
+    e0.record()
+    for _ in range(reps):
+        pairwise_device(_lib.PAIR_LAPLACIAN, X2, n, X2, n, d, 724.0, True, K2)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    pf = n * (n + 1) / 2.0 * d
+    out["laplacian_symmetric_20k"] = {
+        "shape": f"{n} x {n} symmetric, d={d}", "ms": ms, "pair_features_per_s": pf / (ms * 1e-3),
+        "frac_of_fp64_valu": pf / (ms * 1e-3) / 19.65e12}
     # Linear kernel on the same cross-term tiles (qmlh_linear_mfma), symmetric 20k.
     from qml_amd.distance import p_distance  # noqa: F401  (drop-in; device form used below)
     from qml_amd.kernels import pairwise_ext_device
