@@ -30,7 +30,7 @@ namespace qmlh {
 static constexpr int PF_NB = 512;
 // Block size when the trailing updates run as split-FP16 products: the update of C costs a
 // fixed read and write of the FP32 trailing matrix per step, so a deeper update pays off.
-static constexpr int PF_NB_SPLIT = 512;
+static constexpr int PF_NB_SPLIT = 768;
 static constexpr int PF_IB = CP_B;
 static constexpr int UP_TILE = WG_T;
 
