@@ -53,6 +53,47 @@ def train_relres(model, y) -> float:
     return float(r.norm() / yd.norm())
 
 
+def kernel_entry_error(Xd, d, sigma, lam, fp64_cols=1, rows=2048):
+    """Kernel-entry error of the mixed fit's two representations on the first `rows` training
+    molecules, against the direct-difference FP64 Gaussian (the form of qml/fkernels.f90:340-372,
+    pinned to the oracle by the tests): K64 (FP64 MFMA cross term, used for the residuals) and
+    B32 + V V^T (the FP32 deflated matrix and its rank-(d+1) term as the preconditioner holds it:
+    the first fp64_cols columns of V in FP64, the rest rounded to FP32)."""
+    import torch
+
+    from qml_amd import _lib
+    from qml_amd.kernels import pairwise_device, row_sqnorms_device
+
+    L = _lib.lib()
+    n = min(rows, Xd.shape[0])
+    X = Xd[:n].contiguous()
+    dev = X.device
+    norms = row_sqnorms_device(X, n, d)
+    K = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    B = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    V = torch.empty((d + 1, n), dtype=torch.float32, device=dev)
+    u = torch.empty(n, dtype=torch.float64, device=dev)
+    _lib.check(L.qmlh_gaussian_deflated(_lib.ptr(X), n, X.shape[1], _lib.ptr(norms), d, sigma,
+                                        lam, 0.0, _lib.ptr(K), n, _lib.ptr(B), n, _lib.ptr(V), n,
+                                        _lib.ptr(u), _lib.stream_handle()))
+    Kd = pairwise_device(_lib.PAIR_GAUSSIAN, X, n, X, n, d, sigma, True)
+    V64 = torch.empty((d + 1, n), dtype=torch.float64, device=dev)
+    _lib.check(L.qmlh_gaussian_lowrank(_lib.ptr(X), n, X.shape[1], d, sigma, _lib.ptr(u),
+                                       _lib.ptr(V64), n, _lib.stream_handle()))
+    # column-major lower triangles: device tensors are (n, n) row-major views of them
+    # .T of each device tensor is the matrix indexed (row, col); compare the lower triangles
+    low = torch.ones((n, n), dtype=torch.bool, device=dev).tril()
+    Kt = Kd.T + lam * torch.eye(n, dtype=torch.float64, device=dev)
+    k64 = ((K.T - Kt).abs() / Kt.abs())[low].max()
+    Vd = V.double()
+    Vd[:fp64_cols] = V64[:fp64_cols]
+    rep = B.double().T + Vd.T @ Vd  # B32 + V V^T
+    dfl = (rep - Kt)[low].abs().max()
+    return {"rows": n, "reference": "direct-difference FP64 Gaussian on the GPU (pinned to the oracle)",
+            "k64_max_rel": float(k64), "fp32_deflated_max_abs": float(dfl),
+            "fp32_deflated_max_abs_over_lambda": float(dfl) / lam}
+
+
 def _short(x: float) -> str:
     """1e-06 -> 1e-6: the exponent without its leading zero."""
     return (f"{x:g}").replace("e-0", "e-").replace("e+0", "e+")
@@ -78,9 +119,13 @@ def parse():
     # Refinement stops at |y - K alpha| <= tol |y|. 1e-9 keeps the prediction difference to the
     # FP64 fit near 1e-8 of std(y) (measured below), far inside the 1e-6 target.
     p.add_argument("--tol", type=float, default=1e-9)
-    p.add_argument("--serve-rate", type=float, default=500.0,
-                   help="Poisson arrival rate (requests/s) of the serving-load phase")
-    p.add_argument("--serve-seconds", type=float, default=4.0)
+    p.add_argument("--serve-rates", default="200,500,1000,2000",
+                   help="Poisson arrival rates (requests/s) of the serving-load phase")
+    p.add_argument("--serve-seconds", type=float, default=2.0,
+                   help="seconds of load per rate and model (0 = skip)")
+    p.add_argument("--refit-every", type=float, default=0.5,
+                   help="seconds between refit requests (N ~ U[1000, 16000]) under load "
+                        "(0 = none)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="run a device copy of this many MiB first (HBM counter calibration)")
     p.add_argument("--kernels", type=int, default=1,
@@ -734,7 +779,15 @@ def main():
             # Both solutions' residuals, recomputed alike with the prediction kernel over the
             # training rows: which of the two fits is the more accurate at small lambda.
             "relres_recomputed": {"fp64_direct": train_relres(model64, tr.labels),
-                                  "mixed": train_relres(modelmx, tr.labels)}}
+                                  "mixed": train_relres(modelmx, tr.labels)},
+            "backward_error": rep_mx.backward_error,
+            "woodbury_term": modelmx.lowrank_mode()}
+        if args.kernel == "gaussian":
+            Xs_d, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size)
+            mode = modelmx.lowrank_mode()
+            cols = {"fp32": 0, "fp64": d + 1}.get(mode, modelmx.lowrank_cols)
+            mixed_err["kernel_entries"] = kernel_entry_error(Xs_d, d, args.sigma, args.lam, cols)
+            del Xs_d
     fit_s = fitmx_s if fitmx_s is not None else fit64_s
     model = modelmx if modelmx is not None else model64
 
@@ -764,25 +817,27 @@ def main():
     lat = np.array(lat) if lat else np.full(1, np.nan)
     log(f"fit {fit_s:.3f}s; predict p50 {np.percentile(lat, 50):.2f} ms")
 
-    # Serving load: Poisson arrivals into the continuous-batching server, per model.
+    # Serving load: Poisson arrivals into the continuous-batching server, per model and rate,
+    # with refits of N ~ U[1000, 16000] training molecules (the model's precision) submitted
+    # under load and admitted by the free-HBM budget.
     serve_out = {}
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
     for tag, mdl in (("mixed", modelmx), ("fp64", model64)):
         if mdl is None or args.serve_seconds <= 0:
             continue
-        with serve.KRRServer(mdl, size, max_molecules=2048, max_inflight=3) as srv:
-            reqs = serve.poisson_load(srv, te, args.serve_rate, args.serve_seconds,
-                                      np.random.default_rng(9), max_mol=512, size=size)
-            for r in reqs:
-                r.wait(120)
-            sl = np.array([r.latency for r in reqs]) * 1e3
-            span = max(r.t_done for r in reqs) - min(r.t_arrival for r in reqs)
-            serve_out[tag] = {
-                "p50_ms": float(np.percentile(sl, 50)), "p99_ms": float(np.percentile(sl, 99)),
-                "requests": len(reqs), "batches": srv.batches,
-                "molecules_per_s": float(sum(r.nmol for r in reqs) / span),
-                "rate_req_per_s": args.serve_rate}
-        log(f"serve {tag}: p50 {serve_out[tag]['p50_ms']:.2f} ms "
-            f"p99 {serve_out[tag]['p99_ms']:.2f} ms over {len(reqs)} requests")
+        serve_out[tag] = []
+        for rate in rates:
+            with serve.KRRServer(mdl, size, max_molecules=2048, max_inflight=3) as srv:
+                _, _, summ = serve.run_load(
+                    srv, te, rate, args.serve_seconds, np.random.default_rng(9), max_mol=512,
+                    size=size, refit_pool=tr, refit_every=args.refit_every,
+                    refit_precision=mdl.precision)
+            serve_out[tag].append(summ)
+            log(f"serve {tag} {rate:g} req/s: p50 {summ['p50_ms']:.2f} ms p99 "
+                f"{summ['p99_ms']:.2f} ms, {summ['molecules_per_s']:.0f} molecules/s, "
+                f"{summ['molecules_per_batch']:.0f} molecules/batch, "
+                f"{len(summ['refits'])} refits")
+            torch.cuda.empty_cache()
 
     sweep_out = run_sweep(args.sweep, log) if args.sweep > 0 else None
     kernels_out = run_kernels(log) if args.kernels else None

@@ -230,8 +230,13 @@ int qmlh_bkf_solve(const double *A, int64_t n, int64_t lda, const int32_t *ipiv,
  * qmlh_pcg_solve: FP64 preconditioned CG on A x = y (A symmetric, column-major; only its lower
  * triangle is read), with
  * preconditioner M = L L^T + V V^T applied as M^-1 r = L^-T (q - Y C^-T C^-1 Y^T q),
- * q = L^-1 r (r = 0: M = L L^T). Iterates until |y - A x| <= tol |y| or max_iter and reports
- * the iterations and the final relative residual. Synchronises the stream once per iteration.
+ * q = L^-1 r (r = 0: M = L L^T); Y is FP32 (y_fp64 = 0) or FP64 (y_fp64 = 1). Iterates until |y - A x| <= tol |y| or max_iter. The stop
+ * test runs on the CG's recursive residual and is confirmed on the true residual y - A x,
+ * recomputed from A: above tol, the CG restarts from the true residual while each restart at
+ * least halves it and the backward error (below) is above the unit roundoff u = 2^-53 (below
+ * it, x solves a system within the rounding of A's own FP64 entries). relres_h receives the recomputed |y - A x| / |y|; backward_h (nullable) the
+ * normwise backward error |y - A x| / (|A|_inf |x| + |y|) when relres > tol, else -1 (not
+ * computed). Synchronises the stream once per iteration.
  * ---------------------------------------------------------------------------------------- */
 int qmlh_spotrf(float *A, int64_t n, int64_t lda, int32_t *info, qmlh_stream_t stream);
 /* qmlh_spotrf with the O(n^3) trailing updates in split FP16: with the panel scaled by a power
@@ -257,10 +262,13 @@ int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_
  * overwritten), solved as extra panel rows while the trailing updates run (their own updates in
  * FP32 MFMA), and copied back. The factor of A is the same, bit for bit, as without V. On a
  * failed factorisation (info != 0) V is unspecified. Replaces qmlh_spotrf* + the solve of
- * qmlh_woodbury_setup. */
+ * qmlh_woodbury_setup. pivot_floor > 0 selects static pivoting, for factors that only
+ * precondition: a finite pivot at or below pivot_floor is replaced by pivot_floor (the factor
+ * is then of A + a diagonal perturbation in those columns) instead of failing; NaN pivots still
+ * set info. pivot_floor = 0: LAPACK's rule. */
 int qmlh_spotrf_lowrank(float *A, int64_t n, int64_t lda, float *V, int64_t ldv, int64_t r,
-                        int32_t f16x3, int32_t *info, void *ws, size_t ws_bytes,
-                        qmlh_stream_t stream);
+                        int32_t f16x3, double pivot_floor, int32_t *info, void *ws,
+                        size_t ws_bytes, qmlh_stream_t stream);
 /* C = chol(I + Y^T Y) (r x r, lower, FP64; r <= 1024) for Y (n x r, column-major FP32): the
  * rest of qmlh_woodbury_setup once Y = L^-1 V is known. The Gram matrix is summed in FP64 over
  * row slices in a fixed order (deterministic); ws holds
@@ -269,14 +277,41 @@ size_t qmlh_woodbury_capacitance_workspace(int64_t n, int64_t r);
 int qmlh_woodbury_capacitance(const float *Y, int64_t n, int64_t ldy, int64_t r, double *C,
                               int64_t ldc, int32_t *info, void *ws, size_t ws_bytes,
                               qmlh_stream_t stream);
+/* The same capacitance factor for an FP64 Y (qmlh_trsm_f32l). */
+int qmlh_woodbury_capacitance_f64(const double *Y, int64_t n, int64_t ldy, int64_t r, double *C,
+                                  int64_t ldc, int32_t *info, void *ws, size_t ws_bytes,
+                                  qmlh_stream_t stream);
+/* V(i, 0) = u_i, V(i, 1 + k) = u_i X(i, k) / sigma (n x (d+1), column-major FP64; u as written
+ * by qmlh_gaussian_deflated): the deflation's rank-(d+1) term at full precision. */
+int qmlh_gaussian_lowrank(const double *X, int64_t n, int64_t ldx, int64_t d, double sigma,
+                          const double *u, double *V, int64_t ldv, qmlh_stream_t stream);
+/* V := L^-1 V for an FP32 lower factor L (column-major) and an FP64 right-hand side V (n x nrhs,
+ * column-major), in FP64 arithmetic (FP64 inverses of the 64 x 64 diagonal blocks, FP64 MFMA
+ * block updates on L widened to FP64 one 256-wide block column at a time): L^-1 V of the
+ * stored factor to FP64 accuracy, for the Woodbury term of qmlh_pcg_solve with y_fp64 = 1.
+ * ws: qmlh_trsm_f32l_workspace(n) bytes. */
+size_t qmlh_trsm_f32l_workspace(int64_t n);
+int qmlh_trsm_f32l(const float *L, int64_t n, int64_t ldl, double *V, int64_t ldv, int64_t nrhs,
+                   void *ws, size_t ws_bytes, qmlh_stream_t stream);
+/* The hybrid Woodbury term: Y (n x r, column-major FP64) gets columns k .. r-1 widened from
+ * the factorisation's FP32 solve Y32 = L^-1 V (qmlh_spotrf_lowrank), and columns 0 .. k-1
+ * solved again in FP64 arithmetic with the FP32 factor L from the FP64 right-hand sides Y holds
+ * there on entry (qmlh_gaussian_lowrank's leading columns; u first). One single-vector solve
+ * per FP64 column. ws: qmlh_lowrank_hybrid_workspace(n) bytes. */
+size_t qmlh_lowrank_hybrid_workspace(int64_t n);
+int qmlh_lowrank_hybrid(const float *L, int64_t n, int64_t ldl, const float *Y32, int64_t ldy32,
+                        double *Y, int64_t ldy, int64_t r, int64_t k, void *ws, size_t ws_bytes,
+                        qmlh_stream_t stream);
 /* B(i, j) = (float)(A(i, j) + shift [i == j]) on the lower triangle (column-major). */
 int qmlh_lower_to_f32(const double *A, int64_t n, int64_t lda, float *B, int64_t ldb,
                       double shift, qmlh_stream_t stream);
 size_t qmlh_pcg_workspace(int64_t n, int64_t r);
 int qmlh_pcg_solve(const double *A, int64_t n, int64_t lda, const float *L, int64_t ldl,
-                   const float *Y, int64_t ldy, int64_t r, const double *C, int64_t ldc,
+                   const void *Y, int64_t ldy, int64_t r, int32_t y_fp64, const double *C,
+                   int64_t ldc,
                    const double *y, double *x, double tol, int32_t max_iter, void *ws,
-                   size_t ws_bytes, int32_t *iters_h, double *relres_h, qmlh_stream_t stream);
+                   size_t ws_bytes, int32_t *iters_h, double *relres_h, double *backward_h,
+                   qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Fused KRR prediction y_q = sum_j k(Xq_q, Xt_j) alpha_j without materialising the

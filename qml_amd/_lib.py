@@ -82,8 +82,8 @@ _SIGNATURES = {
     "qmlh_gaussian_deflated": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _vp,
                                               _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "qmlh_woodbury_workspace": (_sz, [_i64]),
-    "qmlh_spotrf_lowrank": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _vp,
-                                           _sz, _vp]),
+    "qmlh_spotrf_lowrank": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i32, _f64, _vp,
+                                           _vp, _sz, _vp]),
     "qmlh_woodbury_capacitance_workspace": (_sz, [_i64, _i64]),
     "qmlh_woodbury_capacitance": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp, _sz,
                                                  _vp]),
@@ -91,9 +91,17 @@ _SIGNATURES = {
                                            _vp, _sz, _vp]),
     "qmlh_lower_to_f32": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _f64, _vp]),
     "qmlh_pcg_workspace": (_sz, [_i64, _i64]),
-    "qmlh_pcg_solve": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _i64,
-                                      _vp, _vp, _f64, _i32, _vp, _sz, ctypes.POINTER(_i32),
-                                      ctypes.POINTER(_f64), _vp]),
+    "qmlh_pcg_solve": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _i32, _vp,
+                                      _i64, _vp, _vp, _f64, _i32, _vp, _sz, ctypes.POINTER(_i32),
+                                      ctypes.POINTER(_f64), ctypes.POINTER(_f64), _vp]),
+    "qmlh_woodbury_capacitance_f64": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp,
+                                                     _sz, _vp]),
+    "qmlh_gaussian_lowrank": (ctypes.c_int, [_vp, _i64, _i64, _i64, _f64, _vp, _vp, _i64, _vp]),
+    "qmlh_lowrank_hybrid_workspace": (_sz, [_i64]),
+    "qmlh_lowrank_hybrid": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _vp,
+                                           _sz, _vp]),
+    "qmlh_trsm_f32l_workspace": (_sz, [_i64]),
+    "qmlh_trsm_f32l": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _sz, _vp]),
     "qmlh_add_diagonal": (ctypes.c_int, [_vp, _i64, _i64, _f64, _vp]),
     "qmlh_predict_workspace": (_sz, [_i64, _i64]),
     "qmlh_krr_predict": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp,

@@ -81,8 +81,9 @@ __device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __re
 // scratch). Called by all threads of the workgroup (barriers); waves 1.. only wait. On exit sp
 // holds L in its lower triangle. Returns 0, or c + 1 for the first non-positive (or NaN) pivot
 // c, with the Schur value left on that diagonal entry (as LAPACK leaves it); the value is the
-// same in every lane.
-__device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w) {
+// same in every lane. floor > 0 (static pivoting, for factors that only precondition): a
+// finite pivot at or below floor is replaced by floor instead of failing; NaN still fails.
+__device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w, double floor = 0.0) {
     __shared__ double colbuf[CP_B];
     __shared__ int s_fail;
     const int t = threadIdx.x;
@@ -98,7 +99,8 @@ __device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w) {
 #pragma unroll
         for (int c = 0; c < CP_B; ++c) {
             if (c >= w || f) break;
-            const double piv = readlane_f64(a[c], c);
+            double piv = readlane_f64(a[c], c);
+            if (!(piv > floor) && floor > 0.0 && piv == piv) piv = floor;
             if (!(piv > 0.0)) {
                 f = c + 1;
                 break;
@@ -227,12 +229,12 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_
 template <typename T>
 __device__ __forceinline__ void potf2_body_sp(T* __restrict__ A, int64_t lda, int w,
                                               int64_t col_base, int32_t* __restrict__ info,
-                                              double* __restrict__ sp) {
+                                              double* __restrict__ sp, double floor = 0.0) {
     if (*info) return;
     const int t = threadIdx.x;
     stage_packed64(sp, A, lda, w);
     __syncthreads();
-    const int fail = potf2_wave64(sp, w);
+    const int fail = potf2_wave64(sp, w, floor);
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
@@ -244,9 +246,10 @@ __device__ __forceinline__ void potf2_body_sp(T* __restrict__ A, int64_t lda, in
 
 template <typename T>
 __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
-                                           int64_t col_base, int32_t* __restrict__ info) {
+                                           int64_t col_base, int32_t* __restrict__ info,
+                                           double floor = 0.0) {
     __shared__ double sp[CP_PACK];
-    potf2_body_sp<T>(A, lda, w, col_base, info, sp);
+    potf2_body_sp<T>(A, lda, w, col_base, info, sp, floor);
 }
 
 // Rows of a sub-panel, workgroup `local` of the launch: rows 256 local .. of the m rows below

@@ -56,6 +56,19 @@ __global__ void lowrank_kernel(const double* __restrict__ X, int64_t ldx, int64_
     for (int64_t k = 0; k < d; ++k) V[i + (k + 1) * ldv] = (float)(ui * x[k] * inv_sigma);
 }
 
+// V(i, 0) = u_i, V(i, 1 + k) = u_i x_ik / sigma (column-major, FP64): the rank-(d+1) term of
+// the deflation at full precision, for preconditioners that keep it in FP64.
+__global__ void lowrank64_kernel(const double* __restrict__ X, int64_t ldx, int64_t n, int64_t d,
+                                 const double* __restrict__ u, double inv_sigma,
+                                 double* __restrict__ V, int64_t ldv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ui = u[i];
+    V[i] = ui;
+    const double* x = X + i * ldx;
+    for (int64_t k = 0; k < d; ++k) V[i + (k + 1) * ldv] = ui * x[k] * inv_sigma;
+}
+
 __device__ __forceinline__ void tile_coords(int64_t bid, int symmetric, int64_t tiles_m,
                                             int64_t& bi, int64_t& bj) {
     if (symmetric) {
@@ -311,5 +324,19 @@ extern "C" int qmlh_gaussian_deflated(const double* X, int64_t n, int64_t ldx,
     hipLaunchKernelGGL(gaussian_deflated_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, X, ldx,
                        n, norms, u, d, scale, lambda, shift, K, ldk, B32, ldb);
     QMLH_CHECK_LAUNCH("gaussian_deflated launch");
+    return QMLH_OK;
+}
+
+extern "C" int qmlh_gaussian_lowrank(const double* X, int64_t n, int64_t ldx, int64_t d,
+                                     double sigma, const double* u, double* V, int64_t ldv,
+                                     qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || d < 0 || ldx < d || ldv < n)
+        return set_error(QMLH_ERR_ARG, "gaussian_lowrank: bad shape");
+    if (n == 0) return QMLH_OK;
+    if (!X || !u || !V) return set_error(QMLH_ERR_ARG, "gaussian_lowrank: null pointer");
+    hipLaunchKernelGGL(lowrank64_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, X, ldx, n, d, u, 1.0 / sigma, V, ldv);
+    QMLH_CHECK_LAUNCH("gaussian_lowrank launch");
     return QMLH_OK;
 }

@@ -8,9 +8,22 @@
 // FP64 solution; the preconditioner only sets how fast. (Iterative refinement with the
 // correction equation solved by CG: CG-IR.)
 //
+// Stopping. The CG recurrence updates r recursively (r -= alpha A p), and that vector drifts
+// from y - A x in finite precision. When the recursive residual meets the tolerance, the true
+// residual y - A x is recomputed from A; if it is above the tolerance the CG restarts from it
+// (preconditioner applied to the true residual) for as long as each restart at least halves it
+// and iterations remain. The reported relative residual is always the recomputed one. When it
+// stays above the tolerance -- the attainable floor of an FP64 residual, about
+// u |A| |x| / |y|, lies above a tight tolerance when |x| is large -- the normwise backward
+// error |y - A x| / (|A|_inf |x| + |y|) is computed as well, so that a caller can tell a
+// solution at the FP64 floor (backward error below the unit roundoff, as a backward-stable
+// FP64 Cholesky solve reaches) from one that did not converge.
+//
 // Reductions are deterministic: a fixed number of workgroups each reduce a fixed contiguous
 // range in a fixed tree order, and one workgroup adds the partials in order. PCG scalars stay
 // on the device; the host reads one residual norm per iteration to decide convergence.
+#include <cfloat>
+
 #include "linalg.h"
 #include "timer.h"
 
@@ -71,6 +84,8 @@ static inline size_t symv_ws_doubles(int64_t n) {
     return (size_t)(symv_blocks(n) * n + symv_blocks(n) * symv_chunks(n) * SY_B);
 }
 
+// ABS: the same product with |A| and |x| (the scale of a backward error).
+template <bool ABS>
 __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restrict__ A, int64_t lda,
                                                          int64_t n, const double* __restrict__ x,
                                                          double* __restrict__ part_r,
@@ -87,7 +102,7 @@ __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restric
     const int t = threadIdx.x;
     const int lane = t & 63, q = t >> 6;
     const int64_t col0 = J * SY_B;
-    if (t < SY_B) xj[t] = col0 + t < n ? x[col0 + t] : 0.0;
+    if (t < SY_B) xj[t] = col0 + t < n ? (ABS ? fabs(x[col0 + t]) : x[col0 + t]) : 0.0;
     double csum = 0.0;  // column lane, rows 16 q .. 16 q + 15 of every tile, in tile order
     for (int64_t blk = first; blk < last; ++blk) {
         const int64_t row0 = blk * SY_B;
@@ -99,6 +114,7 @@ __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restric
             const int r = idx & (SY_B - 1), c = idx / SY_B;  // consecutive lanes: consecutive rows
             const bool ok = row0 + r < n && col0 + c < n && (!diag || r >= c);
             v[k] = ok ? A[(row0 + r) + (col0 + c) * lda] : 0.0;
+            if (ABS) v[k] = fabs(v[k]);
         }
         __syncthreads();  // previous tile's readers are done
 #pragma unroll
@@ -106,7 +122,7 @@ __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restric
             const int idx = t + 256 * k;
             tile[idx & (SY_B - 1)][idx / SY_B] = v[k];
         }
-        if (t < SY_B) xr[t] = row0 + t < n ? x[row0 + t] : 0.0;
+        if (t < SY_B) xr[t] = row0 + t < n ? (ABS ? fabs(x[row0 + t]) : x[row0 + t]) : 0.0;
         __syncthreads();
         // row part: lane = row, columns 16 q ..
         double rs = 0.0;
@@ -146,11 +162,16 @@ __global__ void symv_reduce_kernel(const double* __restrict__ part_r,
 }
 
 static int symv_lower(const double* A, int64_t lda, int64_t n, const double* x, double* y,
-                      double* ws, hipStream_t s) {
+                      double* ws, hipStream_t s, bool abs_values = false) {
     double* part_r = ws;
     double* part_c = ws + symv_blocks(n) * n;
-    hipLaunchKernelGGL(symv_lower_kernel, dim3((unsigned)symv_chunks(n), (unsigned)symv_blocks(n)),
-                       dim3(256), 0, s, A, lda, n, x, part_r, part_c);
+    const dim3 grid((unsigned)symv_chunks(n), (unsigned)symv_blocks(n));
+    if (abs_values)
+        hipLaunchKernelGGL(symv_lower_kernel<true>, grid, dim3(256), 0, s, A, lda, n, x, part_r,
+                           part_c);
+    else
+        hipLaunchKernelGGL(symv_lower_kernel<false>, grid, dim3(256), 0, s, A, lda, n, x, part_r,
+                           part_c);
     hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
                        part_r, part_c, n, y);
     QMLH_CHECK_LAUNCH("pcg symv launch");
@@ -168,12 +189,50 @@ __global__ void axpy2_kernel(double* __restrict__ x, double* __restrict__ r,
     r[i] = fma(-alpha, q[i], r[i]);
 }
 
-// t_a = sum_i Y(i, a) q_i: one workgroup per column a, fixed-order reduction.
-__global__ __launch_bounds__(256) void yt_q_kernel(const float* __restrict__ Y, int64_t ldy,
+// r = y - q (the true residual once q = A x).
+__global__ void residual_kernel(const double* __restrict__ y, const double* __restrict__ q,
+                                double* __restrict__ r, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = y[i] - q[i];
+}
+
+// Largest entry of a (n entries, all >= 0): per-workgroup maxima, then one workgroup.
+__global__ __launch_bounds__(256) void max_partial_kernel(const double* __restrict__ a, int64_t n,
+                                                          double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int64_t chunk = (n + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    double m = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) m = fmax(m, a[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void max_final_kernel(const double* __restrict__ partial,
+                                                        double* __restrict__ out) {
+    __shared__ double red[256];
+    red[threadIdx.x] = partial[threadIdx.x];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// t_a = sum_i Y(i, a) q_i: one workgroup per column a, fixed-order reduction (Y FP32 or FP64).
+template <typename TY>
+__global__ __launch_bounds__(256) void yt_q_kernel(const TY* __restrict__ Y, int64_t ldy,
                                                    int64_t n, const double* __restrict__ q,
                                                    double* __restrict__ t) {
     __shared__ double red[256];
-    const float* col = Y + (int64_t)blockIdx.x * ldy;
+    const TY* col = Y + (int64_t)blockIdx.x * ldy;
     double s = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 256) s = fma((double)col[i], q[i], s);
     const double tot = block_reduce_256(s, red);
@@ -181,7 +240,8 @@ __global__ __launch_bounds__(256) void yt_q_kernel(const float* __restrict__ Y, 
 }
 
 // q_i -= sum_a Y(i, a) t_a (rank r <= WB_MAX_RANK).
-__global__ __launch_bounds__(256) void q_minus_yt_kernel(const float* __restrict__ Y, int64_t ldy,
+template <typename TY>
+__global__ __launch_bounds__(256) void q_minus_yt_kernel(const TY* __restrict__ Y, int64_t ldy,
                                                          int64_t n, int64_t r,
                                                          const double* __restrict__ t,
                                                          double* __restrict__ q) {
@@ -232,7 +292,8 @@ static int64_t gram_slices(int64_t n) {
     return s < 1 ? 1 : (s > GR_SLICES_MAX ? GR_SLICES_MAX : s);
 }
 
-__global__ __launch_bounds__(256) void gram_partial_kernel(const float* __restrict__ Y,
+template <typename TY>
+__global__ __launch_bounds__(256) void gram_partial_kernel(const TY* __restrict__ Y,
                                                            int64_t ldy, int64_t n, int64_t r,
                                                            int64_t slice_rows,
                                                            double* __restrict__ partial) {
@@ -247,7 +308,7 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(const float* __restri
     const int t = threadIdx.x, ta = t & 15, tb = t >> 4;
     double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int64_t i0 = lo; i0 < hi; i0 += 64) {
-        float ra[8], rb[8];
+        TY ra[8], rb[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {  // element (ii, cc) of the 64 x 32 chunk, ii fastest
             const int idx = t + 256 * q, ii = idx & 63, cc = idx >> 6;
@@ -362,7 +423,8 @@ static int copy(const PcgCtx& c, const double* a, double* b) {
 }
 
 struct Woodbury {
-    const float* Y;
+    const void* Y;  // float, or double with y64
+    bool y64;
     int64_t ldy;
     int64_t r;
     const double* C;  // Cholesky factor of I + Y^T Y
@@ -381,12 +443,20 @@ static int precond(const PcgCtx& c, const float* L, int64_t ldl, const Woodbury&
     else
         QMLH_TRY(trsv_lower_fwd<float>(L, c.n, ldl, tmp, q, nullptr, c.s));
     if (wb.r > 0) {
-        hipLaunchKernelGGL(yt_q_kernel, dim3((unsigned)wb.r), dim3(256), 0, c.s, wb.Y, wb.ldy, c.n,
-                           q, wb.t);
+        if (wb.y64)
+            hipLaunchKernelGGL(yt_q_kernel<double>, dim3((unsigned)wb.r), dim3(256), 0, c.s,
+                               static_cast<const double*>(wb.Y), wb.ldy, c.n, q, wb.t);
+        else
+            hipLaunchKernelGGL(yt_q_kernel<float>, dim3((unsigned)wb.r), dim3(256), 0, c.s,
+                               static_cast<const float*>(wb.Y), wb.ldy, c.n, q, wb.t);
         QMLH_CHECK_LAUNCH("pcg Y^T q launch");
         QMLH_TRY(potrs_impl<double>(wb.C, wb.r, wb.ldc, wb.t, wb.tmp_r, nullptr, c.s));
-        hipLaunchKernelGGL(q_minus_yt_kernel, dim3(grid1(c.n)), dim3(256), 0, c.s, wb.Y, wb.ldy,
-                           c.n, wb.r, wb.t, q);
+        if (wb.y64)
+            hipLaunchKernelGGL(q_minus_yt_kernel<double>, dim3(grid1(c.n)), dim3(256), 0, c.s,
+                               static_cast<const double*>(wb.Y), wb.ldy, c.n, wb.r, wb.t, q);
+        else
+            hipLaunchKernelGGL(q_minus_yt_kernel<float>, dim3(grid1(c.n)), dim3(256), 0, c.s,
+                               static_cast<const float*>(wb.Y), wb.ldy, c.n, wb.r, wb.t, q);
         QMLH_CHECK_LAUNCH("pcg q - Y t launch");
     }
     if (big) return trsv_syncfree128<float>(L, c.n, ldl, q, z, c.sync, false, c.s, c.dinv128);
@@ -452,10 +522,11 @@ extern "C" size_t qmlh_woodbury_capacitance_workspace(int64_t n, int64_t r) {
     return (size_t)qmlh::gram_slices(n) * (size_t)r * (size_t)r * sizeof(double);
 }
 
-extern "C" int qmlh_woodbury_capacitance(const float* Y, int64_t n, int64_t ldy, int64_t r,
-                                         double* C, int64_t ldc, int32_t* info, void* ws,
-                                         size_t ws_bytes, qmlh_stream_t stream) {
-    using namespace qmlh;
+namespace qmlh {
+template <typename TY>
+static int woodbury_capacitance_impl(const TY* Y, int64_t n, int64_t ldy, int64_t r, double* C,
+                                     int64_t ldc, int32_t* info, void* ws, size_t ws_bytes,
+                                     hipStream_t s) {
     if (n < 0 || r < 0 || ldy < (n > 1 ? n : 1) || ldc < r)
         return set_error(QMLH_ERR_ARG, "woodbury_capacitance: bad shape");
     if (r > WB_MAX_RANK)
@@ -466,13 +537,12 @@ extern "C" int qmlh_woodbury_capacitance(const float* Y, int64_t n, int64_t ldy,
         return set_error(QMLH_ERR_ARG, "woodbury_capacitance: null pointer");
     if (ws_bytes < qmlh_woodbury_capacitance_workspace(n, r))
         return set_error(QMLH_ERR_WORKSPACE, "woodbury_capacitance: workspace too small");
-    hipStream_t s = (hipStream_t)stream;
     double* partial = static_cast<double*>(ws);
     const int64_t slices = n > 0 ? gram_slices(n) : 0;
     if (slices > 0) {
         const int64_t nbk = ceil_div(r, (int64_t)GR_T);
         const int64_t slice_rows = ceil_div(ceil_div(n, slices), (int64_t)64) * 64;
-        hipLaunchKernelGGL(gram_partial_kernel, dim3((unsigned)(nbk * (nbk + 1) / 2), (unsigned)slices),
+        hipLaunchKernelGGL(gram_partial_kernel<TY>, dim3((unsigned)(nbk * (nbk + 1) / 2), (unsigned)slices),
                            dim3(256), 0, s, Y, ldy, n, r, slice_rows, partial);
         QMLH_CHECK_LAUNCH("woodbury gram launch");
     }
@@ -480,6 +550,21 @@ extern "C" int qmlh_woodbury_capacitance(const float* Y, int64_t n, int64_t ldy,
                        0, s, partial, r, slices, C, ldc);
     QMLH_CHECK_LAUNCH("woodbury gram reduce launch");
     return potrf_impl<double>(C, r, ldc, info, s);
+}
+}  // namespace qmlh
+
+extern "C" int qmlh_woodbury_capacitance(const float* Y, int64_t n, int64_t ldy, int64_t r,
+                                         double* C, int64_t ldc, int32_t* info, void* ws,
+                                         size_t ws_bytes, qmlh_stream_t stream) {
+    return qmlh::woodbury_capacitance_impl<float>(Y, n, ldy, r, C, ldc, info, ws, ws_bytes,
+                                                  (hipStream_t)stream);
+}
+
+extern "C" int qmlh_woodbury_capacitance_f64(const double* Y, int64_t n, int64_t ldy, int64_t r,
+                                             double* C, int64_t ldc, int32_t* info, void* ws,
+                                             size_t ws_bytes, qmlh_stream_t stream) {
+    return qmlh::woodbury_capacitance_impl<double>(Y, n, ldy, r, C, ldc, info, ws, ws_bytes,
+                                                   (hipStream_t)stream);
 }
 
 extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
@@ -492,16 +577,18 @@ extern "C" size_t qmlh_pcg_workspace(int64_t n, int64_t r) {
 }
 
 extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const float* L,
-                              int64_t ldl, const float* Y, int64_t ldy, int64_t r,
+                              int64_t ldl, const void* Y, int64_t ldy, int64_t r, int32_t y_fp64,
                               const double* C, int64_t ldc, const double* y, double* x,
                               double tol, int32_t max_iter, void* ws, size_t ws_bytes,
-                              int32_t* iters_h, double* relres_h, qmlh_stream_t stream) {
+                              int32_t* iters_h, double* relres_h, double* backward_h,
+                              qmlh_stream_t stream) {
     using namespace qmlh;
     if (n < 0 || lda < n || ldl < n || r < 0) return set_error(QMLH_ERR_ARG, "pcg_solve: bad shape");
     if (r > 0 && (!Y || !C || ldy < n || ldc < r || r > WB_MAX_RANK))
         return set_error(QMLH_ERR_ARG, "pcg_solve: bad low-rank term");
     if (iters_h) *iters_h = 0;
     if (relres_h) *relres_h = 0.0;
+    if (backward_h) *backward_h = 0.0;
     if (n == 0) return QMLH_OK;
     if (!A || !L || !y || !x || !ws) return set_error(QMLH_ERR_ARG, "pcg_solve: null pointer");
     if (ws_bytes < qmlh_pcg_workspace(n, r))
@@ -516,13 +603,18 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     double* t = tmp + n;
     double* tmp_r = t + r;
     double* partial = tmp_r + r;
-    double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 6 rz_new
+    double* sc = partial + RED_BLOCKS;  // 8 scalars: 0 rz, 1 pq, 2 rr, 3 |A|_inf, 6 rz_new, 7 xx
     double* dinv = sc + 8;
     double* dinv128 = dinv + trsv_dinv_doubles(n);
     double* symws = dinv128 + trsv_dinv128_doubles(n);
     int32_t* sync = reinterpret_cast<int32_t*>(symws + symv_ws_doubles(n));
     PcgCtx c{s, n, partial, sync, dinv128};
-    Woodbury wb{Y, ldy, r, C, ldc, t, tmp_r};
+    Woodbury wb{Y, y_fp64 != 0, ldy, r, C, ldc, t, tmp_r};
+    auto read_scalar = [&](const double* dev, double* host, const char* what) -> int {
+        QMLH_TRY(check_hip(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, s),
+                           what));
+        return check_hip(hipStreamSynchronize(s), "pcg: sync");
+    };
     // The preconditioner's triangular solves use explicit FP64 inverses of the 128 x 128
     // diagonal blocks of the low-precision factor (computed once here), so each block's
     // diagonal solve is one parallel product instead of a sequential substitution.
@@ -530,41 +622,79 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
 
     double yy_h = 0.0;
     QMLH_TRY(dot(c, y, y, sc + 2));
-    QMLH_TRY(check_hip(hipMemcpyAsync(&yy_h, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s),
-                       "pcg: read |y|"));
-    QMLH_TRY(check_hip(hipStreamSynchronize(s), "pcg: sync"));
+    QMLH_TRY(read_scalar(sc + 2, &yy_h, "pcg: read |y|"));
     const double ynorm = sqrt(yy_h);
     hipLaunchKernelGGL(fill_kernel, dim3(grid1(n)), dim3(256), 0, s, x, 0.0, n);
     QMLH_CHECK_LAUNCH("pcg fill launch");
     if (ynorm == 0.0) return QMLH_OK;
+    // (Re)start from the residual in rv: z = M^-1 r, p = z, rz = r.z.
+    auto restart = [&]() -> int {
+        QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
+        QMLH_TRY(dot(c, rv, z, sc + 0));
+        hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 1);
+        QMLH_CHECK_LAUNCH("pcg p launch");
+        return QMLH_OK;
+    };
     QMLH_TRY(copy(c, y, rv));
-    QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
-    QMLH_TRY(dot(c, rv, z, sc + 0));
-    hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 1);
-    QMLH_CHECK_LAUNCH("pcg p launch");
-    double relres = 1.0;
+    QMLH_TRY(restart());
+    double relres = 1.0, prev_true = HUGE_VAL, anorm = -1.0, backward = -1.0;
     int it = 0;
-    while (it < max_iter) {
-        ++it;
-        QMLH_TRY(symv_lower(A, lda, n, p, q, symws, s));
-        QMLH_TRY(dot(c, p, q, sc + 1));
-        hipLaunchKernelGGL(axpy2_kernel, dim3(grid1(n)), dim3(256), 0, s, x, rv, p, q, sc, n);
-        QMLH_CHECK_LAUNCH("pcg axpy launch");
+    for (;;) {
+        while (it < max_iter) {
+            ++it;
+            QMLH_TRY(symv_lower(A, lda, n, p, q, symws, s));
+            QMLH_TRY(dot(c, p, q, sc + 1));
+            hipLaunchKernelGGL(axpy2_kernel, dim3(grid1(n)), dim3(256), 0, s, x, rv, p, q, sc, n);
+            QMLH_CHECK_LAUNCH("pcg axpy launch");
+            QMLH_TRY(dot(c, rv, rv, sc + 2));
+            double rr_h = 0.0;
+            QMLH_TRY(read_scalar(sc + 2, &rr_h, "pcg: read |r|"));
+            relres = sqrt(rr_h) / ynorm;
+            if (!(relres > tol)) break;  // converged (or NaN: stop and check)
+            QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
+            QMLH_TRY(dot(c, rv, z, sc + 6));
+            hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 0);
+            QMLH_CHECK_LAUNCH("pcg p launch");
+            hipLaunchKernelGGL(rz_roll_kernel, dim3(1), dim3(1), 0, s, sc);
+            QMLH_CHECK_LAUNCH("pcg roll launch");
+        }
+        // true residual r = y - A x
+        QMLH_TRY(symv_lower(A, lda, n, x, q, symws, s));
+        hipLaunchKernelGGL(residual_kernel, dim3(grid1(n)), dim3(256), 0, s, y, q, rv, n);
+        QMLH_CHECK_LAUNCH("pcg residual launch");
         QMLH_TRY(dot(c, rv, rv, sc + 2));
         double rr_h = 0.0;
-        QMLH_TRY(check_hip(hipMemcpyAsync(&rr_h, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s),
-                           "pcg: read |r|"));
-        QMLH_TRY(check_hip(hipStreamSynchronize(s), "pcg: sync"));
+        QMLH_TRY(read_scalar(sc + 2, &rr_h, "pcg: read true |r|"));
         relres = sqrt(rr_h) / ynorm;
-        if (!(relres > tol)) break;  // converged (or NaN: stop and report)
-        QMLH_TRY(precond(c, L, ldl, wb, rv, z, q, tmp));
-        QMLH_TRY(dot(c, rv, z, sc + 6));
-        hipLaunchKernelGGL(p_update_kernel, dim3(grid1(n)), dim3(256), 0, s, p, z, sc, n, 0);
-        QMLH_CHECK_LAUNCH("pcg p launch");
-        hipLaunchKernelGGL(rz_roll_kernel, dim3(1), dim3(1), 0, s, sc);
-        QMLH_CHECK_LAUNCH("pcg roll launch");
+        if (!(relres > tol)) {
+            backward = -1.0;  // not computed: the residual test passed
+            break;
+        }
+        // normwise backward error; |A|_inf = max_i (|A| 1)_i, the lower triangle read as the
+        // symmetric matrix (computed once)
+        if (anorm < 0.0) {
+            hipLaunchKernelGGL(fill_kernel, dim3(grid1(n)), dim3(256), 0, s, tmp, 1.0, n);
+            QMLH_CHECK_LAUNCH("pcg fill launch");
+            QMLH_TRY(symv_lower(A, lda, n, tmp, q, symws, s, true));
+            hipLaunchKernelGGL(max_partial_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, q, n, partial);
+            hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, s, partial, sc + 3);
+            QMLH_CHECK_LAUNCH("pcg norm launch");
+            QMLH_TRY(read_scalar(sc + 3, &anorm, "pcg: read |A|"));
+        }
+        QMLH_TRY(dot(c, x, x, sc + 7));
+        double xx = 0.0;
+        QMLH_TRY(read_scalar(sc + 7, &xx, "pcg: read |x|"));
+        backward = relres * ynorm / (anorm * sqrt(xx) + ynorm);
+        // At the FP64 floor another restart cannot lower the residual: a backward error below
+        // the unit roundoff u = 2^-53 means x solves a system whose matrix is within the
+        // rounding of A's own FP64 entries.
+        const bool at_floor = backward <= 0.5 * DBL_EPSILON;
+        if (at_floor || it >= max_iter || !(relres < 0.5 * prev_true)) break;
+        prev_true = relres;
+        QMLH_TRY(restart());
     }
     if (iters_h) *iters_h = it;
     if (relres_h) *relres_h = relres;
+    if (backward_h) *backward_h = backward;
     return QMLH_OK;
 }

@@ -18,7 +18,9 @@
 // A device int32 `info` follows LAPACK: the first non-positive (or NaN) pivot j sets
 // info = j + 1, its Schur-complement value is left on the diagonal, and every later kernel of
 // the factorisation returns without work.
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "chol_panel.h"
 #include "linalg.h"
@@ -55,7 +57,7 @@ template <typename T, int MODE, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
     int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, int32_t* __restrict__ info,
-    int64_t tiles_m, int w_next, int64_t col_base_next) {
+    int64_t tiles_m, int w_next, int64_t col_base_next, double pivot_floor) {
     __shared__ __attribute__((aligned(16))) T lds[WG_LDS_ELEMS];
     constexpr bool MASK = MODE != UPD_FULL;
     if (info && *info) return;
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
         // every wave of this workgroup got here; the barrier makes the tile's stores visible
         // to the whole workgroup and frees the operand LDS for the factor
         __syncthreads();
-        potf2_body_sp<T>(C, ldc, w_next, col_base_next, info, reinterpret_cast<double*>(lds));
+        potf2_body_sp<T>(C, ldc, w_next, col_base_next, info, reinterpret_cast<double*>(lds),
+                         pivot_floor);
     }
 }
 
@@ -100,7 +103,7 @@ template <typename T, int LB = LAYOUT_COL>
 static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
                          int64_t N, int64_t Kd, T* C, int64_t ldc, int32_t* info,
                          hipStream_t s, int timer_class = -1, int w_next = 0,
-                         int64_t col_base_next = 0) {
+                         int64_t col_base_next = 0, double pivot_floor = 0.0) {
     if (M <= 0 || N <= 0 || Kd <= 0) return QMLH_OK;
     const int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
     if (timer_class < 0) timer_class = mode == UPD_SYRK ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
@@ -113,27 +116,27 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
         case UPD_SYRK:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0);
+                               (int64_t)0, 0.0);
             break;
         case UPD_SYRK_UT:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK_UT, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0);
+                               (int64_t)0, 0.0);
             break;
         case UPD_TRAP:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0);
+                               (int64_t)0, 0.0);
             break;
         case UPD_TRAP_POTF2:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP_POTF2, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, w_next,
-                               col_base_next);
+                               col_base_next, pivot_floor);
             break;
         default:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_FULL, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0);
+                               (int64_t)0, 0.0);
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
@@ -142,8 +145,9 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
 // 64-wide sub-panel: the diagonal block (one workgroup), then the rows below it.
 template <typename T>
 __global__ __launch_bounds__(256) void potf2_kernel(T* __restrict__ A, int64_t lda, int w,
-                                                    int64_t col_base, int32_t* __restrict__ info) {
-    potf2_body<T>(A, lda, w, col_base, info);
+                                                    int64_t col_base, int32_t* __restrict__ info,
+                                                    double pivot_floor) {
+    potf2_body<T>(A, lda, w, col_base, info, pivot_floor);
 }
 
 template <typename T>
@@ -163,14 +167,16 @@ __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 // (UPD_TRAP_POTF2), so only the first diagonal block of the column has a launch of its own.
 template <typename T>
 static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64_t kb,
-                               int32_t* info, hipStream_t s, hipEvent_t rows_wait = nullptr) {
+                               int32_t* info, hipStream_t s, hipEvent_t rows_wait = nullptr,
+                               double pivot_floor = 0.0) {
     for (int64_t jj = 0; jj < kb; jj += PF_IB) {
         const int64_t c = k + jj;
         const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
         T* Acc = A + c + c * lda;
         const int64_t m = rows - c - w;  // rows below the diagonal block
         if (jj == 0) {
-            hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Acc, lda, w, c, info);
+            hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Acc, lda, w, c, info,
+                               pivot_floor);
             QMLH_CHECK_LAUNCH("potf2 launch");
         }
         if (jj == 0 && rows_wait)  // the rows' last update may still run on another stream
@@ -184,7 +190,8 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
         if (ncols > 0) {
             const int w2 = (int)(ncols < PF_IB ? ncols : PF_IB);
             QMLH_TRY(launch_update<T>(UPD_TRAP_POTF2, Acc + w, lda, Acc + w, lda, m, ncols, w,
-                                      Acc + w + w * lda, lda, info, s, -1, w2, c + w));
+                                      Acc + w + w * lda, lda, info, s, -1, w2, c + w,
+                                      pivot_floor));
         }
     }
     return QMLH_OK;
@@ -214,6 +221,14 @@ static int trailing_syrk(T* P, int64_t lda, int64_t m, int64_t kb, T* C, int32_t
 
 // Matrices at least this large are factored with look-ahead.
 static constexpr int64_t PF_LOOKAHEAD_MIN = 4096;
+// Split-FP16 steps whose trailing matrix has at least this many rows skip the look-ahead: the
+// whole trailing triangle takes the dense update variant (three workgroups per CU, no
+// registers left for a concurrent panel), then the next block column is factored alone. The
+// update's saving grows as m^2 and passes the exposed panel time (about 0.8 ms for a 768-wide
+// block column) near this size.
+#ifndef PF_DENSE_MIN
+#define PF_DENSE_MIN (int64_t)1 << 62
+#endif
 // Trailing matrices at least this large keep the extra rows' update on the side stream.
 static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 
@@ -228,21 +243,68 @@ static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 // panel reads or writes: while the trailing matrix is large (the caller's stream is the
 // critical path) it runs on the side stream after the panel, later (the panels are the
 // critical path) on the caller's stream after the trailing update.
-template <typename T>
-static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
-                           hipStream_t s, void* split_ws, int64_t nb) {
+// The look-ahead's side streams and events, created once and reused: a per-device pool, one
+// context per factorisation in flight on that device (concurrent callers take different ones),
+// returned when the factorisation's launches are enqueued. Streams are non-blocking at the
+// default priority.
+struct LookaheadCtx {
+    int device = -1;
     hipStream_t s2 = nullptr, s3 = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr, ev_e = nullptr;
-    const bool third = r > 0 && split_ws;  // extra rows beside split-FP16 updates
-    int rc = check_hip(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "potrf: side stream");
-    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming), "potrf: event");
-    if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_p, hipEventDisableTiming), "potrf: event");
-    if (rc == QMLH_OK && third) {
-        rc = check_hip(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking), "potrf: third stream");
-        if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_x, hipEventDisableTiming), "potrf: event");
-        if (rc == QMLH_OK) rc = check_hip(hipEventCreateWithFlags(&ev_e, hipEventDisableTiming), "potrf: event");
+};
+
+static std::mutex g_la_mu;
+static std::vector<LookaheadCtx*> g_la_free;
+
+static int lookahead_acquire(LookaheadCtx** out) {
+    int dev = 0;
+    QMLH_TRY(check_hip(hipGetDevice(&dev), "potrf: device"));
+    {
+        std::lock_guard<std::mutex> lock(g_la_mu);
+        for (size_t i = 0; i < g_la_free.size(); ++i) {
+            if (g_la_free[i]->device == dev) {
+                *out = g_la_free[i];
+                g_la_free.erase(g_la_free.begin() + (int64_t)i);
+                return QMLH_OK;
+            }
+        }
     }
-    if (rc == QMLH_OK) rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s);
+    LookaheadCtx* c = new LookaheadCtx;
+    c->device = dev;
+    int rc = check_hip(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking), "potrf: side stream");
+    if (rc == QMLH_OK)
+        rc = check_hip(hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking), "potrf: third stream");
+    hipEvent_t* evs[4] = {&c->ev_a, &c->ev_p, &c->ev_x, &c->ev_e};
+    for (int i = 0; i < 4 && rc == QMLH_OK; ++i)
+        rc = check_hip(hipEventCreateWithFlags(evs[i], hipEventDisableTiming), "potrf: event");
+    if (rc != QMLH_OK) {  // nothing was enqueued on them yet
+        if (c->s2) (void)hipStreamDestroy(c->s2);
+        if (c->s3) (void)hipStreamDestroy(c->s3);
+        for (int i = 0; i < 4; ++i)
+            if (*evs[i]) (void)hipEventDestroy(*evs[i]);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return QMLH_OK;
+}
+
+// Returning a context once its launches are enqueued is safe: the next user's records and
+// waits are ordered after this factorisation's on the same streams, and the caller's stream
+// already waits for this factorisation's last side-stream work.
+static void lookahead_release(LookaheadCtx* c) {
+    std::lock_guard<std::mutex> lock(g_la_mu);
+    g_la_free.push_back(c);
+}
+
+template <typename T>
+static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
+                           hipStream_t s, void* split_ws, int64_t nb, double pf) {
+    LookaheadCtx* ctx = nullptr;
+    QMLH_TRY(lookahead_acquire(&ctx));
+    hipStream_t s2 = ctx->s2, s3 = ctx->s3;
+    hipEvent_t ev_a = ctx->ev_a, ev_p = ctx->ev_p, ev_x = ctx->ev_x, ev_e = ctx->ev_e;
+    int rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s, nullptr, pf);
     for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
         const int64_t m = n - k - kb;
@@ -252,6 +314,28 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         T* C = P + kb * lda;  // A(k + kb, k + kb)
         const bool split = std::is_same<T, float>::value && split_ws && m > kb2;
         hipEvent_t rows_wait = nullptr;  // what the next panel's rows wait for
+        if (split && m >= PF_DENSE_MIN) {
+            // No look-ahead: the extra rows' part of the next block column on the third
+            // stream (as below), the whole trailing triangle with the dense update, then the
+            // next block column's panel on this stream, then the rest of the extra rows.
+            if (r > 0) {
+                rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
+                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
+                if (rc == QMLH_OK)
+                    rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
+                                          info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+                rows_wait = ev_e;
+            }
+            if (rc == QMLH_OK) rc = f16x3_split(reinterpret_cast<float*>(P), lda, m, kb, split_ws, s);
+            if (rc == QMLH_OK)
+                rc = f16x3_update(split_ws, m, kb, 0, m, 0, reinterpret_cast<float*>(C), lda, info,
+                                  s, true);
+            if (rc == QMLH_OK)
+                rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s, rows_wait, pf);
+            if (rc == QMLH_OK) rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
+            continue;
+        }
         if (split) {
             // split-FP16 trailing updates: split the panel once; the next block column gets
             // the same split-FP16 update as the rest, then the rest of the trailing matrix
@@ -279,7 +363,7 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
         if (rc == QMLH_OK)
-            rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2, rows_wait);
+            rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2, rows_wait, pf);
         const bool ext_side = m >= PF_EXT_SIDE_MIN;
         if (rc == QMLH_OK && ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s2);
@@ -296,20 +380,15 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
     }
-    // Destroying them here is safe: pending work completes first, and `s` already waits for it.
-    // (the third stream's last work precedes the last panel's rows, which `s` waits for)
-    if (ev_a) (void)hipEventDestroy(ev_a);
-    if (ev_p) (void)hipEventDestroy(ev_p);
-    if (ev_x) (void)hipEventDestroy(ev_x);
-    if (ev_e) (void)hipEventDestroy(ev_e);
-    if (s2) (void)hipStreamDestroy(s2);
-    if (s3) (void)hipStreamDestroy(s3);
+    // `s` already waits for the last panel (and the third stream's last work precedes the
+    // last panel's rows), so every later use of these streams is ordered after this one.
+    lookahead_release(ctx);
     return rc;
 }
 
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* split_ws,
-               int64_t r) {
+               int64_t r, double pivot_floor) {
     if (n < 0 || r < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0 or r < 0");
     if (lda < (n + r > 1 ? n + r : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n + r");
     if (!info) return set_error(QMLH_ERR_ARG, "potrf: info is null");
@@ -322,10 +401,11 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     const int64_t nb = split16 ? PF_NB_SPLIT : PF_NB;
     if (split16) QMLH_TRY(f16x3_prepare(reinterpret_cast<const float*>(A), n, lda, split_ws, s));
     if (n >= PF_LOOKAHEAD_MIN)
-        return potrf_lookahead<T>(A, n, lda, r, info, s, split16 ? split_ws : nullptr, nb);
+        return potrf_lookahead<T>(A, n, lda, r, info, s, split16 ? split_ws : nullptr, nb,
+                                  pivot_floor);
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
-        QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s));
+        QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s, nullptr, pivot_floor));
         const int64_t m = n - k - kb;
         T* P = A + (k + kb) + k * lda;
         QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, split16 ? split_ws : nullptr));
@@ -574,6 +654,61 @@ int trsm_lower_fwd_multi(const T* L, int64_t n, int64_t ldl, T* V, int64_t ldv, 
     return QMLH_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// V := L^-1 V with an FP32 factor L and an FP64 right-hand side, in FP64 arithmetic: the same
+// 256-row blocked scheme, with each 256-wide block column of L (from its diagonal block down)
+// first widened to FP64 in a panel buffer, so every product is an FP64 MFMA update. The
+// result is L^-1 V for the stored (FP32-rounded) L to FP64 accuracy, which a preconditioner
+// built from that L needs when V's directions reach L's smallest pivots.
+// ------------------------------------------------------------------------------------------
+static constexpr int64_t TRSM64_NB = 256;
+
+// P(i, j) = (double)L(i, j), i < rows, j < cols (column-major, grid-stride).
+__global__ __launch_bounds__(256) void widen_block_kernel(const float* __restrict__ L,
+                                                          int64_t ldl, int64_t rows,
+                                                          int64_t cols, double* __restrict__ P,
+                                                          int64_t ldp) {
+    const int64_t total = rows * cols;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx % rows, j = idx / rows;
+        P[i + j * ldp] = (double)L[i + j * ldl];
+    }
+}
+
+size_t trsm_f32l_panel_doubles(int64_t n) { return (size_t)(n > 0 ? n : 1) * TRSM64_NB; }
+
+int trsm_lower_fwd_f32l(const float* L, int64_t n, int64_t ldl, double* V, int64_t ldv,
+                        int64_t nrhs, const double* dinv, double* panel, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0) return QMLH_OK;
+    const int64_t ldp = n;
+    for (int64_t k = 0; k < n; k += TRSM64_NB) {
+        const int64_t kb = n - k < TRSM64_NB ? n - k : TRSM64_NB;
+        const int64_t rows = n - k;
+        const int64_t total = rows * kb;
+        const unsigned g = (unsigned)(ceil_div(total, (int64_t)256) < 8192 ? ceil_div(total, (int64_t)256) : 8192);
+        hipLaunchKernelGGL(widen_block_kernel, dim3(g), dim3(256), 0, s, L + k + k * ldl, ldl, rows,
+                           kb, panel, ldp);
+        QMLH_CHECK_LAUNCH("trsm f32l widen launch");
+        for (int64_t jj = 0; jj < kb; jj += PF_IB) {
+            const int64_t c = k + jj;
+            const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
+            hipLaunchKernelGGL(dinv_apply_kernel<double>, dim3((unsigned)ceil_div(nrhs, (int64_t)PF_IB)),
+                               dim3(256), 0, s, dinv + (c / PF_IB) * (PF_IB * PF_IB), V + c, ldv, w,
+                               nrhs);
+            QMLH_CHECK_LAUNCH("trsm f32l diagonal launch");
+            const int64_t rest_in = kb - jj - w;
+            QMLH_TRY((launch_update<double, LAYOUT_ROW>(UPD_FULL, panel + (jj + w) + jj * ldp, ldp,
+                                                        V + c, ldv, rest_in, nrhs, w, V + (c + w),
+                                                        ldv, nullptr, s)));
+        }
+        QMLH_TRY((launch_update<double, LAYOUT_ROW>(UPD_FULL, panel + kb, ldp, V + k, ldv,
+                                                    n - k - kb, nrhs, kb, V + (k + kb), ldv,
+                                                    nullptr, s)));
+    }
+    return QMLH_OK;
+}
+
 // out(j, i) = in(i, j) for the R x Cc column-major `in`: 64 x 64 tiles through LDS (padded
 // rows, so the transposed reads hit distinct banks); one-dimensional grid of tiles.
 __global__ __launch_bounds__(256) void transpose_f32_kernel(const float* __restrict__ in,
@@ -679,8 +814,9 @@ __global__ __launch_bounds__(256) void negate_symmetrize_kernel(const double* __
 }
 
 template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*,
-                                int64_t);
-template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*, int64_t);
+                                int64_t, double);
+template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*, int64_t,
+                               double);
 template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, int32_t*,
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
@@ -721,8 +857,8 @@ extern "C" int qmlh_spotrf_f16x3(float* A, int64_t n, int64_t lda, int32_t* info
 }
 
 extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
-                                   int64_t r, int32_t f16x3, int32_t* info, void* ws,
-                                   size_t ws_bytes, qmlh_stream_t stream) {
+                                   int64_t r, int32_t f16x3, double pivot_floor, int32_t* info,
+                                   void* ws, size_t ws_bytes, qmlh_stream_t stream) {
     using namespace qmlh;
     if (n < 0 || r < 0 || lda < n + r || (r > 0 && ldv < n))
         return set_error(QMLH_ERR_ARG, "spotrf_lowrank: bad shape (lda >= n + r, ldv >= n)");
@@ -735,8 +871,9 @@ extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, i
             return set_error(QMLH_ERR_ARG, "spotrf_lowrank: workspace must be 16-byte aligned");
     }
     hipStream_t s = (hipStream_t)stream;
+    if (!(pivot_floor >= 0.0)) return set_error(QMLH_ERR_ARG, "spotrf_lowrank: pivot_floor < 0");
     if (n > 0) QMLH_TRY(transpose_f32(V, ldv, n, r, A + n, lda, s));
-    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, f16x3 ? ws : nullptr, r));
+    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, f16x3 ? ws : nullptr, r, pivot_floor));
     if (n > 0) QMLH_TRY(transpose_f32(A + n, lda, r, n, V, ldv, s));
     return QMLH_OK;
 }
@@ -794,5 +931,81 @@ extern "C" int qmlh_cho_invert(double* A, int64_t n, int64_t lda, int32_t* info,
     hipLaunchKernelGGL(negate_symmetrize_kernel, dim3((unsigned)(tr * (tr + 1) / 2)), dim3(256),
                        0, s, W, ldw, n, A, lda);
     QMLH_CHECK_LAUNCH("cho_invert symmetrize launch");
+    return QMLH_OK;
+}
+
+extern "C" size_t qmlh_trsm_f32l_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    return (qmlh::trsv_dinv_doubles(n) + qmlh::trsm_f32l_panel_doubles(n)) * sizeof(double);
+}
+
+extern "C" int qmlh_trsm_f32l(const float* L, int64_t n, int64_t ldl, double* V, int64_t ldv,
+                              int64_t nrhs, void* ws, size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || nrhs < 0 || ldl < (n > 1 ? n : 1) || ldv < (n > 1 ? n : 1))
+        return set_error(QMLH_ERR_ARG, "trsm_f32l: bad shape");
+    if (n == 0 || nrhs == 0) return QMLH_OK;
+    if (!L || !V || !ws) return set_error(QMLH_ERR_ARG, "trsm_f32l: null pointer");
+    if (ws_bytes < qmlh_trsm_f32l_workspace(n))
+        return set_error(QMLH_ERR_WORKSPACE, "trsm_f32l: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    double* dinv = static_cast<double*>(ws);
+    double* panel = dinv + trsv_dinv_doubles(n);
+    QMLH_TRY(trsv_diag_inverse<float>(L, n, ldl, dinv, s));
+    TimerScope ts(QMLH_TIMER_GEMM, (double)n * (double)n * (double)nrhs, s);
+    return trsm_lower_fwd_f32l(L, n, ldl, V, ldv, nrhs, dinv, panel, s);
+}
+
+// Y (n x r, FP64) := the hybrid Woodbury term: columns k .. r-1 widened from the factorisation's
+// FP32 solve Y32 = L^-1 V, columns 0 .. k-1 solved again in FP64 arithmetic from the FP64
+// right-hand sides Y holds there on entry (the leading columns of V, where u = V(:, 0) is the
+// direction of the kernel's largest eigenvalues).
+namespace qmlh {
+__global__ __launch_bounds__(256) void widen_columns_kernel(const float* __restrict__ Y32,
+                                                            int64_t ldy32, double* __restrict__ Y,
+                                                            int64_t ldy, int64_t n, int64_t k0,
+                                                            int64_t r) {
+    const int64_t total = n * (r - k0);
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx % n, j = k0 + idx / n;
+        Y[i + j * ldy] = (double)Y32[i + j * ldy32];
+    }
+}
+}  // namespace qmlh
+
+extern "C" size_t qmlh_lowrank_hybrid_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    return (size_t)n * sizeof(double) + qmlh::trsv_sync_ints(n) * sizeof(int32_t);
+}
+
+extern "C" int qmlh_lowrank_hybrid(const float* L, int64_t n, int64_t ldl, const float* Y32,
+                                   int64_t ldy32, double* Y, int64_t ldy, int64_t r, int64_t k,
+                                   void* ws, size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (n < 0 || r < 0 || k < 0 || k > r || ldl < (n > 1 ? n : 1) || ldy < (n > 1 ? n : 1) ||
+        (r > k && ldy32 < (n > 1 ? n : 1)))
+        return set_error(QMLH_ERR_ARG, "lowrank_hybrid: bad shape");
+    if (n == 0 || r == 0) return QMLH_OK;
+    if (!L || !Y || (r > k && !Y32) || (k > 0 && !ws))
+        return set_error(QMLH_ERR_ARG, "lowrank_hybrid: null pointer");
+    if (k > 0 && ws_bytes < qmlh_lowrank_hybrid_workspace(n))
+        return set_error(QMLH_ERR_WORKSPACE, "lowrank_hybrid: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    if (r > k) {
+        const int64_t total = n * (r - k);
+        const unsigned g = (unsigned)(ceil_div(total, (int64_t)256) < 8192 ? ceil_div(total, (int64_t)256) : 8192);
+        hipLaunchKernelGGL(widen_columns_kernel, dim3(g), dim3(256), 0, s, Y32, ldy32, Y, ldy, n, k,
+                           r);
+        QMLH_CHECK_LAUNCH("lowrank_hybrid widen launch");
+    }
+    double* tmp = static_cast<double*>(ws);
+    int32_t* sync = reinterpret_cast<int32_t*>(tmp + n);
+    for (int64_t j = 0; j < k; ++j) {
+        QMLH_TRY(check_hip(hipMemcpyAsync(tmp, Y + j * ldy, (size_t)n * sizeof(double),
+                                          hipMemcpyDeviceToDevice, s),
+                           "lowrank_hybrid: copy"));
+        QMLH_TRY(trsv_lower_fwd<float>(L, n, ldl, tmp, Y + j * ldy, sync, s));
+    }
     return QMLH_OK;
 }

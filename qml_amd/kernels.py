@@ -70,6 +70,20 @@ def expansion_ok(maxnorm_a: float, maxnorm_b: float, d: int, sigma: float) -> bo
     return bool(bound < EXPANSION_TOL)
 
 
+def device_max(t) -> float:
+    """Largest entry of a device vector, read on the host (0 for an empty one)."""
+    return float(t.max()) if t is not None and t.numel() else 0.0
+
+
+def check_expansion(maxnorm_a: float, maxnorm_b: float, d: int, sigma: float, what: str) -> None:
+    """Raise when the expanded Gaussian (the MFMA cross-term form) would leave its error bound
+    for rows of these squared norms; paths with no direct-form fallback call this."""
+    if not expansion_ok(maxnorm_a, maxnorm_b, d, sigma):
+        raise ValueError(f"{what}: the expanded Gaussian form exceeds its error bound "
+                         f"({EXPANSION_TOL:g} relative) for representations of squared norm up "
+                         f"to {max(maxnorm_a, maxnorm_b):.3g} at sigma = {sigma:g}")
+
+
 def _to_host_fortran(Kd):
     # (nb, na) row-major on device == (na, nb) column-major: transpose view of the host copy.
     return Kd.cpu().numpy().T

@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib
 from .device import device, padded_ld, ragged_to_device, rows_to_device
-from .kernels import gaussian_mfma_device, pairwise_device, row_sqnorms_device
+from .kernels import (check_expansion, device_max, expansion_ok, gaussian_mfma_device,
+                      pairwise_device, row_sqnorms_device)
 from .math import add_diagonal_device, potrf_device, potrs_device, print_potrf_warning
 from .representations import cm_length, generate_coulomb_matrices_device
 
@@ -25,6 +26,16 @@ KERNELS = {"gaussian": _lib.PAIR_GAUSSIAN, "laplacian": _lib.PAIR_LAPLACIAN}
 
 class CholeskyError(RuntimeError):
     """The regularised kernel matrix is not numerically positive definite (LAPACK info > 0)."""
+
+
+def check_labels(y, n: int) -> None:
+    """Labels must be one value per training row: the solves read and write n doubles of y.
+
+    Raises the ValueError text of qml/math.py:79-80 (cho_solve's vector-size check) before
+    anything touches the device."""
+    shape = tuple(getattr(y, "shape", np.shape(y)))
+    if shape != (n,):
+        raise ValueError('expected matrix and vector of same stride size')
 
 
 @dataclass
@@ -36,6 +47,9 @@ class FitReport:
     iterations: int = 0
     relres: float = 0.0
     shift: float = 0.0
+    # normwise backward error |y - A alpha| / (|A|_inf |alpha| + |y|) of the mixed fit, when
+    # its recomputed residual is above the tolerance (-1: the residual test passed)
+    backward_error: float = -1.0
 
 
 class KRR:
@@ -52,12 +66,23 @@ class KRR:
                  deflated matrix B = K - V V^T (V = u [1, x / sigma], u_i =
                  exp(-|x_i|^2 / 2 sigma^2); the constant and linear Taylor terms of the
                  kernel), with the rank-(d+1) term restored by a Woodbury correction inside the
-                 preconditioner; Y = L^-1 V is solved during the factorisation itself.
+                 preconditioner. lowrank sets how Y = L^-1 V is formed: "fp32" -- V rounded to
+                 FP32 and solved during the factorisation itself as extra panel rows (free);
+                 "fp64" -- V in FP64, solved after the factorisation with the FP32 factor in
+                 FP64 arithmetic (n^2 (d+1) FP64 MFMA flops); "hybrid" -- the FP32 solve, with
+                 its first lowrank_cols columns (u, the direction of the largest eigenvalues,
+                 first) solved again in FP64 (one single-vector solve each); "auto" (default)
+                 -- "hybrid" with one FP64 column. The FP32 Y's error is amplified through L^-1
+                 by up to 1 / sqrt(lambda) and costs CG iterations; on the sigma = 4000
+                 Coulomb-matrix workloads the FP64 u column alone recovers what the whole FP64
+                 term does (50,000 molecules: 8 -> 4 iterations at lambda 1e-6, 70 -> 29 at
+                 1e-9) for one triangular solve instead of d + 1.
     """
 
     def __init__(self, kernel: str = "gaussian", sigma: float = 4000.0, lam: float = 1e-9,
                  precision: str = "fp64", tol: float = 1e-10, max_iter: int = 200,
-                 factor: str | None = None):
+                 factor: str | None = None, lowrank: str = "auto", lowrank_cols: int = 1,
+                 pivot_floor: float = 0.0):
         if kernel not in KERNELS:
             raise ValueError(f"kernel must be one of {sorted(KERNELS)}")
         if precision not in ("fp64", "mixed"):
@@ -75,6 +100,15 @@ class KRR:
         if factor not in ("fp32", "f16x3"):
             raise ValueError("factor must be 'fp32' or 'f16x3'")
         self.factor = factor
+        if lowrank not in ("auto", "fp32", "fp64", "hybrid"):
+            raise ValueError("lowrank must be 'auto', 'fp32', 'fp64' or 'hybrid'")
+        self.lowrank = lowrank
+        self.lowrank_cols = int(lowrank_cols)
+        # Static pivoting of the low-precision factor: pivots at or below
+        # pivot_floor * (lambda + shift) are raised to it (0: a breakdown retries with a shift).
+        if not pivot_floor >= 0.0:
+            raise ValueError("pivot_floor must be >= 0")
+        self.pivot_floor = float(pivot_floor)
         self.Xd = None
         self.norms = None
         self.alpha = None
@@ -87,13 +121,19 @@ class KRR:
         if self.kernel == "gaussian":
             if norms is None:
                 norms = row_sqnorms_device(Xd, n, d, stream)
-            return gaussian_mfma_device(Xd, norms, n, Xd, norms, n, d, self.sigma, True, out,
-                                        stream)
+            mx = device_max(norms)
+            if expansion_ok(mx, mx, d, self.sigma):
+                return gaussian_mfma_device(Xd, norms, n, Xd, norms, n, d, self.sigma, True, out,
+                                            stream)
+            # outside the expansion's error bound: the direct-difference form
+            return pairwise_device(_lib.PAIR_GAUSSIAN, Xd, n, Xd, n, d, self.sigma, True, out,
+                                   stream)
         return pairwise_device(_lib.PAIR_LAPLACIAN, Xd, n, Xd, n, d, self.sigma, True, out, stream)
 
     # -- fit ----------------------------------------------------------------------------------
     def fit_device(self, Xd, d: int, y, raise_on_error: bool = True, stream=None) -> FitReport:
         """Fit from an (n, ld) device representation tensor; y host or device (n,)."""
+        check_labels(y, Xd.shape[0])
         if self.precision == "mixed":
             return self._fit_mixed(Xd, d, y, raise_on_error, stream)
         import torch
@@ -132,75 +172,136 @@ class KRR:
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         norms = u = V = C = None
         r = d + 1 if self.kernel == "gaussian" else 0
-        # B32 column-major with ld = n + r: the r rows below the matrix carry V^T through the
-        # factorisation, which solves them into (L^-1 V)^T along the way.
-        ldb = n + r
+        mode = self.lowrank_mode() if r else "fp32"
+        y64 = mode != "fp32"
+        # B32 column-major with ld = n + rx: with an FP32 (or hybrid) Woodbury term the r rows
+        # below the matrix carry V^T through the factorisation, which solves them into
+        # (L^-1 V)^T along the way; with an FP64 one, Y is solved after the factorisation.
+        rx = r if mode in ("fp32", "hybrid") else 0
+        ldb = n + rx
         B = torch.empty((n, ldb), dtype=torch.float32, device=dev)
+        V64 = None
         if self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
+            mx = device_max(norms)  # the deflation and the residuals use the expanded form
+            check_expansion(mx, mx, d, self.sigma, "mixed Gaussian fit")
             u = torch.empty(n, dtype=torch.float64, device=dev)
-            V = torch.empty((r, n), dtype=torch.float32, device=dev)  # (n, r) column-major
+            # (n, r) column-major: V (FP32) rides through the factorisation; V64 the FP64 term
+            V = torch.empty((r, n), dtype=torch.float32, device=dev) if rx else None
+            V64 = torch.empty((r, n), dtype=torch.float64, device=dev) if y64 else None
             C = torch.empty((r, r), dtype=torch.float64, device=dev)
         else:
             self.build_kernel(Xd, n, d, out=K, stream=stream)
             add_diagonal_device(K, n, self.lam, stream)
-        # Low-precision factor; on breakdown retry with a diagonal shift (the CG on the FP64
-        # matrix still converges to the unshifted solution, only more slowly).
+        # Low-precision factor and Woodbury term; when the factor breaks down (or the
+        # capacitance matrix of the Woodbury term is not numerically positive definite), retry
+        # with a diagonal shift (the CG on the FP64 matrix still converges to the unshifted
+        # solution, only more slowly).
         shifts = [0.0] + [max(self.lam, 1e-12) * 10.0 ** k for k in range(1, 9)]
         bws = None
+        Yw = None  # the Woodbury term handed to the CG: V (FP32) or V64, after the solve
         for attempt, shift in enumerate(shifts):
             if self.kernel == "gaussian":
                 _lib.check(L.qmlh_gaussian_deflated(
                     _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(norms), d, self.sigma, self.lam, shift,
-                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), ldb, _lib.ptr(V), n,
-                    _lib.ptr(u), h))
+                    _lib.ptr(K) if attempt == 0 else None, n, _lib.ptr(B), ldb,
+                    _lib.ptr(V) if rx else None, n, _lib.ptr(u), h))
             else:
                 _lib.check(L.qmlh_lower_to_f32(_lib.ptr(K), n, n, _lib.ptr(B), ldb, shift, h))
             split16 = self.factor == "f16x3"
             if split16 and bws is None:
                 nb = L.qmlh_spotrf_f16x3_workspace(n)
                 bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
-            _lib.check(L.qmlh_spotrf_lowrank(_lib.ptr(B), n, ldb, _lib.ptr(V) if r else None, n,
-                                             r, int(split16), _lib.ptr(info),
+            _lib.check(L.qmlh_spotrf_lowrank(_lib.ptr(B), n, ldb, _lib.ptr(V) if rx else None, n,
+                                             rx, int(split16),
+                                             self.pivot_floor * (self.lam + shift),
+                                             _lib.ptr(info),
                                              _lib.ptr(bws) if split16 else None,
                                              bws.numel() * 8 if split16 else 0, h))
             rep.info = _lib.host_int(info, stream)
             rep.shift = shift
-            if rep.info == 0:
+            if rep.info != 0:
+                continue
+            if not r:
                 break
-        if rep.info != 0:
-            if raise_on_error:
-                raise CholeskyError(f"FP32 factorisation failed at every shift: info={rep.info}")
-            return rep
-        if r:  # V now holds Y = L^-1 V
+            Yw = V
+            if y64:  # V64 = u [1, x / sigma] in FP64, then Y = L^-1 V64 in FP64 arithmetic
+                _lib.check(L.qmlh_gaussian_lowrank(_lib.ptr(Xd), n, Xd.shape[1], d, self.sigma,
+                                                   _lib.ptr(u), _lib.ptr(V64), n, h))
+                if mode == "fp64":
+                    tws = torch.empty(max(1, L.qmlh_trsm_f32l_workspace(n) // 8),
+                                      dtype=torch.float64, device=dev)
+                    _lib.check(L.qmlh_trsm_f32l(_lib.ptr(B), n, ldb, _lib.ptr(V64), n, r,
+                                                _lib.ptr(tws), tws.numel() * 8, h))
+                else:  # hybrid: the factorisation's FP32 Y, leading columns again in FP64
+                    k = max(0, min(self.lowrank_cols, r))
+                    tws = torch.empty(max(1, L.qmlh_lowrank_hybrid_workspace(n) // 8 + 1),
+                                      dtype=torch.float64, device=dev)
+                    _lib.check(L.qmlh_lowrank_hybrid(_lib.ptr(B), n, ldb, _lib.ptr(V), n,
+                                                     _lib.ptr(V64), n, r, k, _lib.ptr(tws),
+                                                     tws.numel() * 8, h))
+                del tws
+                Yw = V64
+            # C = chol(I + Y^T Y)
             gws = torch.empty(max(1, L.qmlh_woodbury_capacitance_workspace(n, r) // 8),
                               dtype=torch.float64, device=dev)
-            _lib.check(L.qmlh_woodbury_capacitance(_lib.ptr(V), n, n, r, _lib.ptr(C), r,
-                                                   _lib.ptr(info), _lib.ptr(gws),
-                                                   gws.numel() * 8, h))
+            cap = L.qmlh_woodbury_capacitance_f64 if y64 else L.qmlh_woodbury_capacitance
+            _lib.check(cap(_lib.ptr(Yw), n, n, r, _lib.ptr(C), r, _lib.ptr(info), _lib.ptr(gws),
+                           gws.numel() * 8, h))
             del gws
-            if _lib.host_int(info, stream) != 0:
-                raise CholeskyError("Woodbury capacitance matrix is not positive definite")
+            if _lib.host_int(info, stream) == 0:
+                break
+            rep.info = -1  # capacitance matrix not positive definite: next shift
+        if rep.info != 0:
+            if raise_on_error:
+                raise CholeskyError("FP32 factorisation (or its Woodbury capacitance) failed at "
+                                    f"every shift: info={rep.info}")
+            return rep
+        V = Yw
         alpha = torch.zeros(n, dtype=torch.float64, device=dev)
         yd = torch.as_tensor(y, dtype=torch.float64).to(dev).contiguous()
         ws_bytes = L.qmlh_pcg_workspace(n, r)
         ws = torch.empty(max(1, ws_bytes // 8 + 1), dtype=torch.float64, device=dev)
         iters = ctypes.c_int32()
         relres = ctypes.c_double()
+        backward = ctypes.c_double()
         _lib.check(L.qmlh_pcg_solve(_lib.ptr(K), n, n, _lib.ptr(B), ldb, _lib.ptr(V), n, r,
-                                    _lib.ptr(C), r, _lib.ptr(yd), _lib.ptr(alpha), self.tol,
+                                    int(y64), _lib.ptr(C), r, _lib.ptr(yd), _lib.ptr(alpha), self.tol,
                                     self.max_iter, _lib.ptr(ws), ws.numel() * 8,
-                                    ctypes.byref(iters), ctypes.byref(relres), h))
+                                    ctypes.byref(iters), ctypes.byref(relres),
+                                    ctypes.byref(backward), h))
         rep.iterations, rep.relres = iters.value, relres.value
+        rep.backward_error = backward.value
         rep.seconds["fit"] = time.perf_counter() - t0
         del K, B, ws, V, C, bws
         if norms is None and self.kernel == "gaussian":
             norms = row_sqnorms_device(Xd, n, d, stream)
         self.Xd, self.norms, self.alpha, self.n, self.d = Xd, norms, alpha, n, d
-        if not (rep.relres <= self.tol) and raise_on_error:
+        if not self.converged(rep) and raise_on_error:
             raise CholeskyError(f"refinement stopped at relative residual {rep.relres:.3e} "
-                                f"after {rep.iterations} iterations")
+                                f"(backward error {rep.backward_error:.3e}) after "
+                                f"{rep.iterations} iterations")
         return rep
+
+    def lowrank_mode(self) -> str:
+        """How the Woodbury term is formed: "fp32", "fp64" or "hybrid" (class docstring)."""
+        return "hybrid" if self.lowrank == "auto" else self.lowrank
+
+    def lowrank_fp64(self) -> bool:
+        """Whether the Woodbury term Y is held in FP64 (wholly or in part)."""
+        return self.lowrank_mode() != "fp32"
+
+    # Normwise backward error accepted as the FP64 floor: the unit roundoff u = 2^-53. Below it,
+    # alpha solves a system whose matrix lies within the rounding of K's own FP64 entries; a
+    # backward-stable FP64 Cholesky solve reaches about u / 10 on these kernels.
+    BACKWARD_FLOOR = 0.5 * float(np.finfo(np.float64).eps)
+
+    def converged(self, rep: FitReport) -> bool:
+        """The recomputed residual meets the tolerance, or the solution is at the FP64 floor
+        (its backward error is that of an FP64 solve)."""
+        if rep.relres <= self.tol:
+            return True
+        return 0.0 <= rep.backward_error <= self.BACKWARD_FLOOR
 
     def fit(self, X, y, **kw) -> FitReport:
         """Fit from host (n, d) representations."""
@@ -229,8 +330,12 @@ class KRR:
         ws_bytes = L.qmlh_predict_workspace(m, self.n)
         if workspace is None or workspace.numel() * workspace.element_size() < ws_bytes:
             workspace = torch.empty(max(1, ws_bytes // 8), dtype=torch.float64, device=Qd.device)
-        if self.kernel == "gaussian" and qnorms is None:
-            qnorms = row_sqnorms_device(Qd, m, self.d, stream)
+        if self.kernel == "gaussian":
+            if qnorms is None:
+                qnorms = row_sqnorms_device(Qd, m, self.d, stream)
+            # the fused predict kernel has only the expanded form
+            check_expansion(device_max(qnorms), self.max_norm(), self.d, self.sigma,
+                            "Gaussian prediction")
         _lib.check(L.qmlh_krr_predict(
             KERNELS[self.kernel], _lib.ptr(Qd), m, Qd.shape[1], _lib.ptr(qnorms),
             _lib.ptr(self.Xd), self.n, self.Xd.shape[1], _lib.ptr(self.norms),
@@ -247,6 +352,12 @@ class KRR:
         Zd, Rd, Od = ragged_to_device(Z, R, offsets)
         Qd, _ = generate_coulomb_matrices_device(Zd, Rd, Od, size, sorting, ld=self.Xd.shape[1])
         return self.predict_device(Qd, Qd.shape[0]).cpu().numpy()
+
+    def max_norm(self) -> float:
+        """Largest squared norm of the training representations (Gaussian models)."""
+        if getattr(self, "_max_norm", None) is None or self._max_norm[0] is not self.norms:
+            self._max_norm = (self.norms, device_max(self.norms))
+        return self._max_norm[1]
 
     # -- checkpoint ---------------------------------------------------------------------------
     def save(self, path: str) -> None:

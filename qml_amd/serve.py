@@ -33,7 +33,7 @@ import numpy as np
 
 from . import _lib
 from .device import padded_ld
-from .krr import KERNELS, KRR
+from .krr import KERNELS, KRR, check_labels
 from .representations import cm_length
 
 # --------------------------------------------------------------------------------------------
@@ -77,6 +77,7 @@ class RefitRequest:
         self.kernel, self.sigma, self.lam, self.precision, self.size = (
             kernel, sigma, lam, precision, size)
         self.n = int(len(offsets) - 1)
+        check_labels(self.y, self.n)  # before the request can reach the device
         self.t_arrival = time.perf_counter()
         self.t_start: Optional[float] = None
         self.t_done: Optional[float] = None
@@ -99,13 +100,27 @@ def refit_bytes(n: int, d: int, precision: str) -> int:
     """Device memory a fit of n molecules needs (kernel matrix dominates)."""
     ld = padded_ld(d)
     per_elem = 8 if precision == "fp64" else 12  # K64, or K64 + B32
-    extra = n * (ld * 8 + 64) + (n * (d + 1) * 4 if precision == "mixed" else 0)
+    # mixed: B32's extra rows or the FP64 Woodbury term, and the FP64 solve's panel buffer
+    extra = n * (ld * 8 + 64) + (n * (d + 1) * 12 + n * 256 * 8 if precision == "mixed" else 0)
     return per_elem * n * n + extra
 
 
 # --------------------------------------------------------------------------------------------
 # Policies (pure host logic, testable without a GPU)
 # --------------------------------------------------------------------------------------------
+
+
+def should_launch(inflight: bool, queued_requests: int, queued_molecules: int,
+                  oldest_wait_s: float, min_batch: int, window_s: float) -> bool:
+    """Whether a free slot takes a batch now. With nothing on the GPU: at once (no added
+    latency at low load). With batches in flight: once min_batch molecules are queued or the
+    oldest queued request has waited window_s, so that under load requests coalesce into
+    fewer, larger batches (one read of the training set and one launch sequence per batch)."""
+    if queued_requests <= 0:
+        return False
+    if not inflight:
+        return True
+    return queued_molecules >= min_batch or oldest_wait_s >= window_s
 
 
 def plan_batch(queue: "collections.deque", max_molecules: int, max_atoms: int):
@@ -183,7 +198,8 @@ class KRRServer:
     """Serve predictions of a fitted ``KRR`` (training set and alpha resident in HBM)."""
 
     def __init__(self, model: KRR, size: int, max_molecules: int = 2048, max_inflight: int = 3,
-                 sorting: str = "row-norm", admission: Optional[MemoryAdmission] = None):
+                 sorting: str = "row-norm", admission: Optional[MemoryAdmission] = None,
+                 min_batch: int = 128, window_s: float = 0.0002):
         import torch
 
         if model.alpha is None:
@@ -194,10 +210,21 @@ class KRRServer:
         self.d = cm_length(size)
         if self.d != model.d:
             raise ValueError("Coulomb matrix size does not match the model's features")
+        if model.kernel == "gaussian":
+            # The fused predict kernel uses the expanded Gaussian; its error bound is checked
+            # for queries as large as the training set's largest representation (requests of
+            # larger molecules than any training molecule are outside what is validated here).
+            from .kernels import check_expansion
+
+            mx = model.max_norm()
+            check_expansion(mx, mx, model.d, model.sigma, "Gaussian serving")
         self.device = model.Xd.device
         self.max_molecules = max_molecules
         self.max_atoms = max_molecules * size
         self._queue = collections.deque()
+        self._queued_mols = 0
+        self.min_batch = int(min_batch)
+        self.window_s = float(window_s)
         self._refits = collections.deque()
         self._cv = threading.Condition()
         self._running = False
@@ -236,7 +263,11 @@ class KRRServer:
             if req.nmol and int(np.diff(req.offsets).max()) > self.size:
                 raise ValueError(f"molecule with more than {self.size} atoms")
         with self._cv:
-            (self._queue if isinstance(req, PredictRequest) else self._refits).append(req)
+            if isinstance(req, PredictRequest):
+                self._queue.append(req)
+                self._queued_mols += req.nmol
+            else:
+                self._refits.append(req)
             self._cv.notify_all()
         return req
 
@@ -320,7 +351,13 @@ class KRRServer:
                 if slot.busy:
                     continue
                 with self._cv:
+                    inflight = any(s.busy for s in self._slots)
+                    wait = (time.perf_counter() - self._queue[0].t_arrival) if self._queue else 0.0
+                    if not should_launch(inflight, len(self._queue), self._queued_mols, wait,
+                                         self.min_batch, self.window_s):
+                        break
                     reqs = plan_batch(self._queue, self.max_molecules, self.max_atoms)
+                    self._queued_mols -= sum(r.nmol for r in reqs)
                 if not reqs:
                     break
                 try:
@@ -396,3 +433,53 @@ def poisson_load(server: KRRServer, pool, rate: float, duration: float, rng, max
             time.sleep(delay)
         reqs.append(server.submit(PredictRequest(Z, R, off)))
     return reqs
+
+
+def run_load(server: KRRServer, pool, rate: float, duration: float, rng, max_mol: int = 512,
+             size: int = 29, refit_pool=None, refit_every: float = 0.0,
+             refit_n=(1000, 16000), refit_precision: str = "fp64"):
+    """Poisson prediction load (``poisson_load``) plus, when ``refit_every`` > 0, refit
+    requests of N ~ U[refit_n] molecules from ``refit_pool`` submitted every ``refit_every``
+    seconds while the load runs (admitted by the server's memory budget). Waits for every
+    request; returns (predict requests, refit requests, summary dict)."""
+    refits = []
+    stop = threading.Event()
+
+    def submit_refits():
+        rr = np.random.default_rng(int(rng.integers(1 << 31)))
+        while not stop.wait(refit_every):
+            n = int(rr.integers(refit_n[0], refit_n[1] + 1))
+            sub = refit_pool.subset(0, min(n, refit_pool.nmol))
+            refits.append(server.submit(RefitRequest(
+                *sub.ragged(), sub.labels, kernel=server.model.kernel, sigma=server.model.sigma,
+                lam=server.model.lam, precision=refit_precision, size=size)))
+
+    th = None
+    if refit_every > 0 and refit_pool is not None:
+        th = threading.Thread(target=submit_refits, daemon=True)
+        th.start()
+    b0 = server.batches
+    reqs = poisson_load(server, pool, rate, duration, rng, max_mol=max_mol, size=size)
+    stop.set()
+    if th is not None:
+        th.join()
+    for r in reqs:
+        r.wait(600)
+    fitted = [f for f in refits if f.wait(600) is not None]
+    lat = np.array([r.latency for r in reqs]) * 1e3 if reqs else np.full(1, np.nan)
+    span = (max(r.t_done for r in reqs) - min(r.t_arrival for r in reqs)) if reqs else float("nan")
+    nb = server.batches - b0
+    summary = {
+        "rate_req_per_s": rate, "requests": len(reqs), "batches": nb,
+        "molecules_per_batch": float(sum(r.nmol for r in reqs) / max(nb, 1)),
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "molecules_per_s": float(sum(r.nmol for r in reqs) / span) if reqs else 0.0,
+        "refits": [{"n": f.n, "admitted_after_ms": (f.t_start - f.t_arrival) * 1e3,
+                    "fit_ms": (f.t_done - f.t_start) * 1e3} for f in refits],
+        "refit_precision": refit_precision if refits else None,
+        "predictions_during_refits": int(sum(
+            1 for r in reqs if any(f.t_start is not None and f.t_start < r.t_done < f.t_done
+                                   for f in refits))),
+    }
+    del fitted
+    return reqs, refits, summary

@@ -17,7 +17,8 @@ import numpy as np
 
 from . import _lib
 from .autotune import Autotuner, make_key, pow2_bucket
-from .kernels import gaussian_mfma_device, pairwise_device, row_sqnorms_device
+from .kernels import (check_expansion, device_max, gaussian_mfma_device, pairwise_device,
+                      row_sqnorms_device)
 from .math import add_diagonal_device, potrf_device, potrs_device
 
 KINDS = {"gaussian": _lib.PAIR_GAUSSIAN, "laplacian": _lib.PAIR_LAPLACIAN}
@@ -89,6 +90,11 @@ class BatchedFit:
         self._structs, self.offsets, total = _structs(self.problems)
         self.norms = (row_sqnorms_device(Xd, Xd.shape[0], d)
                       if any(p.kernel == "gaussian" for p in self.problems) else None)
+        if self.norms is not None:  # the batched Gaussian build has only the expanded form
+            mx = device_max(self.norms)
+            for p in self.problems:
+                if p.kernel == "gaussian":
+                    check_expansion(mx, mx, d, p.sigma, "batched Gaussian fit")
         self.alpha = torch.zeros(max(total, 1), dtype=torch.float64, device=dev)
         self.info_d = torch.zeros(len(self.problems), dtype=torch.int32, device=dev)
         self._plan = None
