@@ -327,6 +327,19 @@ int qmlh_krr_predict(int32_t kind, const double *Xq, int64_t m, int64_t ldq,
                      const double *normsQ, const double *Xt, int64_t n, int64_t ldt,
                      const double *normsT, const double *alpha, int64_t d, double sigma,
                      double *y, void *ws, size_t ws_bytes, qmlh_stream_t stream);
+/* qmlh_krr_predict without a workspace argument, allocating nothing: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The per-chunk partials live in a
+ * library-owned device array of qmlh_krr_predict_noalloc_capacity() bytes (16 MiB) that each
+ * GPU holds until the process ends, resolved for a device by qmlh_init (or by the first call
+ * on it). Calls needing more -- ceil(n / 128) * m doubles -- return QMLH_ERR_ARG. Results are
+ * bitwise those of qmlh_krr_predict. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams at once) race. */
+size_t qmlh_krr_predict_noalloc_capacity(void);
+int qmlh_krr_predict_noalloc(int32_t kind, const double *Xq, int64_t m, int64_t ldq,
+                             const double *normsQ, const double *Xt, int64_t n, int64_t ldt,
+                             const double *normsT, const double *alpha, int64_t d, double sigma,
+                             double *y, qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Batched KRR fits: many independent problems of uneven size -- a learning curve over

@@ -106,6 +106,9 @@ _SIGNATURES = {
     "qmlh_predict_workspace": (_sz, [_i64, _i64]),
     "qmlh_krr_predict": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp,
                                         _i64, _f64, _vp, _vp, _sz, _vp]),
+    "qmlh_krr_predict_noalloc_capacity": (_sz, []),
+    "qmlh_krr_predict_noalloc": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp,
+                                                _vp, _i64, _f64, _vp, _vp]),
     "qmlh_krr_batched_workspace": (_sz, [_vp, _i32, _i32]),
     "qmlh_krr_batched_create": (ctypes.c_int, [_vp, _i32, _i32, _vp, _i64, _i64, _i64, _vp, _vp,
                                                _i64, _vp, _i64, _vp, _vp, _sz,

@@ -19,6 +19,8 @@ int set_error(int code, const char* fmt, ...) {
 
 // Defined in cm.hip: uploads the diagonal table to the current device.
 int upload_cm_diag_table(const double* table_h, int n);
+// Defined in predict.hip: resolves the current device's copy of the prediction scratch.
+int predict_scratch_setup(double** out);
 
 }  // namespace qmlh
 
@@ -33,5 +35,6 @@ extern "C" int qmlh_init(void) {
     std::call_once(once, [] {
         for (int z = 0; z < 256; ++z) table[z] = 0.5 * pow((double)z, 2.4);
     });
-    return qmlh::upload_cm_diag_table(table, 256);
+    QMLH_TRY(qmlh::upload_cm_diag_table(table, 256));
+    return qmlh::predict_scratch_setup(nullptr);
 }

@@ -6,6 +6,8 @@
 // that depends only on the training index, writes one partial per (chunk, query), and a second
 // kernel adds the partials of a query in chunk order. Nothing in a query's arithmetic depends
 // on the other queries or on m, so a query's result is bitwise the same alone or in a batch.
+#include <mutex>
+
 #include "mfma_core.h"
 #include "timer.h"
 
@@ -155,13 +157,103 @@ __global__ void predict_reduce_kernel(const double* __restrict__ part, int64_t m
     y[q] = s;
 }
 
+__global__ void zero_kernel(double* __restrict__ y, int64_t m) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) y[q] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Scratch of qmlh_krr_predict_noalloc: the per-(chunk, query) partials in a file-scope device
+// array instead of a caller's workspace. Each GPU holds its own copy for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name on the host is a stand-in, not a device address).
+// ------------------------------------------------------------------------------------------
+static constexpr int64_t PREDICT_SCRATCH_DOUBLES = (int64_t)2 << 20;  // 16 MiB
+static __device__ __align__(256) double scratch[PREDICT_SCRATCH_DOUBLES];
+
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Resolve (once per device) the current device's copy of the scratch array.
+int predict_scratch_setup(double** out) {
+    int dev = 0;
+    QMLH_TRY(check_hip(hipGetDevice(&dev), "predict scratch: hipGetDevice"));
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return set_error(QMLH_ERR_ARG, "predict scratch: device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        QMLH_TRY(check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                           "predict scratch: hipGetSymbolAddress"));
+        if (!p) return set_error(QMLH_ERR_HIP, "predict scratch: null symbol address");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    if (out) *out = g_scratch_ptr[dev];
+    return QMLH_OK;
+}
+
+size_t predict_workspace_bytes(int64_t m, int64_t n) {
+    if (m <= 0 || n <= 0) return 0;
+    const int64_t chunks = ceil_div(n, PL_T);  // the finer of the two chunkings
+    return (size_t)(chunks * m) * sizeof(double);
+}
+
+// The prediction with its partials in `part` (at least predict_workspace_bytes(m, n)).
+static int predict_impl(int32_t kind, const double* Xq, int64_t m, int64_t ldq,
+                        const double* normsQ, const double* Xt, int64_t n, int64_t ldt,
+                        const double* normsT, const double* alpha, int64_t d, double sigma,
+                        double* y, double* part, hipStream_t s) {
+    if (n == 0) {  // empty training set: y = 0 (a kernel, not a memset, so capture is safe)
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, y, m);
+        QMLH_CHECK_LAUNCH("krr_predict zero launch");
+        return QMLH_OK;
+    }
+    int64_t nchunks;
+    TimerScope ts(QMLH_TIMER_PREDICT, (double)m * (double)n, s);
+    if (kind == QMLH_PAIR_GAUSSIAN) {
+        nchunks = ceil_div(n, PG_T);
+        dim3 grid((unsigned)ceil_div(m, PG_Q), (unsigned)nchunks);
+        hipLaunchKernelGGL(predict_gauss_kernel, grid, dim3(256), 0, s, Xq, ldq, m, normsQ, Xt,
+                           ldt, n, normsT, alpha, d, -0.5 / (sigma * sigma), part);
+    } else {
+        nchunks = ceil_div(n, PL_T);
+        dim3 grid((unsigned)ceil_div(m, PL_Q), (unsigned)nchunks);
+        hipLaunchKernelGGL(predict_laplace_kernel, grid, dim3(256), 0, s, Xq, ldq, m, Xt, ldt, n,
+                           alpha, d, -1.0 / sigma, part);
+    }
+    QMLH_CHECK_LAUNCH("krr_predict launch");
+    hipLaunchKernelGGL(predict_reduce_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s,
+                       part, m, nchunks, y);
+    QMLH_CHECK_LAUNCH("krr_predict reduce launch");
+    return QMLH_OK;
+}
+
+// Argument checks shared by both entry points (everything but the workspace).
+static int predict_check(int32_t kind, const double* Xq, int64_t m, int64_t ldq,
+                         const double* normsQ, const double* Xt, int64_t n, int64_t ldt,
+                         const double* normsT, const double* alpha, int64_t d, double* y,
+                         const char* who) {
+    if (m < 0 || n < 0 || d < 0) return set_error(QMLH_ERR_ARG, "%s: negative size", who);
+    if (ldq < d || ldt < d) return set_error(QMLH_ERR_ARG, "%s: ld < d", who);
+    if (kind != QMLH_PAIR_GAUSSIAN && kind != QMLH_PAIR_LAPLACIAN)
+        return set_error(QMLH_ERR_ARG, "%s: kind must be Gaussian or Laplacian", who);
+    if (m == 0) return QMLH_OK;
+    if (!y) return set_error(QMLH_ERR_ARG, "%s: null pointer", who);
+    if (n == 0) return QMLH_OK;
+    if (!Xq || !Xt || !alpha) return set_error(QMLH_ERR_ARG, "%s: null pointer", who);
+    if (kind == QMLH_PAIR_GAUSSIAN) {
+        if (ldq % 4 || ldt % 4)
+            return set_error(QMLH_ERR_ARG, "%s: ldq, ldt must be multiples of 4", who);
+        if (!normsQ || !normsT) return set_error(QMLH_ERR_ARG, "%s: norms missing", who);
+    }
+    return QMLH_OK;
+}
+
 }  // namespace qmlh
 
 extern "C" size_t qmlh_predict_workspace(int64_t m, int64_t n) {
-    using namespace qmlh;
-    if (m <= 0 || n <= 0) return 0;
-    int64_t chunks = ceil_div(n, PL_T);  // the finer of the two chunkings
-    return (size_t)(chunks * m) * sizeof(double);
+    return qmlh::predict_workspace_bytes(m, n);
 }
 
 extern "C" int qmlh_krr_predict(int32_t kind, const double* Xq, int64_t m, int64_t ldq,
@@ -170,38 +262,37 @@ extern "C" int qmlh_krr_predict(int32_t kind, const double* Xq, int64_t m, int64
                                 double sigma, double* y, void* ws, size_t ws_bytes,
                                 qmlh_stream_t stream) {
     using namespace qmlh;
-    if (m < 0 || n < 0 || d < 0) return set_error(QMLH_ERR_ARG, "krr_predict: negative size");
-    if (ldq < d || ldt < d) return set_error(QMLH_ERR_ARG, "krr_predict: ld < d");
+    QMLH_TRY(predict_check(kind, Xq, m, ldq, normsQ, Xt, n, ldt, normsT, alpha, d, y,
+                           "krr_predict"));
     if (m == 0) return QMLH_OK;
-    if (!Xq || !Xt || !alpha || !y) return set_error(QMLH_ERR_ARG, "krr_predict: null pointer");
-    hipStream_t s = (hipStream_t)stream;
-    if (n == 0) {
-        return check_hip(hipMemsetAsync(y, 0, sizeof(double) * m, s), "krr_predict: zero y");
-    }
-    if (ws_bytes < qmlh_predict_workspace(m, n) || !ws)
+    if (n > 0 && (ws_bytes < predict_workspace_bytes(m, n) || !ws))
         return set_error(QMLH_ERR_WORKSPACE, "krr_predict: workspace too small");
-    double* part = static_cast<double*>(ws);
-    int64_t nchunks;
-    TimerScope ts(QMLH_TIMER_PREDICT, (double)m * (double)n, s);
-    if (kind == QMLH_PAIR_GAUSSIAN) {
-        if (ldq % 4 || ldt % 4)
-            return set_error(QMLH_ERR_ARG, "krr_predict: ldq, ldt must be multiples of 4");
-        if (!normsQ || !normsT) return set_error(QMLH_ERR_ARG, "krr_predict: norms missing");
-        nchunks = ceil_div(n, PG_T);
-        dim3 grid((unsigned)ceil_div(m, PG_Q), (unsigned)nchunks);
-        hipLaunchKernelGGL(predict_gauss_kernel, grid, dim3(256), 0, s, Xq, ldq, m, normsQ, Xt,
-                           ldt, n, normsT, alpha, d, -0.5 / (sigma * sigma), part);
-    } else if (kind == QMLH_PAIR_LAPLACIAN) {
-        nchunks = ceil_div(n, PL_T);
-        dim3 grid((unsigned)ceil_div(m, PL_Q), (unsigned)nchunks);
-        hipLaunchKernelGGL(predict_laplace_kernel, grid, dim3(256), 0, s, Xq, ldq, m, Xt, ldt, n,
-                           alpha, d, -1.0 / sigma, part);
-    } else {
-        return set_error(QMLH_ERR_ARG, "krr_predict: kind must be Gaussian or Laplacian");
-    }
-    QMLH_CHECK_LAUNCH("krr_predict launch");
-    hipLaunchKernelGGL(predict_reduce_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s,
-                       part, m, nchunks, y);
-    QMLH_CHECK_LAUNCH("krr_predict reduce launch");
-    return QMLH_OK;
+    return predict_impl(kind, Xq, m, ldq, normsQ, Xt, n, ldt, normsT, alpha, d, sigma, y,
+                        static_cast<double*>(ws), (hipStream_t)stream);
+}
+
+extern "C" size_t qmlh_krr_predict_noalloc_capacity(void) {
+    return (size_t)qmlh::PREDICT_SCRATCH_DOUBLES * sizeof(double);
+}
+
+extern "C" int qmlh_krr_predict_noalloc(int32_t kind, const double* Xq, int64_t m, int64_t ldq,
+                                        const double* normsQ, const double* Xt, int64_t n,
+                                        int64_t ldt, const double* normsT, const double* alpha,
+                                        int64_t d, double sigma, double* y,
+                                        qmlh_stream_t stream) {
+    using namespace qmlh;
+    QMLH_TRY(predict_check(kind, Xq, m, ldq, normsQ, Xt, n, ldt, normsT, alpha, d, y,
+                           "krr_predict_noalloc"));
+    if (m == 0) return QMLH_OK;
+    const size_t need = predict_workspace_bytes(m, n);
+    if (need > qmlh_krr_predict_noalloc_capacity())
+        return set_error(QMLH_ERR_ARG,
+                         "krr_predict_noalloc: %lld queries x %lld training rows need %zu bytes "
+                         "of scratch, above its %zu (ceil(n / 128) * m <= %lld)",
+                         (long long)m, (long long)n, need, qmlh_krr_predict_noalloc_capacity(),
+                         (long long)PREDICT_SCRATCH_DOUBLES);
+    double* part = nullptr;
+    QMLH_TRY(predict_scratch_setup(&part));
+    return predict_impl(kind, Xq, m, ldq, normsQ, Xt, n, ldt, normsT, alpha, d, sigma, y, part,
+                        (hipStream_t)stream);
 }
