@@ -85,6 +85,21 @@ int qmlh_atomic_cm_batch(const double *Z, const double *R, const int64_t *offset
                          double central_decay, double interaction_cutoff,
                          double interaction_decay, double *out, int64_t ldo, int32_t *status,
                          qmlh_stream_t stream);
+/* Rows of molecules with more than QMLH_ACM_MAX_ATOMS atoms (up to max_atoms), which
+ * qmlh_atomic_cm_batch reports with status bit 2 and leaves zero: the same rows, with each
+ * molecule staged in a slice of the workspace instead of LDS. Same arguments and status bits
+ * (bit 2 now for molecules above max_atoms); rows of smaller molecules are not touched, so a
+ * batch of mixed sizes is one qmlh_atomic_cm_batch call plus this one. ws:
+ * qmlh_atomic_cm_large_workspace(max_atoms, size) bytes. No counterpart of its own in the
+ * reference, whose fgenerate_*_atomic_coulomb_matrix (qml/frepresentations.f90:207-598) has
+ * no atom cap. */
+size_t qmlh_atomic_cm_large_workspace(int64_t max_atoms, int32_t size);
+int qmlh_atomic_cm_large(const double *Z, const double *R, const int64_t *offsets, int64_t nmol,
+                         const int64_t *central, const int64_t *cmol, int64_t nrows, int32_t size,
+                         int32_t sorting, double central_cutoff, double central_decay,
+                         double interaction_cutoff, double interaction_decay, double *out,
+                         int64_t ldo, int32_t *status, int64_t max_atoms, void *ws,
+                         size_t ws_bytes, qmlh_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Pairwise kernels and distances, VALU tile kernels (direct differences).

@@ -91,37 +91,50 @@ def lib():
     return _lib
 
 
-def ref_available() -> bool:
-    return os.path.exists(os.path.join(_HERE, "_ref", "libqmlref.so"))
+def ref_available(build: str = "_ref") -> bool:
+    """Whether a reference build exists: "_ref" (make -C oracle ref) or "_ref_fma" (make -C
+    oracle ref-fma: the reference's flags in full, -march=native, FMA contraction)."""
+    return os.path.exists(os.path.join(_HERE, build, "libqmlref.so"))
 
 
-def ref():
-    """Load the reference build (oracle/_ref/libqmlref.so)."""
+_refs = {}
+
+
+def ref(build: str = "_ref"):
+    """Load a reference build (oracle/<build>/libqmlref.so)."""
+    if build != "_ref":
+        if build not in _refs:
+            _refs[build] = _load_ref(build)
+        return _refs[build]
     global _ref
     if _ref is None:
-        path = os.path.join(_HERE, "_ref", "libqmlref.so")
-        if not os.path.exists(path):
-            raise RuntimeError(f"reference build missing: {path} (run make -C oracle ref)")
-        L = ctypes.CDLL(path)
-        ci = ctypes.c_int
-        L.ref_cm.argtypes = [_c_double_p, _c_double_p, ci, ci, ci, _c_double_p]
-        L.ref_cm.restype = None
-        L.ref_kernel.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p,
-                                 ctypes.c_double]
-        L.ref_kernel.restype = None
-        dbl = ctypes.c_double
-        _c_int_p = ctypes.POINTER(ctypes.c_int)
-        L.ref_atomic_cm.argtypes = [_c_double_p, _c_double_p, ci, _c_int_p, ci, ci, ci, dbl, dbl,
-                                    dbl, dbl, _c_double_p]
-        L.ref_atomic_cm.restype = None
-        L.ref_local_kernels.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_int_p, ci,
-                                        _c_int_p, ci, _c_double_p, ci, _c_double_p]
-        L.ref_local_kernels.restype = None
-        L.ref_kernel_ext.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p, dbl,
-                                     _c_double_p, ci, ci, dbl, ci]
-        L.ref_kernel_ext.restype = None
-        _ref = L
+        _ref = _load_ref(build)
     return _ref
+
+
+def _load_ref(build: str):
+    path = os.path.join(_HERE, build, "libqmlref.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"reference build missing: {path} (run make -C oracle ref)")
+    L = ctypes.CDLL(path)
+    ci = ctypes.c_int
+    L.ref_cm.argtypes = [_c_double_p, _c_double_p, ci, ci, ci, _c_double_p]
+    L.ref_cm.restype = None
+    L.ref_kernel.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p,
+                             ctypes.c_double]
+    L.ref_kernel.restype = None
+    dbl = ctypes.c_double
+    _c_int_p = ctypes.POINTER(ctypes.c_int)
+    L.ref_atomic_cm.argtypes = [_c_double_p, _c_double_p, ci, _c_int_p, ci, ci, ci, dbl, dbl,
+                                dbl, dbl, _c_double_p]
+    L.ref_atomic_cm.restype = None
+    L.ref_local_kernels.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_int_p, ci,
+                                    _c_int_p, ci, _c_double_p, ci, _c_double_p]
+    L.ref_local_kernels.restype = None
+    L.ref_kernel_ext.argtypes = [ci, _c_double_p, ci, _c_double_p, ci, ci, _c_double_p, dbl,
+                                 _c_double_p, ci, ci, dbl, ci]
+    L.ref_kernel_ext.restype = None
+    return L
 
 
 # --------------------------------------------------------------------------------------------
@@ -158,13 +171,13 @@ def coulomb_matrix_batch(Z, R, offsets, size, sorted_=True):
     return out
 
 
-def ref_coulomb_matrix(Z, R, size, sorting="row-norm"):
-    """The reference's own fgenerate_*coulomb_matrix via oracle/_ref."""
+def ref_coulomb_matrix(Z, R, size, sorting="row-norm", build="_ref"):
+    """The reference's own fgenerate_*coulomb_matrix via oracle/_ref (or another build)."""
     Z = np.ascontiguousarray(Z, dtype=np.float64)
     R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
     out = np.zeros(size * (size + 1) // 2)
-    ref().ref_cm(_ptr(Z), _ptr(R), Z.shape[0], size, 1 if sorting == "row-norm" else 0,
-                 _ptr(out))
+    ref(build).ref_cm(_ptr(Z), _ptr(R), Z.shape[0], size, 1 if sorting == "row-norm" else 0,
+                      _ptr(out))
     return out
 
 
@@ -295,11 +308,11 @@ def l2_distance(A, B):
 REF_KIND = {"laplacian": 0, "gaussian": 1, "manhattan": 2, "l2": 3}
 
 
-def ref_kernel(kind, A, B, sigma=1.0):
-    """The reference's own f*_kernel / f*_distance via oracle/_ref."""
+def ref_kernel(kind, A, B, sigma=1.0, build="_ref"):
+    """The reference's own f*_kernel / f*_distance via oracle/_ref (or another build)."""
     A, B, K = _prep(A, B)
-    ref().ref_kernel(REF_KIND[kind], _ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1],
-                     _ptr(K), float(sigma))
+    ref(build).ref_kernel(REF_KIND[kind], _ptr(A), A.shape[0], _ptr(B), B.shape[0], A.shape[1],
+                          _ptr(K), float(sigma))
     return K
 
 

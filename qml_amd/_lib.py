@@ -55,6 +55,10 @@ _SIGNATURES = {
     "qmlh_cm_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _i64, _vp, _vp]),
     "qmlh_atomic_cm_batch": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32,
                                             _f64, _f64, _f64, _f64, _vp, _i64, _vp, _vp]),
+    "qmlh_atomic_cm_large_workspace": (_sz, [_i64, _i32]),
+    "qmlh_atomic_cm_large": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32,
+                                            _f64, _f64, _f64, _f64, _vp, _i64, _vp, _i64, _vp,
+                                            _sz, _vp]),
     "qmlh_local_kernels": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _i64,
                                           _i64, _vp, _vp, _vp, _i64, _i64, _vp, _i32, _vp, _i64,
                                           _i64, _i32, _vp]),

@@ -427,6 +427,159 @@ __global__ __launch_bounds__(CM_WAVE) void atomic_cm_kernel(
     }
 }
 
+// Molecules of more than ACM_MAX_ATOMS atoms: the same rows as atomic_cm_kernel, with the
+// molecule, the distances to the central atom and the ordering keys held in this workgroup's
+// slice of global scratch instead of LDS and registers (one wavefront per row; the keys of
+// picked atoms are overwritten in place, as the selection above does in registers). Rows of
+// smaller molecules are left to atomic_cm_kernel.
+__host__ __device__ inline int64_t acm_large_stride(int64_t max_atoms, int size) {
+    return 6 * max_atoms + size;  // R (3 n), Z, distances, keys (n each), order (size)
+}
+
+template <bool ROWNORM>
+__global__ __launch_bounds__(CM_WAVE) void atomic_cm_large_kernel(
+    const double* __restrict__ Z, const double* __restrict__ R,
+    const int64_t* __restrict__ offsets, int64_t nmol, const int64_t* __restrict__ central,
+    const int64_t* __restrict__ cmol, int64_t nrows, int size, AcmParams p, int64_t ldo,
+    double* __restrict__ out, int32_t* __restrict__ status, int64_t max_atoms,
+    double* __restrict__ ws) {
+    const int lane = threadIdx.x;
+    const int d = size * (size + 1) / 2;
+    double* gR = ws + (int64_t)blockIdx.x * acm_large_stride(max_atoms, size);
+    double* gZ = gR + 3 * max_atoms;
+    double* gDk = gZ + max_atoms;
+    double* gKey = gDk + max_atoms;
+    double* gOrd = gKey + max_atoms;  // atom indices, stored as doubles
+
+    for (int64_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int64_t mol = cmol[row];
+        if (mol < 0 || mol >= nmol) continue;  // reported by atomic_cm_kernel
+        const int64_t a0 = offsets[mol];
+        const int64_t n = offsets[mol + 1] - a0;
+        if (n <= ACM_MAX_ATOMS) continue;  // atomic_cm_kernel's row
+        double* dst = out + row * ldo;
+        const int64_t k = central[row];
+        if (n > max_atoms || k < 0 || k >= n) {
+            for (int64_t q = lane; q < ldo; q += CM_WAVE) dst[q] = 0.0;
+            if (lane == 0 && status) atomicOr(status, n > max_atoms ? 2 : 4);
+            continue;
+        }
+        for (int64_t a = lane; a < n; a += CM_WAVE) {
+            gR[3 * a + 0] = R[3 * (a0 + a) + 0];
+            gR[3 * a + 1] = R[3 * (a0 + a) + 1];
+            gR[3 * a + 2] = R[3 * (a0 + a) + 2];
+            gZ[a] = Z[a0 + a];
+        }
+        __syncthreads();
+        int cnt_lane = 0;
+        for (int64_t a = lane; a < n; a += CM_WAVE) {
+            const double dk = acm_dist(gR, (int)a, (int)k);
+            gDk[a] = dk;
+            if (a != k && dk < p.cc) ++cnt_lane;
+        }
+#pragma unroll
+        for (int off = CM_WAVE / 2; off > 0; off >>= 1) cnt_lane += __shfl_xor(cnt_lane, off, CM_WAVE);
+        const int cnt = 1 + cnt_lane;
+        __syncthreads();
+        if (cnt > size) {
+            for (int64_t q = lane; q < ldo; q += CM_WAVE) dst[q] = 0.0;
+            if (lane == 0 && status) atomicOr(status, 1);
+            __syncthreads();
+            continue;
+        }
+        // ordering keys: row norm (picked -> 0, maximum first) or distance (picked -> huge)
+        for (int64_t x = lane; x < n; x += CM_WAVE) {
+            double v;
+            if (ROWNORM) {
+                if (x == k) {
+                    v = DBL_MAX;
+                } else {
+                    double acc = 0.0;
+                    for (int64_t j = 0; j < n; ++j) {
+                        const double e = acm_rn_entry(gR, gZ, gDk, (int)x, (int)j, p);
+                        const double e2 = e * e;
+                        acc = acc + e2;
+                    }
+                    v = acc;
+                }
+            } else {
+                v = gDk[x];
+            }
+            gKey[x] = v;
+        }
+        __syncthreads();
+        for (int it = 0; it < cnt; ++it) {
+            double bv = ROWNORM ? -1.0 : DBL_MAX;
+            int64_t bi = INT64_MAX;
+            for (int64_t x = lane; x < n; x += CM_WAVE) {
+                const double kv = gKey[x];
+                const bool better = ROWNORM ? (kv > bv || (kv == bv && x < bi))
+                                            : (kv < bv || (kv == bv && x < bi));
+                if (better) {
+                    bv = kv;
+                    bi = x;
+                }
+            }
+#pragma unroll
+            for (int off = CM_WAVE / 2; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(bv, off, CM_WAVE);
+                const int64_t oi = __shfl_xor(bi, off, CM_WAVE);
+                const bool better = ROWNORM ? (ov > bv || (ov == bv && oi < bi))
+                                            : (ov < bv || (ov == bv && oi < bi));
+                if (better) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (bi >= n) bi = 0;  // only with NaN keys (NaN coordinates): keep indices valid
+            if (lane == 0) {
+                gOrd[it] = (double)bi;
+                gKey[bi] = ROWNORM ? 0.0 : DBL_MAX;
+            }
+            __syncthreads();  // the picked key is visible before the next scan
+        }
+        for (int64_t q = lane; q < ldo; q += CM_WAVE) {
+            double v = 0.0;
+            if (q < d) {
+                int m, c;
+                cm_unpack((int)q, m, c);
+                if (m < cnt) {
+                    const int i = (int)gOrd[m], j = (int)gOrd[c];
+                    if (ROWNORM) {
+                        v = acm_rn_entry(gR, gZ, gDk, i, j, p);
+                    } else {
+                        const double di = gDk[i];
+                        if (!(di > p.cc)) {
+                            double pref = 1.0;
+                            if (di > p.cc - p.cd) pref = acm_decay(di, p.cc, p.cd);
+                            v = pref * acm_pair(gR, gZ, i, j, p);
+                            const double dj = gDk[j];
+                            if (dj > p.cc - p.cd)
+                                v = v * 0.5 * (cos(3.141592653589793 * (dj - p.cc + p.cd) / p.cd) + 1);
+                        }
+                    }
+                }
+            }
+            dst[q] = v;
+        }
+        __syncthreads();  // this row's scratch reads are done before the next row's stores
+    }
+}
+
+static constexpr int64_t ACM_LARGE_GRID = 1024;
+
+static AcmParams acm_params(double cc, double cd, double ic, double id) {
+    // Cut-off normalisation of qml/frepresentations.f90:254-273 (identical at 460-479).
+    AcmParams p{cc, cd, ic, id};
+    if (p.cc < 0) p.cc = DBL_MAX;
+    if (p.ic < 0 || p.ic > 2 * p.cc) p.ic = 2 * p.cc;
+    if (p.cd < 0) p.cd = 0.0;
+    else if (p.cd > p.cc) p.cd = p.cc;
+    if (p.id < 0) p.id = 0.0;
+    else if (p.id > p.ic) p.id = p.ic;
+    return p;
+}
+
 }  // namespace qmlh
 
 extern "C" int qmlh_atomic_cm_batch(const double* Z, const double* R, const int64_t* offsets,
@@ -449,14 +602,8 @@ extern "C" int qmlh_atomic_cm_batch(const double* Z, const double* R, const int6
     if (nrows == 0) return QMLH_OK;
     if (!Z || !R || !offsets || !central || !cmol || !out)
         return set_error(QMLH_ERR_ARG, "atomic_cm_batch: null pointer");
-    // Cut-off normalisation of qml/frepresentations.f90:254-273 (identical at 460-479).
-    AcmParams p{central_cutoff, central_decay, interaction_cutoff, interaction_decay};
-    if (p.cc < 0) p.cc = DBL_MAX;
-    if (p.ic < 0 || p.ic > 2 * p.cc) p.ic = 2 * p.cc;
-    if (p.cd < 0) p.cd = 0.0;
-    else if (p.cd > p.cc) p.cd = p.cc;
-    if (p.id < 0) p.id = 0.0;
-    else if (p.id > p.ic) p.id = p.ic;
+    const AcmParams p = acm_params(central_cutoff, central_decay, interaction_cutoff,
+                                   interaction_decay);
     int64_t grid = nrows < 262144 ? nrows : 262144;
     hipStream_t s = (hipStream_t)stream;
     TimerScope ts(QMLH_TIMER_CM, (double)nrows, s);
@@ -468,5 +615,51 @@ extern "C" int qmlh_atomic_cm_batch(const double* Z, const double* R, const int6
                            Z, R, offsets, nmol, central, cmol, nrows, (int)size, p, ldo, out,
                            status);
     QMLH_CHECK_LAUNCH("atomic_cm_batch launch");
+    return QMLH_OK;
+}
+
+extern "C" size_t qmlh_atomic_cm_large_workspace(int64_t max_atoms, int32_t size) {
+    if (max_atoms <= 0 || size < 1) return 0;
+    return (size_t)qmlh::ACM_LARGE_GRID * (size_t)qmlh::acm_large_stride(max_atoms, size) *
+           sizeof(double);
+}
+
+extern "C" int qmlh_atomic_cm_large(const double* Z, const double* R, const int64_t* offsets,
+                                    int64_t nmol, const int64_t* central, const int64_t* cmol,
+                                    int64_t nrows, int32_t size, int32_t sorting,
+                                    double central_cutoff, double central_decay,
+                                    double interaction_cutoff, double interaction_decay,
+                                    double* out, int64_t ldo, int32_t* status, int64_t max_atoms,
+                                    void* ws, size_t ws_bytes, qmlh_stream_t stream) {
+    using namespace qmlh;
+    if (nrows < 0 || nmol < 0 || max_atoms < 0)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_large: negative count");
+    if (size < 1 || size > QMLH_CM_MAX_SIZE)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_large: size %d outside [1, %d]", size,
+                         QMLH_CM_MAX_SIZE);
+    if (sorting != QMLH_ACM_ROW_NORM && sorting != QMLH_ACM_DISTANCE)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_large: unknown sorting %d", sorting);
+    const int64_t d = (int64_t)size * (size + 1) / 2;
+    if (ldo < d) return set_error(QMLH_ERR_ARG, "atomic_cm_large: ldo < size (size + 1) / 2");
+    if (nrows == 0 || max_atoms <= QMLH_ACM_MAX_ATOMS) return QMLH_OK;
+    if (max_atoms > (int64_t)1 << 30) return set_error(QMLH_ERR_ARG, "atomic_cm_large: max_atoms too large");
+    if (!Z || !R || !offsets || !central || !cmol || !out || !ws)
+        return set_error(QMLH_ERR_ARG, "atomic_cm_large: null pointer");
+    if (ws_bytes < qmlh_atomic_cm_large_workspace(max_atoms, size))
+        return set_error(QMLH_ERR_WORKSPACE, "atomic_cm_large: workspace too small");
+    const AcmParams p = acm_params(central_cutoff, central_decay, interaction_cutoff,
+                                   interaction_decay);
+    const int64_t grid = nrows < ACM_LARGE_GRID ? nrows : ACM_LARGE_GRID;
+    hipStream_t s = (hipStream_t)stream;
+    double* w = static_cast<double*>(ws);
+    if (sorting == QMLH_ACM_ROW_NORM)
+        hipLaunchKernelGGL(atomic_cm_large_kernel<true>, dim3((unsigned)grid), dim3(CM_WAVE), 0, s,
+                           Z, R, offsets, nmol, central, cmol, nrows, (int)size, p, ldo, out,
+                           status, max_atoms, w);
+    else
+        hipLaunchKernelGGL(atomic_cm_large_kernel<false>, dim3((unsigned)grid), dim3(CM_WAVE), 0,
+                           s, Z, R, offsets, nmol, central, cmol, nrows, (int)size, p, ldo, out,
+                           status, max_atoms, w);
+    QMLH_CHECK_LAUNCH("atomic_cm_large launch");
     return QMLH_OK;
 }

@@ -23,11 +23,9 @@ int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t 
 int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s);
 // C -= P P^T in split FP16 from a split of m_split rows, for the m rows starting at row off
 // (a multiple of 128): ncols == 0 the lower triangle of the m x m block, ncols > 0 the entries
-// i >= j of its first ncols columns. dense: the variant with one LDS buffer and three
-// workgroups per CU (faster alone, but it leaves no registers to kernels on other streams).
+// i >= j of its first ncols columns.
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
-                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
-                  bool dense = false);
+                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s);
 // C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three FP16 products.
 int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s);

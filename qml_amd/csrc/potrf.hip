@@ -221,14 +221,6 @@ static int trailing_syrk(T* P, int64_t lda, int64_t m, int64_t kb, T* C, int32_t
 
 // Matrices at least this large are factored with look-ahead.
 static constexpr int64_t PF_LOOKAHEAD_MIN = 4096;
-// Split-FP16 steps whose trailing matrix has at least this many rows skip the look-ahead: the
-// whole trailing triangle takes the dense update variant (three workgroups per CU, no
-// registers left for a concurrent panel), then the next block column is factored alone. The
-// update's saving grows as m^2 and passes the exposed panel time (about 0.8 ms for a 768-wide
-// block column) near this size.
-#ifndef PF_DENSE_MIN
-#define PF_DENSE_MIN (int64_t)1 << 62
-#endif
 // Trailing matrices at least this large keep the extra rows' update on the side stream.
 static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 
@@ -314,28 +306,6 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         T* C = P + kb * lda;  // A(k + kb, k + kb)
         const bool split = std::is_same<T, float>::value && split_ws && m > kb2;
         hipEvent_t rows_wait = nullptr;  // what the next panel's rows wait for
-        if (split && m >= PF_DENSE_MIN) {
-            // No look-ahead: the extra rows' part of the next block column on the third
-            // stream (as below), the whole trailing triangle with the dense update, then the
-            // next block column's panel on this stream, then the rest of the extra rows.
-            if (r > 0) {
-                rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
-                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
-                if (rc == QMLH_OK)
-                    rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
-                                          info, s3);
-                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
-                rows_wait = ev_e;
-            }
-            if (rc == QMLH_OK) rc = f16x3_split(reinterpret_cast<float*>(P), lda, m, kb, split_ws, s);
-            if (rc == QMLH_OK)
-                rc = f16x3_update(split_ws, m, kb, 0, m, 0, reinterpret_cast<float*>(C), lda, info,
-                                  s, true);
-            if (rc == QMLH_OK)
-                rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s, rows_wait, pf);
-            if (rc == QMLH_OK) rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
-            continue;
-        }
         if (split) {
             // split-FP16 trailing updates: split the panel once; the next block column gets
             // the same split-FP16 update as the rest, then the rest of the trailing matrix

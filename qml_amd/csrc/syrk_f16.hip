@@ -19,7 +19,9 @@
 // (k = 8 g .. 8 g + 7) as one 16-byte read.
 //
 // Tiles: 256 lanes own a 128 x 128 lower tile of C (2 x 2 waves of 64 x 64); chunks of 32
-// features are double buffered through LDS (4 panels x 8 KB per buffer).
+// features are double buffered through LDS (4 panels x 8 KB per buffer), copied there by
+// gfx950's direct global-to-LDS loads (no staging registers: 143 VGPRs, leaving room on each
+// CU for the look-ahead panel's kernels on the side stream).
 //
 // Tile order. Workgroups are handed to the 8 XCDs round robin (workgroup b to XCD b % 8), and
 // each XCD has its own L2. The tiles are grouped in 8 x 8 super-blocks, and the workgroup
@@ -108,32 +110,11 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const float* __restric
     *reinterpret_cast<f16x8_t*>(Pl + idx) = l;
 }
 
-// Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) into an LDS panel.
-__device__ __forceinline__ void sb_load(u32x4_t (&r)[2], const sb_elem* __restrict__ src,
-                                        int64_t m_pad, int64_t row0, int64_t c) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int p = t + 256 * h;  // 0 .. 511
-        const int q = p >> 7, row = p & 127;
-        const sb_elem* g = src + ((c * 4 + q) * m_pad + row0 + row) * 8;
-        r[h] = *reinterpret_cast<const u32x4_t*>(g);
-    }
-}
-
-__device__ __forceinline__ void sb_store(sb_elem* __restrict__ panel, const u32x4_t (&r)[2]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int p = t + 256 * h;
-        reinterpret_cast<u32x4_t*>(panel)[p] = r[h];  // (q * 128 + row) * 8 elements
-    }
-}
-
-// The same chunk copied straight from global memory into an LDS panel (gfx950's 16-byte
-// global_load_lds: lane l of a wave writes 16 bytes at the wave's LDS base + 16 l, so each
-// wave's 64 lanes fill one contiguous 1 KB run, the layout sb_store writes). No registers
-// are staged; the data is in LDS once vmcnt reaches zero.
+// Copy one 32-deep chunk of 128 rows (4 runs of 128 x 16 B) straight from global memory into
+// an LDS panel (gfx950's 16-byte global_load_lds: lane l of a wave writes 16 bytes at the
+// wave's LDS base + 16 l, so each wave's 64 lanes fill one contiguous 1 KB run: element
+// (q, row) of the panel at (q * 128 + row) * 8). No registers are staged; the data is in LDS
+// once vmcnt reaches zero.
 __device__ __forceinline__ void sb_load_lds(sb_elem* __restrict__ panel,
                                             const sb_elem* __restrict__ src, int64_t m_pad,
                                             int64_t row0, int64_t c) {
@@ -165,25 +146,12 @@ __device__ __forceinline__ f16x8_t sb_neg(f16x8_t v) {
 // and the products are subtracted by negating the second operand's fragments; the epilogue
 // only stores.
 //
-// Build-time shape of the kernel (each a -D switch; the defaults are the measured choice):
-//   SB_MFMA32  -- 1: v_mfma_f32_32x32x16_f16 (a wave's 64 x 64 block as 2 x 2 blocks of 32 x 32,
-//                 two 16-deep steps per chunk); 0: v_mfma_f32_16x16x32_f16 (4 x 4 blocks).
-//   SB_SINGLE  -- 1: one LDS buffer (the next chunk waits in registers; two barriers per
-//                 chunk), 32 KB per workgroup; 0: two buffers, 64 KB, one barrier per chunk.
-//   SB_OCC     -- workgroups per CU the register budget is set for (launch bounds).
-//   SB_PERSIST -- 0: one workgroup per tile; k > 0: a persistent grid of k workgroups per CU,
-//                 less SB_RESERVE workgroups in all (slots left to the look-ahead panel's
-//                 kernels on the side stream), each looping over tiles.
-#ifndef SB_MFMA32
-#define SB_MFMA32 0
-#endif
-#ifndef SB_PERSIST
-#define SB_PERSIST 0
-#endif
-#ifndef SB_RESERVE
-#define SB_RESERVE 0
-#endif
-
+// Measured alternatives (mixed fit at 50,000 molecules, trailing updates per fit): staging the
+// chunks through registers 157-161 ms (the DMA copy: 150 ms); v_mfma_f32_32x32x16_f16 with
+// the same wave tile 173 ms; one LDS buffer and three workgroups per CU 125 ms for the updates
+// alone, but the fit slows down (no registers left for the look-ahead panel), also when the
+// look-ahead is dropped for the large steps; persistent grids with slots kept free for the
+// panel 182-195 ms.
 static constexpr int SB_SUPER = 8;  // super-block side in tiles
 static constexpr int SB_XCDS = 8;
 
@@ -194,7 +162,7 @@ __host__ __device__ inline int64_t sb_super_count(int64_t t) {
     return T * (T + 1) / 2;
 }
 
-// Virtual workgroup v -> tile (bi, bj); false for padding and tiles above the diagonal.
+// Workgroup v -> tile (bi, bj); false for padding and tiles above the diagonal.
 // ncols == 0: v -> (XCD x = v % 8, its l-th workgroup) -> super-block 8 (l / 64) + x, tile
 // l % 64, so the workgroups an XCD holds at a time share 8 row and 8 column blocks of the panel
 // (one L2 per XCD). ncols > 0: row order over the next block column's tiles.
@@ -220,72 +188,19 @@ __device__ __forceinline__ bool sb_tile(int64_t v, int64_t t_tiles, int64_t ncol
 }
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-#if SB_MFMA32
-// 2 x 2 blocks of 32 x 32: lane l holds, of block (a, b), accumulator e at output row
-// row0 + 32 b + (l & 31) and column col0 + 32 a + 8 (e / 4) + 4 (l / 32) + e % 4.
-struct SbAcc {
-    f32x16_t v[2][2];
-};
-__device__ __forceinline__ int64_t sb_acc_row(int a, int b, int e, int lane) {
-    return 32 * b + (lane & 31);
-}
-__device__ __forceinline__ int64_t sb_acc_col(int a, int b, int e, int lane) {
-    return 32 * a + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-}
-#define SB_NA 2
-#define SB_NE 16
-#else
-// 4 x 4 blocks of 16 x 16: lane l holds, of block (a, b), accumulator e at output row
-// row0 + 16 b + (l & 15) and column col0 + 16 a + 4 (l / 16) + e.
+// A wave's 64 x 64 block as 4 x 4 blocks of 16 x 16: lane l holds, of block (a, b), accumulator
+// e at output row row0 + 16 b + (l & 15) and column col0 + 16 a + 4 (l / 16) + e.
 struct SbAcc {
     f32x4_t v[4][4];
 };
-__device__ __forceinline__ int64_t sb_acc_row(int a, int b, int e, int lane) {
-    return 16 * b + (lane & 15);
-}
-__device__ __forceinline__ int64_t sb_acc_col(int a, int b, int e, int lane) {
-    return 16 * a + 4 * (lane >> 4) + e;
-}
-#define SB_NA 4
-#define SB_NE 4
-#endif
 
-// The chunk's products on fragments read from one LDS buffer (4 panels).
-#if SB_MFMA32
-struct SbFrags {
-    f16x8_t fh[2][2], fl[2][2], sh[2][2], sl[2][2];  // [block][16-deep step]
-};
-__device__ __forceinline__ void sb_read_frags(SbFrags& F, const sb_elem (*buf)[SB_PANEL], int fr,
-                                              int sr, int lane) {
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int g = 2 * ks + (lane >> 5), r = lane & 31;
-            F.fh[q][ks] = sb_frag(buf[2], fr + 32 * q + r, g);
-            F.fl[q][ks] = sb_frag(buf[3], fr + 32 * q + r, g);
-            F.sh[q][ks] = sb_neg(sb_frag(buf[0], sr + 32 * q + r, g));
-            F.sl[q][ks] = sb_neg(sb_frag(buf[1], sr + 32 * q + r, g));
-        }
-}
-__device__ __forceinline__ void sb_products(SbAcc& A, const SbFrags& F) {
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                A.v[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.fl[a][ks], F.sh[b][ks], A.v[a][b], 0, 0, 0);
-                A.v[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.fh[a][ks], F.sl[b][ks], A.v[a][b], 0, 0, 0);
-                A.v[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.fh[a][ks], F.sh[b][ks], A.v[a][b], 0, 0, 0);
-            }
-}
-#else
 struct SbFrags {
     f16x8_t fh[4], fl[4], sh[4], sl[4];
 };
+
+// The chunk's fragments from one LDS buffer (4 panels): first operand = column blocks, second
+// (negated) = row blocks.
 __device__ __forceinline__ void sb_read_frags(SbFrags& F, const sb_elem (*buf)[SB_PANEL], int fr,
                                               int sr, int lane) {
     const int r = lane & 15, g = lane >> 4;
@@ -297,6 +212,7 @@ __device__ __forceinline__ void sb_read_frags(SbFrags& F, const sb_elem (*buf)[S
         F.sl[q] = sb_neg(sb_frag(buf[1], sr + 16 * q + r, g));
     }
 }
+
 __device__ __forceinline__ void sb_products(SbAcc& A, const SbFrags& F) {
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -307,22 +223,34 @@ __device__ __forceinline__ void sb_products(SbAcc& A, const SbFrags& F) {
             A.v[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.fh[a], F.sh[b], A.v[a][b], 0, 0, 0);
         }
 }
-#endif
 
-// One 128 x 128 tile: 256 lanes, wave w owns the 64 x 64 block at output rows
-// rows0 + 64 (w >> 1), columns cols0 + 64 (w & 1). All 256 lanes must call this (barriers).
-// SINGLE: one LDS buffer (the next chunk waits in registers, two barriers per chunk).
-// DMA: chunks go from global memory straight into LDS (no staging registers).
-template <bool SINGLE, bool DMA>
-__device__ __forceinline__ void sb_tile_update(sb_elem (*lds)[4][SB_PANEL],
-                                               const sb_elem* __restrict__ Ph,
-                                               const sb_elem* __restrict__ Pl, int64_t m,
-                                               int64_t m_pad, int64_t kb_pad, int64_t ncol,
-                                               float* __restrict__ C, int64_t ldc, int64_t bi,
-                                               int64_t bj, float up, float down) {
+// Ph / Pl point at row `off` of a split panel with row stride m_pad; the update covers m rows
+// from there. ncols == 0: the lower triangle of the m x m block (super-block order);
+// ncols > 0: the entries i >= j of its first ncols columns (the next block column of a
+// look-ahead factorisation), tiles in row order. One workgroup per tile.
+__global__ __launch_bounds__(256, 2) void syrk_f16x3_kernel(const sb_elem* __restrict__ Ph,
+                                                             const sb_elem* __restrict__ Pl,
+                                                             int64_t m, int64_t m_pad,
+                                                             int64_t kb_pad, int64_t ncols,
+                                                             float* __restrict__ C, int64_t ldc,
+                                                             const int32_t* __restrict__ info,
+                                                             const unsigned* __restrict__ hdr) {
+    // buffers x {row hi, row lo, col hi, col lo}
+    __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SB_PANEL];
+    if (info && *info) return;
+    const int64_t t_tiles = (m + SB_T - 1) / SB_T;
+    int64_t bi, bj;
+    if (!sb_tile(blockIdx.x, t_tiles, ncols, bi, bj)) return;
+    // the panel is scaled by 2^e, so the products carry 2^(2e): C is scaled up on load and
+    // back on store (exact powers of two)
+    const int e2 = 2 * sb_scale_exp(hdr);
+    const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
+    const int64_t ncol = ncols > 0 ? ncols : m;  // column bound
     const int64_t rows0 = bi * SB_T, cols0 = bj * SB_T;
+
     const int t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
+    const int r = lane & 15, g = lane >> 4;
     // Wave w: first operand (accumulator rows) = output columns cols0 + 64 (w & 1),
     // second operand = output rows rows0 + 64 (w >> 1).
     const int fr = 64 * (w & 1), sr = 64 * (w >> 1);
@@ -332,189 +260,79 @@ __device__ __forceinline__ void sb_tile_update(sb_elem (*lds)[4][SB_PANEL],
     // interior: the wave's 64 x 64 block lies inside the matrix and on or below the diagonal
     // in every entry, so its loads and stores need no masks.
     const bool interior = live && row0 + 63 < m && col0 + 63 < ncol && col0 + 63 <= row0;
+
+    // first chunk in flight before C is read
+    sb_load_lds(lds[0][0], Ph, m_pad, rows0, 0);
+    sb_load_lds(lds[0][1], Pl, m_pad, rows0, 0);
+    sb_load_lds(lds[0][2], Ph, m_pad, cols0, 0);
+    sb_load_lds(lds[0][3], Pl, m_pad, cols0, 0);
     SbAcc acc;
     if (interior) {
 #pragma unroll
-        for (int a = 0; a < SB_NA; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < SB_NA; ++b)
+            for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int e = 0; e < SB_NE; ++e)
-                    acc.v[a][b][e] = C[(row0 + sb_acc_row(a, b, e, lane)) +
-                                       (col0 + sb_acc_col(a, b, e, lane)) * ldc] * up;
+                for (int e = 0; e < 4; ++e)
+                    acc.v[a][b][e] =
+                        C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] * up;
     } else {
 #pragma unroll
-        for (int a = 0; a < SB_NA; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < SB_NA; ++b)
+            for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int e = 0; e < SB_NE; ++e) {
-                    const int64_t i = row0 + sb_acc_row(a, b, e, lane);
-                    const int64_t j = col0 + sb_acc_col(a, b, e, lane);
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t i = row0 + 16 * b + r;
+                    const int64_t j = col0 + 16 * a + 4 * g + e;
                     acc.v[a][b][e] = (live && i < m && j <= i && j < ncol) ? C[i + j * ldc] * up : 0.0f;
                 }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this lane's copies have landed in LDS
+    __syncthreads();
 
     const int64_t nchunks = kb_pad / SB_KC;
-    if (DMA) {
-        sb_load_lds(lds[0][0], Ph, m_pad, rows0, 0);
-        sb_load_lds(lds[0][1], Pl, m_pad, rows0, 0);
-        sb_load_lds(lds[0][2], Ph, m_pad, cols0, 0);
-        sb_load_lds(lds[0][3], Pl, m_pad, cols0, 0);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        int st = 0;
-        for (int64_t c = 0; c < nchunks; ++c) {
-            const bool more = c + 1 < nchunks;
-            const int nx = SINGLE ? 0 : st ^ 1;
-            if (!SINGLE && more) {  // the other buffer is free since the last barrier
-                sb_load_lds(lds[nx][0], Ph, m_pad, rows0, c + 1);
-                sb_load_lds(lds[nx][1], Pl, m_pad, rows0, c + 1);
-                sb_load_lds(lds[nx][2], Ph, m_pad, cols0, c + 1);
-                sb_load_lds(lds[nx][3], Pl, m_pad, cols0, c + 1);
-            }
-            SbFrags F;
-            sb_read_frags(F, lds[st], fr, sr, lane);
-            if (SINGLE) {
-                __syncthreads();  // every wave holds its fragments: the buffer may be refilled
-                if (more) {
-                    sb_load_lds(lds[0][0], Ph, m_pad, rows0, c + 1);
-                    sb_load_lds(lds[0][1], Pl, m_pad, rows0, c + 1);
-                    sb_load_lds(lds[0][2], Ph, m_pad, cols0, c + 1);
-                    sb_load_lds(lds[0][3], Pl, m_pad, cols0, c + 1);
-                }
-            }
-            sb_products(acc, F);
-            __builtin_amdgcn_s_waitcnt(0);  // this lane's copies have landed in LDS
-            __syncthreads();
-            if (!SINGLE) st = nx;
-        }
-    } else {
-    u32x4_t rh[2], rl[2], ch[2], cl[2];
-    sb_load(rh, Ph, m_pad, rows0, 0);
-    sb_load(rl, Pl, m_pad, rows0, 0);
-    sb_load(ch, Ph, m_pad, cols0, 0);
-    sb_load(cl, Pl, m_pad, cols0, 0);
-    sb_store(lds[0][0], rh);
-    sb_store(lds[0][1], rl);
-    sb_store(lds[0][2], ch);
-    sb_store(lds[0][3], cl);
-    __syncthreads();
     int st = 0;
     for (int64_t c = 0; c < nchunks; ++c) {
-        const bool more = c + 1 < nchunks;
-        if (more) {
-            sb_load(rh, Ph, m_pad, rows0, c + 1);
-            sb_load(rl, Pl, m_pad, rows0, c + 1);
-            sb_load(ch, Ph, m_pad, cols0, c + 1);
-            sb_load(cl, Pl, m_pad, cols0, c + 1);
+        if (c + 1 < nchunks) {  // the other buffer is free since the last barrier
+            sb_load_lds(lds[st ^ 1][0], Ph, m_pad, rows0, c + 1);
+            sb_load_lds(lds[st ^ 1][1], Pl, m_pad, rows0, c + 1);
+            sb_load_lds(lds[st ^ 1][2], Ph, m_pad, cols0, c + 1);
+            sb_load_lds(lds[st ^ 1][3], Pl, m_pad, cols0, c + 1);
         }
         SbFrags F;
         sb_read_frags(F, lds[st], fr, sr, lane);
-        if (SINGLE) {
-            __syncthreads();  // every wave holds its fragments: the buffer may be overwritten
-            sb_products(acc, F);
-            if (more) {
-                sb_store(lds[0][0], rh);
-                sb_store(lds[0][1], rl);
-                sb_store(lds[0][2], ch);
-                sb_store(lds[0][3], cl);
-            }
-            __syncthreads();
-        } else {
-            sb_products(acc, F);
-            if (more) {
-                sb_store(lds[st ^ 1][0], rh);
-                sb_store(lds[st ^ 1][1], rl);
-                sb_store(lds[st ^ 1][2], ch);
-                sb_store(lds[st ^ 1][3], cl);
-            }
-            __syncthreads();
-            st ^= 1;
-        }
-    }
+        sb_products(acc, F);
+        __builtin_amdgcn_s_waitcnt(0);  // the next chunk has landed in LDS
+        __syncthreads();
+        st ^= 1;
     }
 
     if (!live) return;
     if (interior) {
 #pragma unroll
-        for (int a = 0; a < SB_NA; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < SB_NA; ++b)
+            for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int e = 0; e < SB_NE; ++e)
-                    C[(row0 + sb_acc_row(a, b, e, lane)) + (col0 + sb_acc_col(a, b, e, lane)) * ldc] =
+                for (int e = 0; e < 4; ++e)
+                    C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] =
                         acc.v[a][b][e] * down;
         return;
     }
 #pragma unroll
-    for (int a = 0; a < SB_NA; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < SB_NA; ++b)
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + r;
+            if (i >= m) continue;
 #pragma unroll
-            for (int e = 0; e < SB_NE; ++e) {
-                const int64_t i = row0 + sb_acc_row(a, b, e, lane);
-                const int64_t j = col0 + sb_acc_col(a, b, e, lane);
-                if (i < m && j <= i && j < ncol) C[i + j * ldc] = acc.v[a][b][e] * down;
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + 16 * a + 4 * g + e;
+                if (j > i || j >= ncol) continue;
+                C[i + j * ldc] = acc.v[a][b][e] * down;
             }
-}
-
-// Ph / Pl point at row `off` of a split panel with row stride m_stride (its m_pad); the update
-// covers m rows from there. ncols == 0: the lower triangle of the m x m block (super-block
-// order); ncols > 0: the entries i >= j of its first ncols columns (the next block column of a
-// look-ahead factorisation), tiles in row order. nvirt virtual workgroups; a workgroup takes
-// virtual workgroups blockIdx.x, blockIdx.x + gridDim.x, ...
-#ifndef SB_DMA
-#define SB_DMA 0
-#endif
-template <bool SINGLE, bool DMA>
-__global__ __launch_bounds__(256, SINGLE ? 3 : 2) void syrk_f16x3_kernel(const sb_elem* __restrict__ Ph,
-                                                                  const sb_elem* __restrict__ Pl,
-                                                                  int64_t m, int64_t m_pad,
-                                                                  int64_t kb_pad, int64_t ncols,
-                                                                  int64_t nvirt,
-                                                                  float* __restrict__ C, int64_t ldc,
-                                                                  const int32_t* __restrict__ info,
-                                                                  const unsigned* __restrict__ hdr) {
-    // buffers x {row hi, row lo, col hi, col lo}
-    __shared__ __attribute__((aligned(16))) sb_elem lds[SINGLE ? 1 : 2][4][SB_PANEL];
-    if (info && *info) return;
-    // the panel is scaled by 2^e, so the products carry 2^(2e): C is scaled up on load and
-    // back on store (exact powers of two)
-    const int e2 = 2 * sb_scale_exp(hdr);
-    const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
-    const int64_t t_tiles = (m + SB_T - 1) / SB_T;
-    const int64_t ncol = ncols > 0 ? ncols : m;  // column bound
-#if SB_PERSIST
-    // Every tile ends on a barrier after its last LDS reads, so the next tile's prologue may
-    // overwrite the buffers.
-    for (int64_t v = blockIdx.x; v < nvirt; v += gridDim.x) {
-        int64_t bi, bj;
-        if (!sb_tile(v, t_tiles, ncols, bi, bj)) continue;  // uniform across the workgroup
-        sb_tile_update<SINGLE, DMA>(lds, Ph, Pl, m, m_pad, kb_pad, ncol, C, ldc, bi, bj, up, down);
-    }
-#else
-    int64_t bi, bj;
-    if (blockIdx.x >= nvirt || !sb_tile(blockIdx.x, t_tiles, ncols, bi, bj)) return;
-    sb_tile_update<SINGLE, DMA>(lds, Ph, Pl, m, m_pad, kb_pad, ncol, C, ldc, bi, bj, up, down);
-#endif
-}
-
-// Workgroups of a persistent launch: SB_PERSIST per CU less SB_RESERVE, a multiple of the XCD
-// count (so that virtual workgroup v keeps v % 8 as its XCD), at least 8.
-static int64_t sb_persistent_grid() {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            n <= 0)
-            n = 256;
-        cus = n;
-    }
-    int64_t g = (int64_t)SB_PERSIST * cus - SB_RESERVE;
-    g = (g / SB_XCDS) * SB_XCDS;
-    return g < SB_XCDS ? SB_XCDS : g;
+        }
 }
 
 size_t syrk_f16_workspace(int64_t n, int64_t kb) {
@@ -551,8 +369,7 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
 }
 
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
-                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
-                  bool dense) {
+                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m_split, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
@@ -562,31 +379,20 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
     const sb_elem* Ph = sb_panels(const_cast<void*>(ws)) + off * 8;
     const sb_elem* Pl = sb_panels(const_cast<void*>(ws)) + m_pad * kb_pad + off * 8;
     const int64_t tm = ceil_div(m, (int64_t)SB_T);
-    int64_t nvirt;
+    int64_t grid;
     double flops;
     if (ncols > 0) {
-        nvirt = tm * ceil_div(ncols, (int64_t)SB_T);
+        grid = tm * ceil_div(ncols, (int64_t)SB_T);
         flops = ((double)m * (double)ncols - (double)ncols * (double)(ncols - 1) / 2.0) * 2.0 *
                 (double)kb;
     } else {
-        nvirt = ceil_div(sb_super_count(tm), (int64_t)SB_XCDS) * SB_XCDS * SB_SUPER * SB_SUPER;
+        grid = ceil_div(sb_super_count(tm), (int64_t)SB_XCDS) * SB_XCDS * SB_SUPER * SB_SUPER;
         flops = (double)m * (double)(m + 1) * (double)kb;
-    }
-    int64_t grid = nvirt;
-    if (SB_PERSIST > 0 && ncols == 0) {
-        const int64_t g = sb_persistent_grid();
-        grid = g < nvirt ? g : nvirt;
     }
     if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_update: grid too large");
     TimerScope ts(ncols > 0 ? QMLH_TIMER_GEMM : QMLH_TIMER_SYRK, flops, s);
-    if (dense)
-        hipLaunchKernelGGL((syrk_f16x3_kernel<true, (SB_DMA & 1) != 0>), dim3((unsigned)grid),
-                           dim3(256), 0, s, Ph, Pl, m, m_pad, kb_pad, ncols, nvirt, C, ldc, info,
-                           static_cast<const unsigned*>(ws));
-    else
-        hipLaunchKernelGGL((syrk_f16x3_kernel<false, (SB_DMA & 2) != 0>), dim3((unsigned)grid),
-                           dim3(256), 0, s, Ph, Pl, m, m_pad, kb_pad, ncols, nvirt, C, ldc, info,
-                           static_cast<const unsigned*>(ws));
+    hipLaunchKernelGGL(syrk_f16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Ph, Pl, m,
+                       m_pad, kb_pad, ncols, C, ldc, info, static_cast<const unsigned*>(ws));
     QMLH_CHECK_LAUNCH("syrk_f16x3 launch");
     return QMLH_OK;
 }
@@ -597,7 +403,7 @@ int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int
                 const int32_t* info, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     QMLH_TRY(f16x3_split(P, ldp, m, kb, ws, s));
-    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s, false);
+    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s);
 }
 
 }  // namespace qmlh

@@ -95,12 +95,13 @@ def generate_atomic_coulomb_matrices_device(Zd, Rd, Od, central, cmol, size: int
                                             sorting: str = "distance", central_cutoff=1e6,
                                             central_decay=-1, interaction_cutoff=1e6,
                                             interaction_decay=-1, ld: int | None = None,
-                                            out=None, stream=None):
+                                            out=None, stream=None, max_atoms: int | None = None):
     """Batched atomic CM from device tensors: Z f64 (total,), R f64 (total, 3), offsets i64
     (nmol+1,), central i64 (rows,) central atom within its molecule, cmol i64 (rows,) its
     molecule. Returns ``(out, status)``: the (rows, ld) FP64 device tensor (pad columns zero)
-    and the device status word (bit 1: a central atom needs more than ``size`` rows; bit 2: a
-    molecule has more than 256 atoms; bit 4: an index out of range)."""
+    and the device status word (bit 1: a central atom needs more than ``size`` rows; bit 4: an
+    index out of range). max_atoms: the largest molecule (read from ``offsets`` when None);
+    molecules above 256 atoms go through the global-memory kernel (qmlh_atomic_cm_large)."""
     import torch
 
     if sorting not in _ACM_SORTINGS:
@@ -112,11 +113,25 @@ def generate_atomic_coulomb_matrices_device(Zd, Rd, Od, central, cmol, size: int
     if out is None:
         out = torch.empty((rows, ld), dtype=torch.float64, device=Od.device)
     status = torch.zeros(1, dtype=torch.int32, device=Od.device)
-    _lib.check(_lib.lib().qmlh_atomic_cm_batch(
-        _lib.ptr(Zd), _lib.ptr(Rd), _lib.ptr(Od), Od.shape[0] - 1, _lib.ptr(central),
-        _lib.ptr(cmol), rows, size, _ACM_SORTINGS[sorting], float(central_cutoff),
-        float(central_decay), float(interaction_cutoff), float(interaction_decay),
-        _lib.ptr(out), ld, _lib.ptr(status), _lib.stream_handle(stream)))
+    L = _lib.lib()
+    args = (_lib.ptr(Zd), _lib.ptr(Rd), _lib.ptr(Od), Od.shape[0] - 1, _lib.ptr(central),
+            _lib.ptr(cmol), rows, size, _ACM_SORTINGS[sorting], float(central_cutoff),
+            float(central_decay), float(interaction_cutoff), float(interaction_decay),
+            _lib.ptr(out), ld)
+    if max_atoms is None:
+        max_atoms = int((Od[1:] - Od[:-1]).max()) if Od.shape[0] > 1 else 0
+    if max_atoms <= _lib.ACM_MAX_ATOMS:
+        _lib.check(L.qmlh_atomic_cm_batch(*args, _lib.ptr(status), _lib.stream_handle(stream)))
+        return out, status
+    # Rows of molecules above the LDS kernel's capacity (reported there as status bit 2) are
+    # then computed by the global-memory kernel, which reports its own status.
+    first = torch.zeros(1, dtype=torch.int32, device=Od.device)
+    _lib.check(L.qmlh_atomic_cm_batch(*args, _lib.ptr(first), _lib.stream_handle(stream)))
+    ws = torch.empty(max(1, L.qmlh_atomic_cm_large_workspace(max_atoms, size) // 8),
+                     dtype=torch.float64, device=Od.device)
+    _lib.check(L.qmlh_atomic_cm_large(*args, _lib.ptr(status), max_atoms, _lib.ptr(ws),
+                                      ws.numel() * 8, _lib.stream_handle(stream)))
+    status |= first & ~2
     return out, status
 
 
@@ -125,7 +140,7 @@ def _raise_acm_status(code: int, size: int) -> None:
         raise ValueError(f"ERROR: Coulomb matrix generation: {size} size set, but more "
                          "atoms lie within the central cut-off")
     if code & 2:
-        raise ValueError(f"molecule with more than {_lib.ACM_MAX_ATOMS} atoms")
+        raise ValueError("molecule larger than the atomic Coulomb matrix workspace")
     if code & 4:
         raise ValueError("central atom index out of range")
 
@@ -148,7 +163,8 @@ def generate_atomic_coulomb_matrices(Z, R, offsets, size: int = 23, sorting: str
     dev = Od.device
     out, status = generate_atomic_coulomb_matrices_device(
         Zd, Rd, Od, torch.from_numpy(central).to(dev), torch.from_numpy(cmol).to(dev), size,
-        sorting, central_cutoff, central_decay, interaction_cutoff, interaction_decay, ld)
+        sorting, central_cutoff, central_decay, interaction_cutoff, interaction_decay, ld,
+        max_atoms=int(nat.max()) if nat.size else 0)
     _raise_acm_status(int(status.item()), size)
     return out
 
@@ -196,6 +212,6 @@ def generate_atomic_coulomb_matrix(nuclear_charges, coordinates, size=23, sortin
     cmol = torch.zeros(idx.size, dtype=torch.int64, device=dev)
     out, status = generate_atomic_coulomb_matrices_device(
         Zd, Rd, Od, central, cmol, size, sorting, central_cutoff, central_decay,
-        interaction_cutoff, interaction_decay)
+        interaction_cutoff, interaction_decay, max_atoms=Z.shape[0])
     _raise_acm_status(int(status.item()), size)
     return out[:, :d].cpu().numpy()

@@ -245,3 +245,46 @@ def test_atomic_kernel_wrappers(qm7_local_gpu):
                              (get_atomic_kernels_laplacian, get_local_kernels_laplacian, 4000.0)):
         assert np.array_equal(bits(wrap(mols, mols, [sig])), bits(local(Xa, Xa, Na, Na, [sig])))
         assert np.array_equal(bits(wrap(tests, mols, [sig])), bits(local(Xt, Xa, Nt, Na, [sig])))
+
+
+@pytest.mark.parametrize("sorting", ["row-norm", "distance"])
+def test_acm_molecules_above_lds_capacity(sorting):
+    """Molecules of more than 256 atoms (beyond the LDS kernel's capacity) in a batch with small
+    ones: their rows come from the global-memory kernel and match the oracle like the rest.
+    A large molecule here is 15 to 25 synthetic molecules laid out 12 A apart, so a 3 A central
+    cut-off keeps every central atom's neighbourhood within size."""
+    from qml_amd.representations import generate_atomic_coulomb_matrices
+
+    parts = synthetic.generate(60, 43, 5, 23)
+    rng = np.random.default_rng(8)
+    Zs, Rs = [], []
+    start = 0
+    for k in (15, 25, 18):  # three large molecules
+        Zl, Rl = [], []
+        for j in range(k):
+            Z, R = parts.molecule(start + j)
+            Zl.append(Z)
+            Rl.append(R + 12.0 * np.array([j % 5, (j // 5) % 5, j // 25], dtype=np.float64))
+        start += k
+        Zs.append(np.concatenate(Zl))
+        Rs.append(np.concatenate(Rl))
+    small = synthetic.generate(20, 44, 3, 29)
+    for m in range(small.nmol):  # small molecules between and after them
+        Z, R = small.molecule(m)
+        pos = int(rng.integers(0, len(Zs) + 1))
+        Zs.insert(pos, Z)
+        Rs.insert(pos, R)
+    nat = np.array([len(z) for z in Zs])
+    assert nat.max() > 256 and nat.min() < 30
+    off = np.zeros(len(Zs) + 1, np.int64)
+    np.cumsum(nat, out=off[1:])
+    kw = dict(central_cutoff=3.0, central_decay=0.5, interaction_cutoff=5.0, interaction_decay=1.0)
+    size = 40
+    got = generate_atomic_coulomb_matrices(np.concatenate(Zs).astype(np.float64),
+                                           np.concatenate(Rs), off, size, sorting,
+                                           **kw).cpu().numpy()
+    d = size * (size + 1) // 2
+    assert np.all(got[:, d:] == 0)
+    ref = np.concatenate([oracle.atomic_coulomb_matrix(Z, R, size, sorting, **kw)
+                          for Z, R in zip(Zs, Rs)])
+    _acm_ok(got[:, :d], ref, True)
