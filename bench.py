@@ -637,14 +637,17 @@ def run_sweep(count, log):
 
     from qml_amd import synthetic
     from qml_amd.representations import cm_length, generate_coulomb_matrices
-    from qml_amd.sweep import BatchedFit, learning_curve_grid, sequential_fit
+    from qml_amd.autotune import Autotuner
+    from qml_amd.sweep import BatchedFit, learning_curve_grid, sequential_fit, tuning_key
 
     probs = learning_curve_grid(count=count)
     nmax = max(p.n for p in probs)
     pool, _, size = synthetic.workload("krr_sweep_64", n_train=nmax, n_test=0)
     Xd = generate_coulomb_matrices(*pool.ragged(), size=size)
     d = cm_length(size)
-    fit = BatchedFit(probs, Xd, d, pool.labels)  # nb from the autotuner
+    tuner = Autotuner()
+    fit = BatchedFit(probs, Xd, d, pool.labels, tuner=tuner)  # nb and branches tuned
+    tuned = tuner._read().get(tuning_key(probs), {}).get("timings", {})
     s = fit.stream
     fit.run()
     reps = 5
@@ -677,11 +680,12 @@ def run_sweep(count, log):
     flops = sum(p.n ** 3 / 3.0 for p in probs)
     pairs = sum(p.n * (p.n + 1) / 2.0 * d for p in probs)
     log(f"sweep: graph {graph_ms:.2f} ms, direct {direct_ms:.2f} ms, sequential {seq_ms:.2f} ms "
-        f"(nb={fit.nb}, {fit.launches} launches)")
+        f"(nb={fit.nb}, branches={fit.branches}, {fit.launches} launches)")
     out = {"workload": f"{count} Laplacian KRR fits, n log-uniform [500, 8000], size-23 CM "
                        "(d=276), sigma x lambda grid, shared pool in HBM",
            "problems": count, "n_min": min(p.n for p in probs), "n_max": nmax,
-           "nb": fit.nb, "launches_in_graph": fit.launches,
+           "nb": fit.nb, "branches": fit.branches, "launches_in_graph": fit.launches,
+           "autotune_ms": {k: 1e3 * v for k, v in tuned.items()},
            "graph_ms": graph_ms, "direct_launches_ms": direct_ms, "sequential_ms": seq_ms,
            "speedup_vs_sequential": seq_ms / graph_ms,
            "cholesky_flops": flops, "build_pair_features": pairs,
@@ -885,7 +889,9 @@ def main():
         "qm7": qm7_out,
         "cho_invert": invert_out,
         "phases_fp64": phases,
-        "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
+        # pcg: the whole refinement (qmlh_pcg_solve), work = CG iterations
+        "phases_mixed": ({k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
+                              "work_per_step": v[2] / args.steps}
                           for k, v in timers_mx.items() if v[1] > 0} if timers_mx else None),
         "roofline": roofmx or roof64,
         "roofline_fp64": roof64,

@@ -367,7 +367,11 @@ int qmlh_krr_predict_noalloc(int32_t kind, const double *Xq, int64_t m, int64_t 
  * rows, ldx >= d), reads its labels y[y_offset .. y_offset + n) and writes alpha[alpha_offset ..
  * alpha_offset + n) (alpha ranges must not overlap); info[p] follows LAPACK (0, or the first
  * failing column + 1). norms (squared row norms of X, qmlh_row_sqnorms) are needed when any
- * problem is Gaussian. nb (64, 128 or 256) is the Cholesky block size.
+ * problem is Gaussian. nb (64, 128 or 256) is the Cholesky block size. branches (1 .. 4): the
+ * problems are split into that many groups of about equal n^3 work whose factorisations step
+ * on streams of their own (the plan creates branches - 1 streams at the default priority and
+ * forks from / joins back to the caller's stream), so small launches of one group overlap the
+ * large ones of another; results do not depend on it.
  *
  * qmlh_krr_batched_create plans the launches and uploads the job tables into ws (a blocking
  * copy; ws 256-byte aligned, qmlh_krr_batched_workspace bytes; it also holds the factors).
@@ -390,9 +394,10 @@ typedef struct qmlh_problem {
 
 typedef struct qmlh_batched_plan qmlh_batched_plan;
 
-size_t qmlh_krr_batched_workspace(const qmlh_problem *probs, int32_t nprob, int32_t nb);
+size_t qmlh_krr_batched_workspace(const qmlh_problem *probs, int32_t nprob, int32_t nb,
+                                  int32_t branches);
 int qmlh_krr_batched_create(const qmlh_problem *probs, int32_t nprob, int32_t nb,
-                            const double *X, int64_t x_rows, int64_t ldx, int64_t d,
+                            int32_t branches, const double *X, int64_t x_rows, int64_t ldx, int64_t d,
                             const double *norms, const double *y, int64_t y_len, double *alpha,
                             int64_t alpha_len, int32_t *info, void *ws, size_t ws_bytes,
                             qmlh_batched_plan **plan);
@@ -456,7 +461,8 @@ enum {
     QMLH_TIMER_POTRS = 6,     /* both triangular solves                     2 n^2    */
     QMLH_TIMER_CM = 7,        /* Coulomb matrices                           molecules */
     QMLH_TIMER_LOCAL = 8,     /* local kernels                  atom-pair features    */
-    QMLH_TIMER_CLASSES = 9
+    QMLH_TIMER_PCG = 9,       /* mixed-fit refinement, qmlh_pcg_solve          iterations */
+    QMLH_TIMER_CLASSES = 10
 };
 int qmlh_timer_enable(int32_t on);
 int qmlh_timer_read(int32_t timer_class, double *total_ms_h, int64_t *launches_h,

@@ -113,8 +113,8 @@ _SIGNATURES = {
     "qmlh_krr_predict_noalloc_capacity": (_sz, []),
     "qmlh_krr_predict_noalloc": (ctypes.c_int, [_i32, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp,
                                                 _vp, _i64, _f64, _vp, _vp]),
-    "qmlh_krr_batched_workspace": (_sz, [_vp, _i32, _i32]),
-    "qmlh_krr_batched_create": (ctypes.c_int, [_vp, _i32, _i32, _vp, _i64, _i64, _i64, _vp, _vp,
+    "qmlh_krr_batched_workspace": (_sz, [_vp, _i32, _i32, _i32]),
+    "qmlh_krr_batched_create": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp, _i64, _i64, _i64, _vp, _vp,
                                                _i64, _vp, _i64, _vp, _vp, _sz,
                                                ctypes.POINTER(_vp)]),
     "qmlh_krr_batched_run": (ctypes.c_int, [_vp, _vp]),
@@ -130,7 +130,7 @@ _SIGNATURES = {
 }
 
 TIMER_CLASSES = {"syrk": 0, "gemm": 1, "gauss": 2, "pair": 3, "predict": 4, "potrf": 5,
-                 "potrs": 6, "cm": 7, "local": 8}
+                 "potrs": 6, "cm": 7, "local": 8, "pcg": 9}
 
 
 def timer_enable(on: bool = True) -> None:

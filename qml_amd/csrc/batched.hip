@@ -14,7 +14,10 @@
 //            diagonal blocks (one workgroup per problem, FP64 in one wavefront's registers),
 //            one solves the panel rows below them (256 rows per workgroup), and an MFMA launch
 //            updates the rest of the nb-wide panel; per block column one MFMA launch updates
-//            the lower trailing tiles;
+//            the lower trailing tiles. With branches > 1 the problems are split into that many
+//            groups of about equal n^3 work, each stepping in lock step on a stream of its own
+//            (forked from and joined back to the caller's stream), so that the small,
+//            latency-bound panel launches of one group run beside the updates of another;
 //   solve    forward and backward substitutions of all problems in two single launches:
 //            workgroups take 64-row blocks from an atomic ticket in level order (block l of
 //            every problem that has one, then block l + 1, ...), wait (bounded) for the blocks
@@ -23,8 +26,8 @@
 // Each launch reads its jobs (one per active problem: pointers, sizes, first tile) from a
 // table the planner uploads once; a workgroup finds its job by binary search over the jobs'
 // first-tile indices. Problems are ordered largest first, so the big factorisations start at
-// once and small ones fill the gaps. The launch sequence only depends on the problem sizes and
-// nb, so it is recorded once and replayed as one graph launch.
+// once and small ones fill the gaps. The launch sequence only depends on the problem sizes, nb
+// and the branch count, so it is recorded once and replayed as one graph launch.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -369,7 +372,10 @@ struct BLaunch {
     int64_t grid;
     size_t table_off;  // byte offset of the job table in the plan tables
     int njobs;
+    int branch;  // factor launches: the problem group (stream) it belongs to; -1: main stream
 };
+
+static constexpr int MAX_BRANCHES = 4;
 
 struct HostProblem {
     int64_t n, x_row0, y_off, a_off, ld, k_off, z_off, flag_off;
@@ -383,6 +389,7 @@ struct PlanLayout {
     std::vector<BLaunch> launches;
     std::vector<unsigned char> tables;
     size_t level_off = 0;  // byte offset of the solve level prefix in the tables
+    int branches = 1;      // problem groups factored on streams of their own
     size_t k_elems = 0, z_elems = 0, sync_ints = 0;
     size_t ws_k = 0, ws_z = 0, ws_sync = 0, ws_tables = 0, ws_total = 0;
 };
@@ -408,8 +415,8 @@ static T* at(uintptr_t base, int64_t elems) {
     return reinterpret_cast<T*>(base + (uintptr_t)elems * sizeof(T));
 }
 
-static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Bases* B,
-                      PlanLayout& L) {
+static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, int32_t branches,
+                      const Bases* B, PlanLayout& L) {
     L = PlanLayout();
     for (int i = 0; i < nprob; ++i) {
         HostProblem h;
@@ -447,7 +454,7 @@ static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Ba
 
     // clear
     L.launches.push_back({BL_CLEAR, (int64_t)ceil_div((int64_t)std::max<size_t>(nprob, foff), 256),
-                          0, 0});
+                          0, 0, -1});
     // build, one launch per kind present
     for (int kind : {QMLH_PAIR_GAUSSIAN, QMLH_PAIR_LAPLACIAN}) {
         std::vector<BBuildJob> jobs;
@@ -469,84 +476,117 @@ static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Ba
         }
         if (jobs.empty()) continue;
         L.launches.push_back({kind == QMLH_PAIR_GAUSSIAN ? BL_BUILD_GAUSS : BL_BUILD_LAPLACE,
-                              tiles, push_table(L.tables, jobs), (int)jobs.size()});
+                              tiles, push_table(L.tables, jobs), (int)jobs.size(), -1});
     }
-    // factor
-    const int64_t nmax = L.probs.empty() ? 0 : L.probs[0].n;
-    for (int64_t k = 0; k < nmax; k += nb) {
-        for (int64_t jj = 0; jj < nb && k + jj < nmax; jj += BW) {
-            std::vector<BPanelJob> pj;
-            std::vector<BUpdateJob> uj;
-            int64_t blocks = 0, tiles = 0;
-            for (const auto& h : L.probs) {
-                const int64_t c = k + jj;
-                if (h.n <= c) continue;
-                const int64_t kb = std::min<int64_t>(nb, h.n - k);
-                if (jj >= kb) continue;
-                const int w = (int)std::min<int64_t>(BW, kb - jj);
-                BPanelJob p;
-                p.A = Kp(h, c, c);
-                p.ld = h.ld;
-                p.m = h.n - c - w;
-                p.col_base = c;
-                p.block_begin = blocks;
-                p.info = infop(h);
-                p.w = w;
-                p.pad = 0;
-                blocks += ceil_div(p.m, (int64_t)BROWS);
-                pj.push_back(p);
-                const int64_t ncols = kb - jj - w;  // rest of this nb-wide panel
-                if (ncols > 0) {
-                    BUpdateJob u;
-                    u.A = Kp(h, c + w, c);
-                    u.C = Kp(h, c + w, c + w);
-                    u.ld = h.ld;
-                    u.M = h.n - c - w;
-                    u.N = ncols;
-                    u.Kd = w;
-                    u.tiles_m = ceil_div(u.M, (int64_t)BT);
-                    u.tile_begin = tiles;
-                    u.info = infop(h);
-                    u.lower_tiles = 0;
-                    u.pad = 0;
-                    tiles += u.tiles_m * ceil_div(u.N, (int64_t)BT);
-                    uj.push_back(u);
+    // factor: the lock-step launch sequence of one group of problems (indices into L.probs,
+    // largest first)
+    auto factor_group = [&](const std::vector<int>& idx, int branch, std::vector<BLaunch>& out) {
+        const int64_t nmax_g = idx.empty() ? 0 : L.probs[idx[0]].n;
+        for (int64_t k = 0; k < nmax_g; k += nb) {
+            for (int64_t jj = 0; jj < nb && k + jj < nmax_g; jj += BW) {
+                std::vector<BPanelJob> pj;
+                std::vector<BUpdateJob> uj;
+                int64_t blocks = 0, tiles = 0;
+                for (int i : idx) {
+                    const HostProblem& h = L.probs[i];
+                    const int64_t c = k + jj;
+                    if (h.n <= c) continue;
+                    const int64_t kb = std::min<int64_t>(nb, h.n - k);
+                    if (jj >= kb) continue;
+                    const int w = (int)std::min<int64_t>(BW, kb - jj);
+                    BPanelJob p;
+                    p.A = Kp(h, c, c);
+                    p.ld = h.ld;
+                    p.m = h.n - c - w;
+                    p.col_base = c;
+                    p.block_begin = blocks;
+                    p.info = infop(h);
+                    p.w = w;
+                    p.pad = 0;
+                    blocks += ceil_div(p.m, (int64_t)BROWS);
+                    pj.push_back(p);
+                    const int64_t ncols = kb - jj - w;  // rest of this nb-wide panel
+                    if (ncols > 0) {
+                        BUpdateJob u;
+                        u.A = Kp(h, c + w, c);
+                        u.C = Kp(h, c + w, c + w);
+                        u.ld = h.ld;
+                        u.M = h.n - c - w;
+                        u.N = ncols;
+                        u.Kd = w;
+                        u.tiles_m = ceil_div(u.M, (int64_t)BT);
+                        u.tile_begin = tiles;
+                        u.info = infop(h);
+                        u.lower_tiles = 0;
+                        u.pad = 0;
+                        tiles += u.tiles_m * ceil_div(u.N, (int64_t)BT);
+                        uj.push_back(u);
+                    }
                 }
+                if (!pj.empty()) {
+                    const size_t off = push_table(L.tables, pj);
+                    out.push_back({BL_POTF2, (int64_t)pj.size(), off, (int)pj.size(), branch});
+                    // jobs with m = 0 own no row blocks: find_job skips them (same first block)
+                    if (blocks > 0) out.push_back({BL_TRSM, blocks, off, (int)pj.size(), branch});
+                }
+                if (!uj.empty())
+                    out.push_back({BL_UPDATE, tiles, push_table(L.tables, uj), (int)uj.size(),
+                                   branch});
             }
-            if (!pj.empty()) {
-                const size_t off = push_table(L.tables, pj);
-                L.launches.push_back({BL_POTF2, (int64_t)pj.size(), off, (int)pj.size()});
-                // jobs with m = 0 own no row blocks: find_job skips them (same first block)
-                if (blocks > 0) L.launches.push_back({BL_TRSM, blocks, off, (int)pj.size()});
+            std::vector<BUpdateJob> tj;
+            int64_t tiles = 0;
+            for (int i : idx) {
+                const HostProblem& h = L.probs[i];
+                if (h.n <= k) continue;
+                const int64_t kb = std::min<int64_t>(nb, h.n - k);
+                const int64_t m = h.n - k - kb;
+                if (m <= 0) continue;
+                BUpdateJob u;
+                u.A = Kp(h, k + kb, k);
+                u.C = Kp(h, k + kb, k + kb);
+                u.ld = h.ld;
+                u.M = m;
+                u.N = m;
+                u.Kd = kb;
+                u.tiles_m = ceil_div(m, (int64_t)BT);
+                u.tile_begin = tiles;
+                u.info = infop(h);
+                u.lower_tiles = 1;
+                u.pad = 0;
+                tiles += u.tiles_m * (u.tiles_m + 1) / 2;
+                tj.push_back(u);
             }
-            if (!uj.empty())
-                L.launches.push_back({BL_UPDATE, tiles, push_table(L.tables, uj), (int)uj.size()});
+            if (!tj.empty())
+                out.push_back({BL_UPDATE, tiles, push_table(L.tables, tj), (int)tj.size(), branch});
         }
-        std::vector<BUpdateJob> tj;
-        int64_t tiles = 0;
-        for (const auto& h : L.probs) {
-            if (h.n <= k) continue;
-            const int64_t kb = std::min<int64_t>(nb, h.n - k);
-            const int64_t m = h.n - k - kb;
-            if (m <= 0) continue;
-            BUpdateJob u;
-            u.A = Kp(h, k + kb, k);
-            u.C = Kp(h, k + kb, k + kb);
-            u.ld = h.ld;
-            u.M = m;
-            u.N = m;
-            u.Kd = kb;
-            u.tiles_m = ceil_div(m, (int64_t)BT);
-            u.tile_begin = tiles;
-            u.info = infop(h);
-            u.lower_tiles = 1;
-            u.pad = 0;
-            tiles += u.tiles_m * (u.tiles_m + 1) / 2;
-            tj.push_back(u);
-        }
-        if (!tj.empty())
-            L.launches.push_back({BL_UPDATE, tiles, push_table(L.tables, tj), (int)tj.size()});
+    };
+    // Problem groups of about equal Cholesky work (n^3): each problem, largest first, joins the
+    // least loaded group. Each group's sequence runs on a stream of its own, so the
+    // latency-bound panel launches of one group overlap the updates of another.
+    const int G = std::max(1, std::min<int>(branches, (int)L.probs.size()));
+    L.branches = G;
+    std::vector<std::vector<int>> groups(G);
+    std::vector<double> load(G, 0.0);
+    for (int i = 0; i < (int)L.probs.size(); ++i) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups[g].push_back(i);
+        const double n = (double)L.probs[i].n;
+        load[g] += n * n * n;
     }
+    std::vector<std::vector<BLaunch>> seq(G);
+    for (int g = 0; g < G; ++g) factor_group(groups[g], g, seq[g]);
+    // interleaved, so that the host enqueues the groups evenly
+    for (size_t i = 0;; ++i) {
+        bool any = false;
+        for (int g = 0; g < G; ++g) {
+            if (i < seq[g].size()) {
+                L.launches.push_back(seq[g][i]);
+                any = true;
+            }
+        }
+        if (!any) break;
+    }
+    const int64_t nmax = L.probs.empty() ? 0 : L.probs[0].n;
     // solves: problem table (forward, backward) and the level prefix
     {
         std::vector<BSolveProb> fw, bw;
@@ -579,8 +619,8 @@ static int build_plan(const qmlh_problem* P, int32_t nprob, int32_t nb, const Ba
             const size_t fw_off = push_table(L.tables, fw);
             const size_t bw_off = push_table(L.tables, bw);
             // For the solves, njobs carries the number of levels.
-            L.launches.push_back({BL_SOLVE_FWD, acc, fw_off, (int)nlev});
-            L.launches.push_back({BL_SOLVE_BWD, acc, bw_off, (int)nlev});
+            L.launches.push_back({BL_SOLVE_FWD, acc, fw_off, (int)nlev, -1});
+            L.launches.push_back({BL_SOLVE_BWD, acc, bw_off, (int)nlev, -1});
         }
     }
     L.ws_k = 0;
@@ -601,16 +641,37 @@ struct qmlh_batched_plan {
     const double* norms = nullptr;
     int32_t* info = nullptr;
     int32_t nprob = 0;
+    // streams of problem groups 1 .. branches - 1 (group 0 runs on the caller's stream), the
+    // fork event and one join event per side stream; all at the default priority
+    int nside = 0;
+    hipStream_t side[qmlh::MAX_BRANCHES - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[qmlh::MAX_BRANCHES - 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
-static int validate(const qmlh_problem* probs, int32_t nprob, int32_t nb) {
+static void release_streams(qmlh_batched_plan* p) {
+    for (int i = 0; i < qmlh::MAX_BRANCHES - 1; ++i) {
+        if (p->join[i]) (void)hipEventDestroy(p->join[i]);
+        if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+        p->join[i] = nullptr;
+        p->side[i] = nullptr;
+    }
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    p->fork = nullptr;
+    p->nside = 0;
+}
+
+static int validate(const qmlh_problem* probs, int32_t nprob, int32_t nb, int32_t branches) {
     using namespace qmlh;
     if (nprob < 0) return set_error(QMLH_ERR_ARG, "batched: nprob < 0");
     if (nprob > 0 && !probs) return set_error(QMLH_ERR_ARG, "batched: problems are null");
     if (nb != 64 && nb != 128 && nb != 256)
         return set_error(QMLH_ERR_ARG, "batched: nb must be 64, 128 or 256 (got %d)", nb);
+    if (branches < 1 || branches > MAX_BRANCHES)
+        return set_error(QMLH_ERR_ARG, "batched: branches must be 1 .. %d (got %d)", MAX_BRANCHES,
+                         branches);
     std::vector<std::pair<int64_t, int64_t>> out;
     for (int i = 0; i < nprob; ++i) {
         const qmlh_problem& p = probs[i];
@@ -630,23 +691,24 @@ static int validate(const qmlh_problem* probs, int32_t nprob, int32_t nb) {
 }
 
 extern "C" size_t qmlh_krr_batched_workspace(const qmlh_problem* probs, int32_t nprob,
-                                             int32_t nb) {
+                                             int32_t nb, int32_t branches) {
     using namespace qmlh;
-    if (validate(probs, nprob, nb) != QMLH_OK) return 0;
+    if (validate(probs, nprob, nb, branches) != QMLH_OK) return 0;
     PlanLayout L;
-    build_plan(probs, nprob, nb, nullptr, L);
+    build_plan(probs, nprob, nb, branches, nullptr, L);
     return L.ws_total;
 }
 
 extern "C" int qmlh_krr_batched_create(const qmlh_problem* probs, int32_t nprob, int32_t nb,
-                                       const double* X, int64_t x_rows, int64_t ldx, int64_t d,
-                                       const double* norms, const double* y, int64_t y_len,
-                                       double* alpha, int64_t alpha_len, int32_t* info, void* ws,
+                                       int32_t branches, const double* X, int64_t x_rows,
+                                       int64_t ldx, int64_t d, const double* norms,
+                                       const double* y, int64_t y_len, double* alpha,
+                                       int64_t alpha_len, int32_t* info, void* ws,
                                        size_t ws_bytes, qmlh_batched_plan** plan) {
     using namespace qmlh;
     if (!plan) return set_error(QMLH_ERR_ARG, "batched: plan is null");
     *plan = nullptr;
-    QMLH_TRY(validate(probs, nprob, nb));
+    QMLH_TRY(validate(probs, nprob, nb, branches));
     if (d < 0 || ldx < d) return set_error(QMLH_ERR_ARG, "batched: ldx < d");
     for (int i = 0; i < nprob; ++i) {
         const qmlh_problem& q = probs[i];
@@ -665,7 +727,7 @@ extern "C" int qmlh_krr_batched_create(const qmlh_problem* probs, int32_t nprob,
     if ((reinterpret_cast<uintptr_t>(ws) & 255) != 0)
         return set_error(QMLH_ERR_ARG, "batched: workspace must be 256-byte aligned");
     PlanLayout probe;
-    build_plan(probs, nprob, nb, nullptr, probe);
+    build_plan(probs, nprob, nb, branches, nullptr, probe);
     if (ws_bytes < probe.ws_total)
         return set_error(QMLH_ERR_ARG, "batched: workspace too small (%zu < %zu bytes)", ws_bytes,
                          probe.ws_total);
@@ -676,7 +738,7 @@ extern "C" int qmlh_krr_batched_create(const qmlh_problem* probs, int32_t nprob,
     Bases b{wb + probe.ws_k, wb + probe.ws_z, wb + probe.ws_sync,
             reinterpret_cast<uintptr_t>(info), reinterpret_cast<uintptr_t>(y),
             reinterpret_cast<uintptr_t>(alpha)};
-    build_plan(probs, nprob, nb, &b, p->layout);
+    build_plan(probs, nprob, nb, branches, &b, p->layout);
     p->ws = w;
     p->X = X;
     p->ldx = ldx;
@@ -684,69 +746,118 @@ extern "C" int qmlh_krr_batched_create(const qmlh_problem* probs, int32_t nprob,
     p->norms = norms;
     p->info = info;
     p->nprob = nprob;
-    if (!p->layout.tables.empty()) {
-        hipError_t e = hipMemcpy(w + p->layout.ws_tables, p->layout.tables.data(),
-                                 p->layout.tables.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            delete p;
-            return check_hip(e, "batched: table upload");
+    int rc = QMLH_OK;
+    if (p->layout.branches > 1) {
+        rc = check_hip(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming), "batched: event");
+        for (int i = 0; rc == QMLH_OK && i < p->layout.branches - 1; ++i) {
+            rc = check_hip(hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking),
+                           "batched: branch stream");
+            if (rc == QMLH_OK)
+                rc = check_hip(hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming),
+                               "batched: event");
+            if (rc == QMLH_OK) p->nside = i + 1;
         }
+    }
+    if (rc == QMLH_OK && !p->layout.tables.empty())
+        rc = check_hip(hipMemcpy(w + p->layout.ws_tables, p->layout.tables.data(),
+                                 p->layout.tables.size(), hipMemcpyHostToDevice),
+                       "batched: table upload");
+    if (rc != QMLH_OK) {
+        release_streams(p);
+        delete p;
+        return rc;
     }
     *plan = p;
     return QMLH_OK;
+}
+
+static int launch_one(const qmlh_batched_plan* plan, const qmlh::BLaunch& l, hipStream_t s) {
+    using namespace qmlh;
+    const PlanLayout& L = plan->layout;
+    unsigned char* tab = plan->ws + L.ws_tables;
+    int32_t* sync = reinterpret_cast<int32_t*>(plan->ws + L.ws_sync);
+    const int64_t* levels = reinterpret_cast<const int64_t*>(tab + L.level_off);
+    const dim3 grid((unsigned)l.grid), block(256);
+    switch (l.kind) {
+        case BL_CLEAR:
+            hipLaunchKernelGGL(bclear_kernel, grid, block, 0, s, plan->info, (int64_t)plan->nprob,
+                               sync, (int64_t)L.sync_ints);
+            break;
+        case BL_BUILD_GAUSS:
+            hipLaunchKernelGGL(bbuild_gauss_kernel, grid, block, 0, s,
+                               reinterpret_cast<const BBuildJob*>(tab + l.table_off), l.njobs,
+                               plan->X, plan->ldx, plan->d, plan->norms);
+            break;
+        case BL_BUILD_LAPLACE:
+            hipLaunchKernelGGL(bbuild_laplace_kernel, grid, block, 0, s,
+                               reinterpret_cast<const BBuildJob*>(tab + l.table_off), l.njobs,
+                               plan->X, plan->ldx, plan->d);
+            break;
+        case BL_POTF2:
+            hipLaunchKernelGGL(bpotf2_kernel, grid, block, 0, s,
+                               reinterpret_cast<const BPanelJob*>(tab + l.table_off));
+            break;
+        case BL_TRSM:
+            hipLaunchKernelGGL(btrsm_kernel, grid, block, 0, s,
+                               reinterpret_cast<const BPanelJob*>(tab + l.table_off), l.njobs);
+            break;
+        case BL_UPDATE:
+            hipLaunchKernelGGL(bupdate_kernel, grid, block, 0, s,
+                               reinterpret_cast<const BUpdateJob*>(tab + l.table_off), l.njobs);
+            break;
+        case BL_SOLVE_FWD:
+            hipLaunchKernelGGL(bsolve_kernel<true>, grid, block, 0, s,
+                               reinterpret_cast<const BSolveProb*>(tab + l.table_off), levels,
+                               l.njobs, sync);
+            break;
+        case BL_SOLVE_BWD:
+            hipLaunchKernelGGL(bsolve_kernel<false>, grid, block, 0, s,
+                               reinterpret_cast<const BSolveProb*>(tab + l.table_off), levels,
+                               l.njobs, sync + 2);
+            break;
+    }
+    return check_hip(hipGetLastError(), "batched launch");
 }
 
 extern "C" int qmlh_krr_batched_run(qmlh_batched_plan* plan, qmlh_stream_t stream) {
     using namespace qmlh;
     if (!plan) return set_error(QMLH_ERR_ARG, "batched: plan is null");
     hipStream_t s = (hipStream_t)stream;
-    const PlanLayout& L = plan->layout;
-    unsigned char* tab = plan->ws + L.ws_tables;
-    int32_t* sync = reinterpret_cast<int32_t*>(plan->ws + L.ws_sync);
-    const int64_t* levels = reinterpret_cast<const int64_t*>(tab + L.level_off);
-    for (const BLaunch& l : L.launches) {
-        const dim3 grid((unsigned)l.grid), block(256);
-        switch (l.kind) {
-            case BL_CLEAR:
-                hipLaunchKernelGGL(bclear_kernel, grid, block, 0, s, plan->info,
-                                   (int64_t)plan->nprob, sync, (int64_t)L.sync_ints);
-                break;
-            case BL_BUILD_GAUSS:
-                hipLaunchKernelGGL(bbuild_gauss_kernel, grid, block, 0, s,
-                                   reinterpret_cast<const BBuildJob*>(tab + l.table_off), l.njobs,
-                                   plan->X, plan->ldx, plan->d, plan->norms);
-                break;
-            case BL_BUILD_LAPLACE:
-                hipLaunchKernelGGL(bbuild_laplace_kernel, grid, block, 0, s,
-                                   reinterpret_cast<const BBuildJob*>(tab + l.table_off), l.njobs,
-                                   plan->X, plan->ldx, plan->d);
-                break;
-            case BL_POTF2:
-                hipLaunchKernelGGL(bpotf2_kernel, grid, block, 0, s,
-                                   reinterpret_cast<const BPanelJob*>(tab + l.table_off));
-                break;
-            case BL_TRSM:
-                hipLaunchKernelGGL(btrsm_kernel, grid, block, 0, s,
-                                   reinterpret_cast<const BPanelJob*>(tab + l.table_off), l.njobs);
-                break;
-            case BL_UPDATE:
-                hipLaunchKernelGGL(bupdate_kernel, grid, block, 0, s,
-                                   reinterpret_cast<const BUpdateJob*>(tab + l.table_off), l.njobs);
-                break;
-            case BL_SOLVE_FWD:
-                hipLaunchKernelGGL(bsolve_kernel<true>, grid, block, 0, s,
-                                   reinterpret_cast<const BSolveProb*>(tab + l.table_off), levels,
-                                   l.njobs, sync);
-                break;
-            case BL_SOLVE_BWD:
-                hipLaunchKernelGGL(bsolve_kernel<false>, grid, block, 0, s,
-                                   reinterpret_cast<const BSolveProb*>(tab + l.table_off), levels,
-                                   l.njobs, sync + 2);
-                break;
+    // Factor launches of group g > 0 go to side stream g - 1, after a fork from the caller's
+    // stream; the first main-stream launch after them (the solves) waits for every side stream
+    // (a join), so the whole fit stays ordered on the caller's stream, also under capture.
+    bool forked = false;
+    auto fork = [&]() -> int {
+        QMLH_TRY(check_hip(hipEventRecord(plan->fork, s), "batched: fork"));
+        for (int i = 0; i < plan->nside; ++i)
+            QMLH_TRY(check_hip(hipStreamWaitEvent(plan->side[i], plan->fork, 0), "batched: fork"));
+        forked = true;
+        return QMLH_OK;
+    };
+    auto join = [&]() -> int {
+        int rc = QMLH_OK;
+        for (int i = 0; i < plan->nside; ++i) {
+            int r = check_hip(hipEventRecord(plan->join[i], plan->side[i]), "batched: join");
+            if (r == QMLH_OK) r = check_hip(hipStreamWaitEvent(s, plan->join[i], 0), "batched: join");
+            if (rc == QMLH_OK) rc = r;
         }
-        QMLH_CHECK_LAUNCH("batched launch");
+        forked = false;
+        return rc;
+    };
+    int rc = QMLH_OK;
+    for (const BLaunch& l : plan->layout.launches) {
+        if (plan->nside > 0 && l.branch >= 0 && !forked) rc = fork();
+        if (rc == QMLH_OK && forked && l.branch < 0) rc = join();
+        if (rc != QMLH_OK) break;
+        hipStream_t ls = l.branch > 0 ? plan->side[l.branch - 1] : s;
+        rc = launch_one(plan, l, ls);
+        if (rc != QMLH_OK) break;
     }
-    return QMLH_OK;
+    if (forked) {  // also after a failed launch: a capture must not end with unjoined streams
+        const int r = join();
+        if (rc == QMLH_OK) rc = r;
+    }
+    return rc;
 }
 
 extern "C" int qmlh_krr_batched_capture(qmlh_batched_plan* plan, qmlh_stream_t stream) {
@@ -807,5 +918,6 @@ extern "C" void qmlh_krr_batched_destroy(qmlh_batched_plan* plan) {
     if (!plan) return;
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    release_streams(plan);
     delete plan;
 }

@@ -594,6 +594,7 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
     if (ws_bytes < qmlh_pcg_workspace(n, r))
         return set_error(QMLH_ERR_WORKSPACE, "pcg_solve: workspace too small");
     hipStream_t s = (hipStream_t)stream;
+    TimerScope timer(QMLH_TIMER_PCG, 0.0, s);
     double* w = static_cast<double*>(ws);
     double* rv = w;
     double* z = rv + n;
@@ -693,6 +694,7 @@ extern "C" int qmlh_pcg_solve(const double* A, int64_t n, int64_t lda, const flo
         prev_true = relres;
         QMLH_TRY(restart());
     }
+    timer.set_work((double)it);
     if (iters_h) *iters_h = it;
     if (relres_h) *relres_h = relres;
     if (backward_h) *backward_h = backward;

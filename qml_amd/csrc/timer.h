@@ -12,6 +12,8 @@ class TimerScope {
    public:
     TimerScope(int cls, double work, hipStream_t s);
     ~TimerScope();
+    // Work known only at the end (iterations, for instance): replaces the constructor's.
+    void set_work(double work) { work_ = work; }
     TimerScope(const TimerScope&) = delete;
     TimerScope& operator=(const TimerScope&) = delete;
 

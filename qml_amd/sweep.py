@@ -23,6 +23,21 @@ from .math import add_diagonal_device, potrf_device, potrs_device
 
 KINDS = {"gaussian": _lib.PAIR_GAUSSIAN, "laplacian": _lib.PAIR_LAPLACIAN}
 NB_CANDIDATES = (64, 128, 256)
+BRANCH_CANDIDATES = (1, 2, 4)
+
+
+def schedule_name(nb: int, branches: int) -> str:
+    return f"nb={nb},branches={branches}"
+
+
+def parse_schedule(name: str) -> tuple[int, int]:
+    fields = dict(kv.split("=") for kv in name.split(","))
+    return int(fields["nb"]), int(fields["branches"])
+
+
+# The autotuner's space: Cholesky block size x number of problem groups on streams of their own.
+SCHEDULE_CANDIDATES = tuple(schedule_name(nb, g) for nb in NB_CANDIDATES
+                            for g in BRANCH_CANDIDATES)
 
 
 @dataclasses.dataclass
@@ -66,18 +81,22 @@ def _structs(problems):
     return arr, offsets, off
 
 
-def tuning_key(problems) -> str:
+def tuning_key(problems, nb: int | None = None, branches: int | None = None) -> str:
+    """Key of a schedule decision; a fixed nb or branch count narrows the space, so it is part
+    of the key."""
     kinds = "+".join(sorted({p.kernel for p in problems}))
     return make_key("krr_batched", precision="fp64", kinds=kinds,
                     count=pow2_bucket(len(problems)),
-                    nmax=pow2_bucket(max(p.n for p in problems)))
+                    nmax=pow2_bucket(max(p.n for p in problems)),
+                    nb="tuned" if nb is None else nb,
+                    branches="tuned" if branches is None else branches)
 
 
 class BatchedFit:
     """All problems' fits as one plan; ``run`` replays the captured graph (one launch)."""
 
     def __init__(self, problems, Xd, d: int, y, nb: int | None = None, graph: bool = True,
-                 tuner: Autotuner | None = None, stream=None):
+                 tuner: Autotuner | None = None, stream=None, branches: int | None = None):
         import torch
 
         if not problems:
@@ -98,36 +117,40 @@ class BatchedFit:
         self.alpha = torch.zeros(max(total, 1), dtype=torch.float64, device=dev)
         self.info_d = torch.zeros(len(self.problems), dtype=torch.int32, device=dev)
         self._plan = None
-        if nb is None:
-            nb = (tuner or Autotuner()).tune(tuning_key(self.problems), NB_CANDIDATES,
-                                             lambda c: self._time_candidate(c))
-        self.nb = int(nb)
-        self._make_plan(self.nb)
+        if nb is None or branches is None:
+            cands = [c for c in SCHEDULE_CANDIDATES
+                     if (nb is None or parse_schedule(c)[0] == nb)
+                     and (branches is None or parse_schedule(c)[1] == branches)]
+            choice = (tuner or Autotuner()).tune(tuning_key(self.problems, nb, branches), cands,
+                                                 lambda c: self._time_candidate(*parse_schedule(c)))
+            nb, branches = parse_schedule(choice)
+        self.nb, self.branches = int(nb), int(branches)
+        self._make_plan(self.nb, self.branches)
         self.graph = False
         if graph:
             _lib.check(_lib.lib().qmlh_krr_batched_capture(self._plan, self.stream.cuda_stream))
             self.graph = True
 
-    def _make_plan(self, nb):
+    def _make_plan(self, nb, branches):
         import torch
 
         L = _lib.lib()
         self.close()
-        nbytes = L.qmlh_krr_batched_workspace(self._structs, len(self.problems), nb)
+        nbytes = L.qmlh_krr_batched_workspace(self._structs, len(self.problems), nb, branches)
         if nbytes == 0:
             _lib.check(-1)
         self.ws = torch.empty(max(1, (nbytes + 7) // 8), dtype=torch.float64, device=self.Xd.device)
         plan = ctypes.c_void_p()
         torch.cuda.synchronize(self.Xd.device)
         _lib.check(L.qmlh_krr_batched_create(
-            self._structs, len(self.problems), nb, _lib.ptr(self.Xd), self.Xd.shape[0],
+            self._structs, len(self.problems), nb, branches, _lib.ptr(self.Xd), self.Xd.shape[0],
             self.Xd.shape[1], self.d, _lib.ptr(self.norms), _lib.ptr(self.y), self.y.numel(),
             _lib.ptr(self.alpha), self.alpha.numel(), _lib.ptr(self.info_d), _lib.ptr(self.ws),
             self.ws.numel() * 8, ctypes.byref(plan)))
         self._plan = plan
 
-    def _time_candidate(self, nb) -> float:
-        self._make_plan(nb)
+    def _time_candidate(self, nb, branches) -> float:
+        self._make_plan(nb, branches)
         L = _lib.lib()
         h = self.stream.cuda_stream
         _lib.check(L.qmlh_krr_batched_run(self._plan, h))  # warm

@@ -62,13 +62,14 @@ def _check_solution(K, y, alpha, alpha_ref):
     assert fwd <= max(1e-12, 64 * len(y) * cond * 2.2e-16), (fwd, cond)
 
 
-@pytest.mark.parametrize("nb", [64, 128, 256])
-def test_batched_matches_oracle(nb):
+@pytest.mark.parametrize("nb,branches", [(64, 1), (128, 1), (256, 1), (64, 2), (128, 3),
+                                         (256, 4)])
+def test_batched_matches_oracle(nb, branches):
     from qml_amd.sweep import BatchedFit
 
     P = _pool()
     probs = _problems(PROBLEMS)
-    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=nb, graph=True).run()
+    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=nb, graph=True, branches=branches).run()
     assert np.all(fit.info() == 0)
     for p, a in zip(probs, fit.alphas()):
         K, y = _oracle_system(p, P["X"], P["y"])
@@ -84,7 +85,7 @@ def test_batched_factor_matches_lapack():
 
     P = _pool()
     probs = _problems(PROBLEMS)
-    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=128, graph=False).run()
+    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=128, graph=False, branches=2).run()
     for i in (3, 9, 11):
         p = probs[i]
         K, _ = _oracle_system(p, P["X"], P["y"])
@@ -101,9 +102,11 @@ def test_each_result_same_alone_and_in_batch():
 
     P = _pool()
     probs = _problems(PROBLEMS)
-    full = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=128, graph=True).run()
+    # the batch in four stream groups, each problem alone on one stream
+    full = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=128, graph=True, branches=4).run()
     for i in (0, 4, 8, 11):
-        alone = BatchedFit([probs[i]], P["Xd"], P["d"], P["y"], nb=128, graph=True).run()
+        alone = BatchedFit([probs[i]], P["Xd"], P["d"], P["y"], nb=128, graph=True,
+                           branches=1).run()
         a = alone.alphas()[0].cpu().numpy()
         b = full.alphas()[i].cpu().numpy()
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), i
@@ -118,7 +121,7 @@ def test_graph_replays_are_identical():
 
     P = _pool()
     probs = _problems(PROBLEMS[4:])
-    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=64, graph=True)
+    fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=64, graph=True, branches=2)
     fit.run_direct()
     ref = fit.alpha.clone()
     for _ in range(3):
@@ -139,12 +142,13 @@ def test_info_matches_lapack_and_isolates_failure():
     P = _pool()
     good = _problems(PROBLEMS[6:10])
     bad = SweepProblem(400, 2000.0, -0.5, "laplacian", 0)
-    fit = BatchedFit(good + [bad], P["Xd"], P["d"], P["y"], nb=128, graph=True).run()
+    fit = BatchedFit(good + [bad], P["Xd"], P["d"], P["y"], nb=128, graph=True,
+                     branches=2).run()
     K, _ = _oracle_system(bad, P["X"], P["y"])
     _, info_ref = lapack.dpotrf(K, lower=1)
     info = fit.info()
     assert info_ref > 0 and info[-1] == info_ref and np.all(info[:-1] == 0)
-    clean = BatchedFit(good, P["Xd"], P["d"], P["y"], nb=128, graph=True).run()
+    clean = BatchedFit(good, P["Xd"], P["d"], P["y"], nb=128, graph=True, branches=1).run()
     for a, b in zip(fit.alphas()[:-1], clean.alphas()):
         assert np.array_equal(a.cpu().numpy().view(np.uint64), b.cpu().numpy().view(np.uint64))
     fit.close()
@@ -157,13 +161,16 @@ def test_batched_vs_sequential_and_autotune(tmp_path):
     from qml_amd import _lib
     from qml_amd.autotune import Autotuner
     from qml_amd.kernels import pairwise_device
-    from qml_amd.sweep import BatchedFit, NB_CANDIDATES, learning_curve_grid, sequential_fit
+    from qml_amd.sweep import (BatchedFit, SCHEDULE_CANDIDATES, learning_curve_grid,
+                               schedule_name, sequential_fit)
 
     P = _pool()
     probs = learning_curve_grid(count=16, n_min=200, n_max=1700, seed=3)
     tuner = Autotuner(str(tmp_path / "tune.json"))
     fit = BatchedFit(probs, P["Xd"], P["d"], P["y"], nb=None, tuner=tuner).run()
-    assert fit.nb in NB_CANDIDATES and tuner.lookup(list(tuner._read())[0]) == fit.nb
+    choice = schedule_name(fit.nb, fit.branches)
+    assert choice in SCHEDULE_CANDIDATES and tuner.lookup(list(tuner._read())[0]) == choice
+    assert len(tuner._read()[list(tuner._read())[0]]["timings"]) == len(SCHEDULE_CANDIDATES)
     seq = sequential_fit(probs, P["Xd"], P["d"], P["y"])
     torch.cuda.synchronize()
     yd = torch.from_numpy(P["y"]).cuda()

@@ -24,18 +24,24 @@ def test_workspace_size_and_validation():
     L = _lib.load_library()
     good = _probs([(500, 0, _lib.PAIR_LAPLACIAN), (1200, 10, _lib.PAIR_GAUSSIAN)])
     for nb in (64, 128, 256):
-        ws = L.qmlh_krr_batched_workspace(good, 2, nb)
+        ws = L.qmlh_krr_batched_workspace(good, 2, nb, 1)
         # at least the two factors (ld rounded to 16) and the solve vectors
         assert ws >= 8 * (512 * 500 + 1200 * 1200)
-    assert L.qmlh_krr_batched_workspace(good, 2, 96) == 0  # nb must be 64/128/256
+        # two groups: the same factors, job tables split between two launch sequences
+        ws2 = L.qmlh_krr_batched_workspace(good, 2, nb, 2)
+        assert ws2 >= ws and ws2 - ws < 1 << 20
+    assert L.qmlh_krr_batched_workspace(good, 2, 96, 1) == 0  # nb must be 64/128/256
+    for branches in (0, 5):
+        assert L.qmlh_krr_batched_workspace(good, 2, 64, branches) == 0
+        assert b"branches" in L.qmlh_last_error()
     bad = _probs([(0, 0, _lib.PAIR_LAPLACIAN)])
-    assert L.qmlh_krr_batched_workspace(bad, 1, 64) == 0
+    assert L.qmlh_krr_batched_workspace(bad, 1, 64, 1) == 0
     overlap = _probs([(100, 0, _lib.PAIR_LAPLACIAN), (100, 0, _lib.PAIR_LAPLACIAN)])
     overlap[1].alpha_offset = 50
-    assert L.qmlh_krr_batched_workspace(overlap, 2, 64) == 0
+    assert L.qmlh_krr_batched_workspace(overlap, 2, 64, 1) == 0
     assert b"overlap" in L.qmlh_last_error()
     wrong_kind = _probs([(100, 0, _lib.PAIR_MANHATTAN)])
-    assert L.qmlh_krr_batched_workspace(wrong_kind, 1, 64) == 0
+    assert L.qmlh_krr_batched_workspace(wrong_kind, 1, 64, 1) == 0
 
 
 def test_create_rejects_out_of_range_problems():
@@ -44,17 +50,17 @@ def test_create_rejects_out_of_range_problems():
     plan = ctypes.c_void_p()
     fake = 1 << 20  # never dereferenced: validation fails first
     # pool of 350 rows: rows 100..399 run past it
-    rc = L.qmlh_krr_batched_create(probs, 1, 64, fake, 350, 280, 276, None, fake, 1000, fake,
+    rc = L.qmlh_krr_batched_create(probs, 1, 64, 1, fake, 350, 280, 276, None, fake, 1000, fake,
                                    1000, fake, fake, 1 << 30, ctypes.byref(plan))
     assert rc != 0 and b"pool" in L.qmlh_last_error() and not plan.value
-    rc = L.qmlh_krr_batched_create(probs, 1, 64, fake, 1000, 280, 276, None, fake, 200, fake,
+    rc = L.qmlh_krr_batched_create(probs, 1, 64, 1, fake, 1000, 280, 276, None, fake, 200, fake,
                                    1000, fake, fake, 1 << 30, ctypes.byref(plan))
     assert rc != 0 and b"labels" in L.qmlh_last_error()
-    rc = L.qmlh_krr_batched_create(probs, 1, 64, fake, 1000, 280, 276, None, fake, 1000, fake,
+    rc = L.qmlh_krr_batched_create(probs, 1, 64, 1, fake, 1000, 280, 276, None, fake, 1000, fake,
                                    1000, fake, fake, 16, ctypes.byref(plan))
     assert rc != 0 and b"workspace" in L.qmlh_last_error()
     g = _probs([(300, 0, _lib.PAIR_GAUSSIAN)])
-    rc = L.qmlh_krr_batched_create(g, 1, 64, fake, 1000, 280, 276, None, fake, 1000, fake,
+    rc = L.qmlh_krr_batched_create(g, 1, 64, 1, fake, 1000, 280, 276, None, fake, 1000, fake,
                                    1000, fake, fake, 1 << 30, ctypes.byref(plan))
     assert rc != 0 and b"norms" in L.qmlh_last_error()
 
