@@ -582,50 +582,67 @@ def run_invert(n, cpu_n, log):
                                "sample": f"LAPACK dpotrf + dpotri (scipy) at n={cpu_n}: "
                                          f"{t1 - t0:.2f}s on {cores} threads; extrapolated as n^3"}
     log(f"cho_invert n={n}: {ms:.1f} ms ({out['tflops_lapack_equiv']:.1f} TF/s LAPACK-equivalent)")
-    # Bunch-Kaufman (bkf_solve / bkf_invert): stepped factorisation, per-column solves, on a
-    # symmetric indefinite matrix of order 2000, against scipy's dsytrf + dsytrs at the same n.
-    from qml_amd.math import bkf_factor_device, bkf_solve_device
-
-    nb = 2000
-    rng = np.random.default_rng(1)
-    M = rng.standard_normal((nb, nb))
-    Ah = M + M.T
-    A0 = torch.from_numpy(np.ascontiguousarray(Ah.T)).cuda()
-    Ab = A0.clone()
-    bkf_factor_device(Ab, nb)
-    torch.cuda.synchronize()
-    Ab.copy_(A0)
-    e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
-    e0.record()
-    ipiv, _ = bkf_factor_device(Ab, nb)
-    e1.record()
-    yb = torch.from_numpy(rng.standard_normal(nb)).cuda()
-    e2.record()
-    bkf_solve_device(Ab, nb, ipiv, yb, 1)
-    Xb = torch.eye(nb, dtype=torch.float64, device=Ab.device)
-    e3.record()
-    bkf_solve_device(Ab, nb, ipiv, Xb, nb)
-    e4 = torch.cuda.Event(enable_timing=True)
-    e4.record()
-    e4.synchronize()
-    bk = {"n": nb, "factor_ms": e0.elapsed_time(e1), "solve_ms": e2.elapsed_time(e3),
-          "inverse_solves_ms": e3.elapsed_time(e4),
-          "factor_gflops": nb ** 3 / 3.0 / (e0.elapsed_time(e1) * 1e-3) / 1e9,
-          "note": "a step per column: pivot search and interchange in one workgroup, the "
-                  "trailing rank-1 / rank-2 update over many"}
+    # Bunch-Kaufman (bkf_solve / bkf_invert) on symmetric indefinite matrices: the factorisation
+    # at orders 2000, 4000 and 8000 (64-column panels and an MFMA trailing update from order
+    # 1024) against scipy's dsytrf at the same n (8000: extrapolated from 4000 as n^3); the
+    # solve and the inverse's n solves at 2000.
     from scipy.linalg import lapack
 
-    t0 = time.perf_counter()
-    lu, piv, _ = lapack.dsytrf(np.asfortranarray(Ah), lower=1)[:3]
-    t1 = time.perf_counter()
-    lapack.dsytrs(lu, piv, np.eye(nb), lower=1)
-    t2 = time.perf_counter()
-    bk["cpu_lapack_ms"] = {"dsytrf": (t1 - t0) * 1e3, "dsytrs_identity": (t2 - t1) * 1e3,
-                           "cores": int(os.environ.get("OMP_NUM_THREADS",
-                                                       len(os.sched_getaffinity(0))))}
+    from qml_amd.math import bkf_factor_device, bkf_solve_device
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    bk = {"factor": [], "cpu_lapack_cores": cores}
+    rng = np.random.default_rng(1)
+    cpu_ms = {}
+    for nb in (2000, 4000, 8000):
+        M = rng.standard_normal((nb, nb))
+        Ah = M + M.T
+        del M
+        A0 = torch.from_numpy(np.ascontiguousarray(Ah.T)).cuda()
+        Ab = A0.clone()
+        bkf_factor_device(Ab, nb)  # warm
+        torch.cuda.synchronize()
+        Ab.copy_(A0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ipiv, info = bkf_factor_device(Ab, nb)
+        e1.record()
+        e1.synchronize()
+        fms = e0.elapsed_time(e1)
+        if nb <= 4000:
+            t0 = time.perf_counter()
+            lu, piv, _ = lapack.dsytrf(np.asfortranarray(Ah), lower=1)[:3]
+            cpu_ms[nb] = (time.perf_counter() - t0) * 1e3
+            same = bool(np.array_equal(ipiv.cpu().numpy()[:nb], piv))
+        else:
+            cpu_ms[nb] = cpu_ms[4000] * (nb / 4000) ** 3
+            same = None
+        bk["factor"].append({"n": nb, "factor_ms": fms,
+                             "factor_gflops": nb ** 3 / 3.0 / (fms * 1e-3) / 1e9,
+                             "info": int(info.item()),
+                             "cpu_lapack_dsytrf_ms": cpu_ms[nb],
+                             "cpu_extrapolated": nb > 4000,
+                             "pivots_equal_lapack": same})
+        log(f"bunch-kaufman n={nb}: factor {fms:.1f} ms (LAPACK {cpu_ms[nb]:.1f} ms)")
+        if nb == 2000:
+            yb = torch.from_numpy(rng.standard_normal(nb)).cuda()
+            Xb = torch.eye(nb, dtype=torch.float64, device=Ab.device)
+            e2, e3, e4 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e2.record()
+            bkf_solve_device(Ab, nb, ipiv, yb, 1)
+            e3.record()
+            bkf_solve_device(Ab, nb, ipiv, Xb, nb)
+            e4.record()
+            e4.synchronize()
+            t1 = time.perf_counter()
+            lapack.dsytrs(lu, piv, np.eye(nb), lower=1)
+            bk["solve_2000"] = {"solve_ms": e2.elapsed_time(e3),
+                                "inverse_solves_ms": e3.elapsed_time(e4),
+                                "cpu_lapack_dsytrs_identity_ms":
+                                    (time.perf_counter() - t1) * 1e3}
+            del Xb
+        del Ab, A0, Ah
     out["bunch_kaufman"] = bk
-    log(f"bunch-kaufman n={nb}: factor {bk['factor_ms']:.1f} ms (LAPACK "
-        f"{bk['cpu_lapack_ms']['dsytrf']:.1f} ms), inverse solves {bk['inverse_solves_ms']:.1f} ms")
     return out
 
 

@@ -491,6 +491,17 @@ def bkf_factor(A):
     return np.tril(lu), piv, info
 
 
+def bkf_factor_blocked(A):
+    """LAPACK dsytrf('L') of A (a copy) with its optimal workspace, as the reference calls it
+    (qml/fcho_solve.f90:90-99: lwork = n nb with nb = ilaenv(1, 'DSYTRF', ...)): LAPACK's
+    blocked Bunch-Kaufman (dlasyf panels of nb columns, unblocked dsytf2 for the last ones).
+    Returns (lower factors, 1-based pivots, info)."""
+    from scipy.linalg import lapack
+
+    lu, piv, info = lapack.dsytrf(np.asfortranarray(A, dtype=np.float64), lower=1)[:3]
+    return np.tril(lu), piv, info
+
+
 def krr_fit_predict(X, y, Xs, kernel, sigma, lam):
     """KRR composition of tests/test_energy_krr_cmat.py:101-111: returns (alpha, y_pred)."""
     kern = laplacian_kernel if kernel == "laplacian" else gaussian_kernel

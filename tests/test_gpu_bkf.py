@@ -1,10 +1,10 @@
 """Bunch-Kaufman on the GPU (qml/math.py:100-164) against LAPACK dsytrf / dsytrs (the oracle).
 
 Bars: the reference tests' own checks on its fixture (tests/test_math.py:73-110); the same
-pivots as LAPACK's unblocked factorisation (orders below its 64-column block) and factors within
-1e-10; solutions and inverses within 1e-8 relative of LAPACK's on well-conditioned indefinite
-matrices (1e-5 at the fixture's condition number 4.4e9); singular matrices report LAPACK's
-info."""
+pivots as LAPACK's unblocked factorisation below order 256 and as its blocked one (the
+reference's call) from 256 on, factors within 1e-9; solutions and inverses within 1e-8
+relative of LAPACK's on well-conditioned indefinite matrices (1e-5 at the fixture's condition
+number 4.4e9); singular matrices report LAPACK's info."""
 
 import numpy as np
 import pytest
@@ -35,9 +35,9 @@ def _factor(A):
     return np.tril(Ad.cpu().numpy().T), ipiv.cpu().numpy()[:n], int(info.item())
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 40, 63, 255, 256, 300, 411])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 40, 63, 255])
 def test_bkf_factor_matches_lapack_pivots(n):
-    # Orders below 256 run in one workgroup, from 256 on as one step of launches per column.
+    # Orders below 256 run unblocked in one workgroup, as LAPACK's dsytf2.
     rng = np.random.default_rng(n)
     for trial in range(6 if n < 100 else 2):
         M = rng.standard_normal((n, n))
@@ -48,6 +48,35 @@ def test_bkf_factor_matches_lapack_pivots(n):
         Lr, pivr, infor = oracle.bkf_factor(A)
         assert np.array_equal(piv, pivr) and info == infor
         assert np.max(np.abs(L - Lr)) <= 1e-9 * max(1.0, np.max(np.abs(Lr)))
+
+
+@pytest.mark.parametrize("n", [256, 300, 411, 1024, 1025, 1090, 1500, 2000])
+def test_bkf_blocked_matches_lapack_pivots(n):
+    # From order 256 the factorisation runs in 64-column panels (dlasyf), as the reference's
+    # dsytrf call does (lwork = n nb): the same pivots as LAPACK's blocked code, factors within
+    # 1e-9 of the largest entry.
+    rng = np.random.default_rng(7 + n)
+    for trial in range(2):
+        M = rng.standard_normal((n, n))
+        A = M + M.T
+        if trial % 2:
+            A[np.diag_indices(n)] *= 0.01
+        L, piv, info = _factor(A)
+        Lr, pivr, infor = oracle.bkf_factor_blocked(A)
+        assert np.array_equal(piv, pivr) and info == infor == 0
+        assert np.max(np.abs(L - Lr)) <= 1e-9 * max(1.0, np.max(np.abs(Lr)))
+
+
+@pytest.mark.parametrize("zero_at", [5, 700, 1070])
+def test_bkf_blocked_singular_reports_info(zero_at):
+    # a zero pivot in the first panel, a later panel, and the unblocked tail
+    n = 1100
+    A = np.diag(np.arange(1.0, n + 1.0))
+    A[zero_at, zero_at] = 0.0
+    L, piv, info = _factor(A)
+    Lr, pivr, infor = oracle.bkf_factor_blocked(A)
+    assert info == infor == zero_at + 1
+    assert np.array_equal(piv, pivr) and np.array_equal(L, Lr)
 
 
 @pytest.mark.parametrize("n", [4, 300])
@@ -75,7 +104,7 @@ def test_bkf_reference_fixture(cho_pair):
     assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) <= 1e-5
 
 
-@pytest.mark.parametrize("n", [5, 64, 65, 300, 1000])
+@pytest.mark.parametrize("n", [5, 64, 65, 300, 1000, 1200])
 def test_bkf_random_indefinite(n):
     from qml_amd.math import bkf_invert, bkf_solve
 

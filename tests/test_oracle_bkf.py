@@ -42,3 +42,25 @@ def test_bkf_random_indefinite_matches_numpy():
                                    rtol=1e-9, atol=1e-10)
         np.testing.assert_allclose(oracle.bkf_invert(A.copy()), np.linalg.inv(A), rtol=1e-8,
                                    atol=1e-9)
+
+
+def test_bkf_blocked_and_unblocked_factors_agree():
+    # LAPACK's blocked dsytrf (the reference's call) and its unblocked dsytf2 are one algorithm:
+    # the same pivots and, to rounding, the same factors; reconstructing P L D L^T P^T gives A.
+    rng = np.random.default_rng(11)
+    for n in (65, 130, 300):
+        for small_diag in (False, True):
+            M = rng.standard_normal((n, n))
+            A = M + M.T
+            if small_diag:
+                A[np.diag_indices(n)] *= 0.01
+            Lb, pb, ib = oracle.bkf_factor_blocked(A)
+            Lu, pu, iu = oracle.bkf_factor(A)
+            assert np.array_equal(pb, pu) and ib == iu == 0
+            assert np.max(np.abs(Lb - Lu)) <= 1e-10 * max(1.0, np.max(np.abs(Lu)))
+            assert np.any(pb < 0)  # 2 x 2 pivots occur
+    # a zero pivot inside the first panel and one in the unblocked tail
+    for zero_at in (40, 180):
+        A = np.diag(np.arange(1.0, 201.0))
+        A[zero_at, zero_at] = 0.0
+        assert oracle.bkf_factor_blocked(A)[2] == zero_at + 1
