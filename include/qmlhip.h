@@ -311,8 +311,10 @@ int qmlh_trsm_f32l(const float *L, int64_t n, int64_t ldl, double *V, int64_t ld
 /* The hybrid Woodbury term: Y (n x r, column-major FP64) gets columns k .. r-1 widened from
  * the factorisation's FP32 solve Y32 = L^-1 V (qmlh_spotrf_lowrank), and columns 0 .. k-1
  * solved again in FP64 arithmetic with the FP32 factor L from the FP64 right-hand sides Y holds
- * there on entry (qmlh_gaussian_lowrank's leading columns; u first). One single-vector solve
- * per FP64 column. ws: qmlh_lowrank_hybrid_workspace(n) bytes. */
+ * there on entry (qmlh_gaussian_lowrank's leading columns; u first). One single-launch solve
+ * per FP64 column; from order 1024 in 128-row blocks whose diagonal step is a product with the
+ * FP64 inverse of the block (computed once per call, as qmlh_pcg_solve's preconditioner does).
+ * ws: qmlh_lowrank_hybrid_workspace(n) bytes. */
 size_t qmlh_lowrank_hybrid_workspace(int64_t n);
 int qmlh_lowrank_hybrid(const float *L, int64_t n, int64_t ldl, const float *Y32, int64_t ldy32,
                         double *Y, int64_t ldy, int64_t r, int64_t k, void *ws, size_t ws_bytes,

@@ -114,7 +114,6 @@ __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restric
             const int r = idx & (SY_B - 1), c = idx / SY_B;  // consecutive lanes: consecutive rows
             const bool ok = row0 + r < n && col0 + c < n && (!diag || r >= c);
             v[k] = ok ? A[(row0 + r) + (col0 + c) * lda] : 0.0;
-            if (ABS) v[k] = fabs(v[k]);
         }
         __syncthreads();  // previous tile's readers are done
 #pragma unroll
@@ -127,13 +126,18 @@ __global__ __launch_bounds__(256) void symv_lower_kernel(const double* __restric
         // row part: lane = row, columns 16 q ..
         double rs = 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) rs = fma(tile[lane][16 * q + c], xj[16 * q + c], rs);
+        for (int c = 0; c < 16; ++c) {
+            // |a| where the tile is read: next to the load the compiler would move it into the
+            // load's branch and wait for each load in turn
+            const double a = ABS ? fabs(tile[lane][16 * q + c]) : tile[lane][16 * q + c];
+            rs = fma(a, xj[16 * q + c], rs);
+        }
         redr[q][lane] = rs;
         // column part: lane = column, rows 16 q .. (strictly below the diagonal on the diagonal tile)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int rr = 16 * q + r;
-            const double a = (diag && rr <= lane) ? 0.0 : tile[rr][lane];
+            const double a = (diag && rr <= lane) ? 0.0 : (ABS ? fabs(tile[rr][lane]) : tile[rr][lane]);
             csum = fma(a, xr[rr], csum);
         }
         __syncthreads();

@@ -944,9 +944,14 @@ __global__ __launch_bounds__(256) void widen_columns_kernel(const float* __restr
 }
 }  // namespace qmlh
 
+// Workspace: the right-hand side copy (n doubles), the FP64 inverses of the factor's 64- and
+// 128-row diagonal blocks (from order TRSV_SYNCFREE_MIN, for the 128-row single-launch solve),
+// the solve's sync words.
 extern "C" size_t qmlh_lowrank_hybrid_workspace(int64_t n) {
+    using namespace qmlh;
     if (n <= 0) return 0;
-    return (size_t)n * sizeof(double) + qmlh::trsv_sync_ints(n) * sizeof(int32_t);
+    const size_t inv = n >= TRSV_SYNCFREE_MIN ? trsv_dinv_doubles(n) + trsv_dinv128_doubles(n) : 0;
+    return ((size_t)n + inv) * sizeof(double) + trsv_sync_ints(n) * sizeof(int32_t);
 }
 
 extern "C" int qmlh_lowrank_hybrid(const float* L, int64_t n, int64_t ldl, const float* Y32,
@@ -969,13 +974,23 @@ extern "C" int qmlh_lowrank_hybrid(const float* L, int64_t n, int64_t ldl, const
                            r);
         QMLH_CHECK_LAUNCH("lowrank_hybrid widen launch");
     }
+    if (k == 0) return QMLH_OK;
     double* tmp = static_cast<double*>(ws);
-    int32_t* sync = reinterpret_cast<int32_t*>(tmp + n);
+    const bool big = n >= TRSV_SYNCFREE_MIN;
+    double* dinv = tmp + n;
+    double* dinv128 = dinv + (big ? trsv_dinv_doubles(n) : 0);
+    int32_t* sync = reinterpret_cast<int32_t*>(dinv128 + (big ? trsv_dinv128_doubles(n) : 0));
+    // Large orders: the single-launch solve in 128-row blocks, each block's diagonal step one
+    // product with the FP64 inverse of its diagonal block (the preconditioner's own L^-1).
+    if (big) QMLH_TRY(trsv_inverse128<float>(L, n, ldl, dinv, dinv128, s));
     for (int64_t j = 0; j < k; ++j) {
         QMLH_TRY(check_hip(hipMemcpyAsync(tmp, Y + j * ldy, (size_t)n * sizeof(double),
                                           hipMemcpyDeviceToDevice, s),
                            "lowrank_hybrid: copy"));
-        QMLH_TRY(trsv_lower_fwd<float>(L, n, ldl, tmp, Y + j * ldy, sync, s));
+        if (big)
+            QMLH_TRY(trsv_syncfree128<float>(L, n, ldl, tmp, Y + j * ldy, sync, true, s, dinv128));
+        else
+            QMLH_TRY(trsv_lower_fwd<float>(L, n, ldl, tmp, Y + j * ldy, sync, s));
     }
     return QMLH_OK;
 }
