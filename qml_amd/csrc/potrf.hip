@@ -848,9 +848,13 @@ extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, i
     return QMLH_OK;
 }
 
+// Workspace: an n-vector, from order TRSV_SYNCFREE_MIN the FP64 inverses of the factor's 64-
+// and 128-row diagonal blocks, the solves' sync words.
 extern "C" size_t qmlh_potrs_workspace(int64_t n) {
+    using namespace qmlh;
     if (n <= 0) return 0;
-    return (size_t)n * sizeof(double) + qmlh::trsv_sync_ints(n) * sizeof(int32_t);
+    const size_t inv = n >= TRSV_SYNCFREE_MIN ? trsv_dinv_doubles(n) + trsv_dinv128_doubles(n) : 0;
+    return ((size_t)n + inv) * sizeof(double) + trsv_sync_ints(n) * sizeof(int32_t);
 }
 
 extern "C" int qmlh_potrs(const double* L, int64_t n, int64_t lda, double* x, void* ws,
@@ -858,9 +862,23 @@ extern "C" int qmlh_potrs(const double* L, int64_t n, int64_t lda, double* x, vo
     using namespace qmlh;
     if (n > 0 && ws_bytes < qmlh_potrs_workspace(n))
         return set_error(QMLH_ERR_WORKSPACE, "potrs: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
     double* tmp = static_cast<double*>(ws);
+    if (n >= TRSV_SYNCFREE_MIN) {
+        if (lda < n) return set_error(QMLH_ERR_ARG, "potrs: lda < n");
+        if (!L || !x) return set_error(QMLH_ERR_ARG, "potrs: null pointer");
+        // Both solves in one launch each, in 128-row blocks whose diagonal step is a product with
+        // the FP64 inverse of the block (half the dependency chain of 64-row substitution).
+        double* dinv = tmp + n;
+        double* dinv128 = dinv + trsv_dinv_doubles(n);
+        int32_t* sync = reinterpret_cast<int32_t*>(dinv128 + trsv_dinv128_doubles(n));
+        TimerScope ts(QMLH_TIMER_POTRS, 2.0 * (double)n * (double)n, s);
+        QMLH_TRY(trsv_inverse128<double>(L, n, lda, dinv, dinv128, s));
+        QMLH_TRY(trsv_syncfree128<double>(L, n, lda, x, tmp, sync, true, s, dinv128));
+        return trsv_syncfree128<double>(L, n, lda, tmp, x, sync, false, s, dinv128);
+    }
     int32_t* sync = n > 0 ? reinterpret_cast<int32_t*>(tmp + n) : nullptr;
-    return potrs_impl<double>(L, n, lda, x, tmp, sync, (hipStream_t)stream);
+    return potrs_impl<double>(L, n, lda, x, tmp, sync, s);
 }
 
 extern "C" size_t qmlh_cho_invert_workspace(int64_t n) {
