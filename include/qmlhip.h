@@ -369,7 +369,7 @@ int qmlh_krr_predict_noalloc(int32_t kind, const double *Xq, int64_t m, int64_t 
  * rows, ldx >= d), reads its labels y[y_offset .. y_offset + n) and writes alpha[alpha_offset ..
  * alpha_offset + n) (alpha ranges must not overlap); info[p] follows LAPACK (0, or the first
  * failing column + 1). norms (squared row norms of X, qmlh_row_sqnorms) are needed when any
- * problem is Gaussian. nb (64, 128 or 256) is the Cholesky block size. branches (1 .. 4): the
+ * problem is Gaussian. nb (64, 128, 256 or 512) is the Cholesky block size. branches (1 .. 4): the
  * problems are split into that many groups of about equal n^3 work whose factorisations step
  * on streams of their own (the plan creates branches - 1 streams at the default priority and
  * forks from / joins back to the caller's stream), so small launches of one group overlap the

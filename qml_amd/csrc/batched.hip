@@ -10,7 +10,7 @@
 //            K_p(i, j), i >= j, plus lambda_p on the diagonal (FP64 MFMA cross term for the
 //            Gaussian, FP64 VALU L1 tile for the Laplacian);
 //   factor   right-looking blocked Cholesky, all problems in lock step over column blocks of
-//            width nb (64, 128 or 256; tuned). Per 64-wide sub-panel one launch factors the
+//            width nb (64, 128, 256 or 512; tuned). Per 64-wide sub-panel one launch factors the
 //            diagonal blocks (one workgroup per problem, FP64 in one wavefront's registers),
 //            one solves the panel rows below them (256 rows per workgroup), and an MFMA launch
 //            updates the rest of the nb-wide panel; per block column one MFMA launch updates
@@ -667,8 +667,8 @@ static int validate(const qmlh_problem* probs, int32_t nprob, int32_t nb, int32_
     using namespace qmlh;
     if (nprob < 0) return set_error(QMLH_ERR_ARG, "batched: nprob < 0");
     if (nprob > 0 && !probs) return set_error(QMLH_ERR_ARG, "batched: problems are null");
-    if (nb != 64 && nb != 128 && nb != 256)
-        return set_error(QMLH_ERR_ARG, "batched: nb must be 64, 128 or 256 (got %d)", nb);
+    if (nb != 64 && nb != 128 && nb != 256 && nb != 512)
+        return set_error(QMLH_ERR_ARG, "batched: nb must be 64, 128, 256 or 512 (got %d)", nb);
     if (branches < 1 || branches > MAX_BRANCHES)
         return set_error(QMLH_ERR_ARG, "batched: branches must be 1 .. %d (got %d)", MAX_BRANCHES,
                          branches);

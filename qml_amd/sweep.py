@@ -22,7 +22,7 @@ from .kernels import (check_expansion, device_max, gaussian_mfma_device, pairwis
 from .math import add_diagonal_device, potrf_device, potrs_device
 
 KINDS = {"gaussian": _lib.PAIR_GAUSSIAN, "laplacian": _lib.PAIR_LAPLACIAN}
-NB_CANDIDATES = (64, 128, 256)
+NB_CANDIDATES = (64, 128, 256, 512)
 BRANCH_CANDIDATES = (1, 2, 4)
 
 

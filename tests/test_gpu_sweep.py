@@ -62,8 +62,8 @@ def _check_solution(K, y, alpha, alpha_ref):
     assert fwd <= max(1e-12, 64 * len(y) * cond * 2.2e-16), (fwd, cond)
 
 
-@pytest.mark.parametrize("nb,branches", [(64, 1), (128, 1), (256, 1), (64, 2), (128, 3),
-                                         (256, 4)])
+@pytest.mark.parametrize("nb,branches", [(64, 1), (128, 1), (256, 1), (512, 1), (64, 2),
+                                         (128, 3), (256, 4), (512, 2)])
 def test_batched_matches_oracle(nb, branches):
     from qml_amd.sweep import BatchedFit
 

@@ -23,14 +23,14 @@ def _probs(spec):
 def test_workspace_size_and_validation():
     L = _lib.load_library()
     good = _probs([(500, 0, _lib.PAIR_LAPLACIAN), (1200, 10, _lib.PAIR_GAUSSIAN)])
-    for nb in (64, 128, 256):
+    for nb in (64, 128, 256, 512):
         ws = L.qmlh_krr_batched_workspace(good, 2, nb, 1)
         # at least the two factors (ld rounded to 16) and the solve vectors
         assert ws >= 8 * (512 * 500 + 1200 * 1200)
         # two groups: the same factors, job tables split between two launch sequences
         ws2 = L.qmlh_krr_batched_workspace(good, 2, nb, 2)
         assert ws2 >= ws and ws2 - ws < 1 << 20
-    assert L.qmlh_krr_batched_workspace(good, 2, 96, 1) == 0  # nb must be 64/128/256
+    assert L.qmlh_krr_batched_workspace(good, 2, 96, 1) == 0  # nb must be 64/128/256/512
     for branches in (0, 5):
         assert L.qmlh_krr_batched_workspace(good, 2, 64, branches) == 0
         assert b"branches" in L.qmlh_last_error()
