@@ -126,6 +126,8 @@ def parse():
     p.add_argument("--refit-every", type=float, default=0.5,
                    help="seconds between refit requests (N ~ U[1000, 16000]) under load "
                         "(0 = none)")
+    p.add_argument("--lowrank-cols", type=int, default=1,
+                   help="leading columns of the Woodbury term solved in FP64 (hybrid form)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="run a device copy of this many MiB first (HBM counter calibration)")
     p.add_argument("--kernels", type=int, default=1,
@@ -788,7 +790,8 @@ def main():
     fitmx_s, timers_mx, rep_mx = None, None, None
     if args.precision in ("mixed", "both"):
         modelmx = KRR(args.kernel, args.sigma, args.lam, "mixed", tol=args.tol,
-                      factor=None if args.factor == "auto" else args.factor)
+                      factor=None if args.factor == "auto" else args.factor,
+                      lowrank_cols=args.lowrank_cols)
         fitmx_s, timers_mx, rep_mx = run_fits(modelmx, "mixed")
         predmx = modelmx.predict_molecules(Zq, Rq, offq, size)
         std = float(np.std(tr.labels))
