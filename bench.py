@@ -296,6 +296,32 @@ def run_kernels(log):
                               "tflops": flops / (ms * 1e-3) / 1e12,
                               "frac_of_fp64_mfma": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]}
     del lin, K2
+    # Fused Laplacian prediction (qmlh_krr_predict): 4096 query molecules against 100,000
+    # training rows and alpha, no kernel matrix written.
+    nq = 4096
+    alpha = torch.ones(100000, dtype=torch.float64, device=X.device)
+    yq = torch.empty(nq, dtype=torch.float64, device=X.device)
+    pws = torch.empty(max(1, L.qmlh_predict_workspace(nq, 100000) // 8), dtype=torch.float64,
+                      device=X.device)
+
+    def predict_laplace():
+        _lib.check(L.qmlh_krr_predict(_lib.PAIR_LAPLACIAN, _lib.ptr(X), nq, X.shape[1], None,
+                                      _lib.ptr(X), 100000, X.shape[1], None, _lib.ptr(alpha), d,
+                                      724.0, _lib.ptr(yq), _lib.ptr(pws),
+                                      pws.numel() * 8, _lib.stream_handle(None)))
+
+    predict_laplace()
+    e0.record()
+    for _ in range(reps):
+        predict_laplace()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    pf = nq * 100000.0 * d
+    out["predict_laplace"] = {"shape": f"{nq} queries x 100000 training, d={d}", "ms": ms,
+                              "pair_features_per_s": pf / (ms * 1e-3),
+                              "frac_of_fp64_valu": pf / (ms * 1e-3) / 19.65e12}
+    del alpha, yq, pws
     # Parameterised VALU kernels on a row block against the 100k set (the Laplacian's shape).
     K = torch.empty((100000, rows), dtype=torch.float64, device=X.device)
     for name, kind, params, nrows in (

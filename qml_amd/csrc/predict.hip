@@ -9,6 +9,7 @@
 #include <mutex>
 
 #include "mfma_core.h"
+#include "pair_tile.h"
 #include "timer.h"
 
 namespace qmlh {
@@ -17,8 +18,6 @@ static constexpr int PG_Q = 64;     // queries per Gaussian workgroup
 static constexpr int PG_T = 256;    // training rows per Gaussian workgroup (4 waves x 64)
 static constexpr int PL_Q = 128;    // queries per Laplacian workgroup
 static constexpr int PL_T = 128;    // training rows per Laplacian workgroup
-static constexpr int PL_BK = 16;
-static constexpr int PL_LDS = 130;
 
 // Gaussian: MFMA cross term, FP64 epilogue, per-chunk partials part[chunk * m + q].
 __global__ __launch_bounds__(256) void predict_gauss_kernel(
@@ -79,13 +78,16 @@ __global__ __launch_bounds__(256) void predict_gauss_kernel(
     }
 }
 
-// Laplacian: VALU L1 tile (as the pairwise kernel), exp * alpha, per-chunk partials.
+// Laplacian: the pairwise kernel's VALU L1 tile (pair_tile_accumulate: LDS-staged chunks, the
+// next chunk's loads in flight during the current chunk's sums), exp * alpha, per-chunk
+// partials.
 __global__ __launch_bounds__(256, 2) void predict_laplace_kernel(
     const double* __restrict__ Q, int64_t ldq, int64_t m, const double* __restrict__ T,
     int64_t ldt, int64_t n, const double* __restrict__ alpha, int64_t d, double scale,
     double* __restrict__ part) {
-    __shared__ __attribute__((aligned(16))) double sA[PL_BK][PL_LDS];
-    __shared__ __attribute__((aligned(16))) double sB[PL_BK][PL_LDS];
+    static_assert(PL_Q == PT_BM && PL_T == PT_BN, "the predict tile is the pairwise tile");
+    __shared__ __attribute__((aligned(16))) double sA[PT_BK][PT_LDS];
+    __shared__ __attribute__((aligned(16))) double sB[PT_BK][PT_LDS];
     __shared__ double red[16][PL_Q];
     const int64_t row0 = (int64_t)blockIdx.x * PL_Q;  // queries
     const int64_t chunk = blockIdx.y;
@@ -93,39 +95,7 @@ __global__ __launch_bounds__(256, 2) void predict_laplace_kernel(
     const int t = threadIdx.x;
     const int tx = t % 16, ty = t / 16;
     double acc[8][8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
-    for (int64_t k0 = 0; k0 < d; k0 += PL_BK) {
-        const int k = t % PL_BK;
-        const int rb = t / PL_BK;
-#pragma unroll
-        for (int qd = 0; qd < 8; ++qd) {
-            const int r = rb + 16 * qd;
-            const int64_t kk = k0 + k;
-            const int64_t ra = row0 + r, rt = col0 + r;
-            sA[k][r] = (ra < m && kk < d) ? Q[ra * ldq + kk] : 0.0;
-            sB[k][r] = (rt < n && kk < d) ? T[rt * ldt + kk] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kq = 0; kq < PL_BK; ++kq) {
-            double a[8], b[8];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double2 va = *reinterpret_cast<const double2*>(&sA[kq][2 * tx + 32 * r]);
-                const double2 vb = *reinterpret_cast<const double2*>(&sB[kq][2 * ty + 32 * r]);
-                a[2 * r] = va.x; a[2 * r + 1] = va.y;
-                b[2 * r] = vb.x; b[2 * r + 1] = vb.y;
-            }
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[r][c] = acc[r][c] + fabs(a[r] - b[c]);
-        }
-        __syncthreads();
-    }
+    pair_tile_accumulate<OP_L1>(acc, sA, sB, Q, ldq, m, T, ldt, n, d, row0, col0);
     // Row sums over this lane's 8 training columns, in column order.
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
