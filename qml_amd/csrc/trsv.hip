@@ -472,7 +472,22 @@ __global__ __launch_bounds__(512) void trsv128_kernel(const T* __restrict__ L, i
     bool ok = true;
     // dependencies: the 64-blocks before this block (FWD) or after it (BWD), in solve order
     const int64_t nj = FWD ? 2 * B : nblocks - 2 * B - 2 > 0 ? nblocks - 2 * B - 2 : 0;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 4 && !FWD) {
+        // The backward solve keeps 16 partial sums per lane: three stages fit its registers
+        // (four spill).
+        TsStage<T> s0, s1, s2;
+        ts_issue<T, FWD>(s0, L, ldl, n, rh, wh, nblocks, 0, nj, out, lane, grp);
+        ts_issue<T, FWD>(s1, L, ldl, n, rh, wh, nblocks, 1, nj, out, lane, grp);
+        ts_issue<T, FWD>(s2, L, ldl, n, rh, wh, nblocks, 2, nj, out, lane, grp);
+        for (int64_t q = 0; q < nj; q += 3) {
+            ts_consume<T, FWD>(s0, n, nblocks, q, nj, out, lane, grp_s, err, ok, acc1, acc16);
+            ts_issue<T, FWD>(s0, L, ldl, n, rh, wh, nblocks, q + 3, nj, out, lane, grp);
+            ts_consume<T, FWD>(s1, n, nblocks, q + 1, nj, out, lane, grp_s, err, ok, acc1, acc16);
+            ts_issue<T, FWD>(s1, L, ldl, n, rh, wh, nblocks, q + 4, nj, out, lane, grp);
+            ts_consume<T, FWD>(s2, n, nblocks, q + 2, nj, out, lane, grp_s, err, ok, acc1, acc16);
+            ts_issue<T, FWD>(s2, L, ldl, n, rh, wh, nblocks, q + 5, nj, out, lane, grp);
+        }
+    } else if constexpr (sizeof(T) == 8) {
         // FP64 tiles are twice the bytes: two stages keep as many bytes in flight as four FP32
         // stages, in registers the workgroup's 512 lanes have.
         TsStage<T> s0, s1;
