@@ -67,6 +67,32 @@ def test_bkf_blocked_matches_lapack_pivots(n):
         assert np.max(np.abs(L - Lr)) <= 1e-9 * max(1.0, np.max(np.abs(Lr)))
 
 
+@pytest.mark.parametrize("n,lda", [(300, 307), (1100, 1152)])
+def test_bkf_blocked_leading_dimension(n, lda):
+    # The C ABI takes lda >= n: the factor of a matrix embedded in a wider column-major array
+    # equals the factor of the same matrix stored densely, and the padding rows stay untouched.
+    from qml_amd import _lib
+
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    L0, piv0, info0 = _factor(A)
+    big = np.full((n, lda), 7.0)  # row c of this (n, lda) view is column c, rows 0 .. lda-1
+    big[:, :n] = A.T
+    Ad = torch.from_numpy(big).cuda()
+    Ld = _lib.lib()
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = torch.empty((Ld.qmlh_bkf_workspace(n) + 7) // 8, dtype=torch.float64, device="cuda")
+    _lib.check(Ld.qmlh_bkf_factor(_lib.ptr(Ad), n, lda, _lib.ptr(ipiv), _lib.ptr(info),
+                                  _lib.ptr(ws), ws.numel() * 8, _lib.stream_handle(None)))
+    out = Ad.cpu().numpy()
+    assert int(info.item()) == info0 == 0
+    assert np.array_equal(ipiv.cpu().numpy(), piv0)
+    assert np.array_equal(np.tril(out[:, :n].T), L0)
+    assert np.all(out[:, n:] == 7.0)
+
+
 @pytest.mark.parametrize("zero_at", [5, 700, 1070])
 def test_bkf_blocked_singular_reports_info(zero_at):
     # a zero pivot in the first panel, a later panel, and the unblocked tail
