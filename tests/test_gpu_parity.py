@@ -269,6 +269,31 @@ def test_potrs_single_launch_solves(n):
     assert np.linalg.norm(A @ outs[0] - y) <= 1e-13 * np.linalg.norm(A) * np.linalg.norm(outs[0])
 
 
+@pytest.mark.parametrize("n,lda", [(1100, 1109), (4100, 4224)])
+def test_potrs_leading_dimension(n, lda):
+    # The factor embedded in a wider column-major array (lda > n): the single-launch solves and
+    # their diagonal-block inverses read it through lda; the result equals the dense case's.
+    from qml_amd import _lib
+    from qml_amd.math import potrf_device, potrs_device
+
+    rng = np.random.default_rng(lda)
+    M = rng.standard_normal((n, 64))
+    A = M @ M.T / 64 + np.eye(n)
+    y = rng.standard_normal(n)
+    Ad = torch.from_numpy(A.copy()).cuda()
+    assert int(potrf_device(Ad, n).item()) == 0
+    xd = torch.from_numpy(y.copy()).cuda()
+    potrs_device(Ad, n, xd)
+    big = torch.full((n, lda), 3.0, dtype=torch.float64, device="cuda")
+    big[:, :n] = Ad  # row c of the (n, lda) view is column c of the factor
+    xw = torch.from_numpy(y.copy()).cuda()
+    L = _lib.lib()
+    ws = torch.empty((L.qmlh_potrs_workspace(n) + 7) // 8, dtype=torch.float64, device="cuda")
+    _lib.check(L.qmlh_potrs(_lib.ptr(big), n, lda, _lib.ptr(xw), _lib.ptr(ws), ws.numel() * 8,
+                            _lib.stream_handle(None)))
+    assert np.array_equal(bits(xw.cpu().numpy()), bits(xd.cpu().numpy()))
+
+
 @pytest.mark.parametrize("n,bad", [(400, 0), (400, 5), (400, 64), (400, 255), (400, 256),
                                    (400, 300), (4500, 0), (4500, 300), (4500, 2100), (4500, 4499)])
 def test_potrf_info_matches_lapack(n, bad):
