@@ -21,7 +21,8 @@
 // Tiles: 256 lanes own a 128 x 128 lower tile of C (2 x 2 waves of 64 x 64); chunks of 32
 // features are double buffered through LDS (4 panels x 8 KB per buffer), copied there by
 // gfx950's direct global-to-LDS loads (no staging registers: 143 VGPRs, leaving room on each
-// CU for the look-ahead panel's kernels on the side stream).
+// CU for the look-ahead panel's kernels on the side stream). Trailing updates of order 4096
+// and above use 256 x 256 tiles instead (syrk_f16x3_wide_kernel).
 //
 // Tile order. Workgroups are handed to the 8 XCDs round robin (workgroup b to XCD b % 8), and
 // each XCD has its own L2. The tiles are grouped in 8 x 8 super-blocks, and the workgroup
@@ -153,6 +154,7 @@ __device__ __forceinline__ f16x8_t sb_neg(f16x8_t v) {
 // look-ahead is dropped for the large steps; persistent grids with slots kept free for the
 // panel 182-195 ms.
 static constexpr int SB_SUPER = 8;  // super-block side in tiles
+static constexpr int64_t SW_MIN = 4096;  // trailing updates of this order and above: wide tiles
 static constexpr int SB_XCDS = 8;
 
 // Launch size for t x t tiles: 64 workgroups per lower super-block, padded to a multiple of
@@ -335,6 +337,186 @@ __global__ __launch_bounds__(256, 2) void syrk_f16x3_kernel(const sb_elem* __res
         }
 }
 
+// ------------------------------------------------------------------------------------------
+// Wide tiles for the large trailing updates: 512 lanes own a 256 x 256 lower tile (4 x 2
+// waves of 64 rows x 128 columns, 128 accumulators per lane), chunks double buffered in
+// 128 KB of LDS, one workgroup per CU. Per flop the tile copies half the panel bytes of the
+// 128 x 128 tile, so the same bytes in flight per CU keep the matrix cores busy twice as long.
+// Measured alone at m = 46,080, kb = 768: 4.6-4.8 ms against 5.1-5.3 ms for the 128 x 128
+// tile; a 256 x 128 tile with a three-stage ring (chunks issued two ahead) took 5.7 ms. What
+// the wide tile leaves exposed is C itself (read and written once per launch, 8.5 GB at that
+// size), which one workgroup per CU cannot overlap with its own products.
+// Tiles are grouped in 4 x 4 super-blocks, each XCD taking whole super-blocks as above.
+// ------------------------------------------------------------------------------------------
+static constexpr int SW_T = 256;
+static constexpr int SW_PANEL = SW_T * SB_KC;  // 16 KB
+static constexpr int SW_SUPER = 4;
+static constexpr int SW_THREADS = 512;
+
+__host__ __device__ inline int64_t sw_super_count(int64_t t) {
+    const int64_t T = (t + SW_SUPER - 1) / SW_SUPER;
+    return T * (T + 1) / 2;
+}
+
+__device__ __forceinline__ bool sw_tile(int64_t v, int64_t t_tiles, int64_t& bi, int64_t& bj) {
+    const int64_t x = v % SB_XCDS, l = v / SB_XCDS;
+    const int64_t sbi = (l / (SW_SUPER * SW_SUPER)) * SB_XCDS + x;
+    const int64_t lt = l % (SW_SUPER * SW_SUPER);
+    if (sbi >= sw_super_count(t_tiles)) return false;
+    int64_t I = (int64_t)((sqrt(8.0 * (double)sbi + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > sbi) --I;
+    while ((I + 1) * (I + 2) / 2 <= sbi) ++I;
+    const int64_t J = sbi - I * (I + 1) / 2;
+    bi = I * SW_SUPER + lt / SW_SUPER;
+    bj = J * SW_SUPER + lt % SW_SUPER;
+    return bi < t_tiles && bj <= bi;
+}
+
+// One 32-deep chunk of 256 rows (4 runs of 256 x 16 B) into an LDS panel, element (q, row) at
+// (q * 256 + row) * 8; rows past rmax (the last row of the split) read row rmax, whose
+// products land only in entries outside the update.
+__device__ __forceinline__ void sw_load_lds(sb_elem* __restrict__ panel,
+                                            const sb_elem* __restrict__ src, int64_t m_pad,
+                                            int64_t row0, int64_t rmax, int64_t c) {
+    const int t = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = t + SW_THREADS * h;  // 0 .. 1023
+        const int q = p >> 8, row = p & 255;
+        const int64_t rr = row0 + row < rmax ? row0 + row : rmax;
+        const sb_elem* g = src + ((c * 4 + q) * m_pad + rr) * 8;
+        __builtin_amdgcn_global_load_lds(
+            (const void*)g,
+            (__attribute__((address_space(3))) void*)(panel + (SW_THREADS * h + 64 * wave) * 8),
+            16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ f16x8_t sw_frag(const sb_elem* __restrict__ panel, int row, int g) {
+    return *reinterpret_cast<const f16x8_t*>(panel + (g * SW_T + row) * 8);
+}
+
+// A wave's 64 x 128 block as 8 x 4 blocks of 16 x 16: lane l holds, of block (a, b),
+// accumulator e at output row row0 + 16 b + (l & 15) and column col0 + 16 a + 4 (l / 16) + e.
+struct SwAcc {
+    f32x4_t v[8][4];
+};
+
+// C(i, j) -= sum_k P(i, k) P(j, k), i >= j, over the lower triangle of the m x m block; Ph / Pl
+// at row `off` of the split, rmax = the last row of the split relative to that.
+__global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
+    const sb_elem* __restrict__ Ph, const sb_elem* __restrict__ Pl, int64_t m, int64_t m_pad,
+    int64_t rmax, int64_t kb_pad, float* __restrict__ C, int64_t ldc,
+    const int32_t* __restrict__ info, const unsigned* __restrict__ hdr) {
+    // buffers x {row hi, row lo, col hi, col lo}
+    __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SW_PANEL];
+    if (info && *info) return;
+    const int64_t t_tiles = (m + SW_T - 1) / SW_T;
+    int64_t bi, bj;
+    if (!sw_tile(blockIdx.x, t_tiles, bi, bj)) return;
+    const int e2 = 2 * sb_scale_exp(hdr);
+    const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
+    const int64_t rows0 = bi * SW_T, cols0 = bj * SW_T;
+
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int r = lane & 15, g = lane >> 4;
+    // wave w: first operand (accumulator rows) = output columns cols0 + 128 (w & 1),
+    // second operand = output rows rows0 + 64 (w >> 1)
+    const int fr = 128 * (w & 1), sr = 64 * (w >> 1);
+    const int64_t col0 = cols0 + fr;
+    const int64_t row0 = rows0 + sr;
+    const bool live = row0 < m && col0 < m && col0 <= row0 + 63;
+    const bool interior = live && row0 + 63 < m && col0 + 127 < m && col0 + 127 <= row0;
+
+    sw_load_lds(lds[0][0], Ph, m_pad, rows0, rmax, 0);
+    sw_load_lds(lds[0][1], Pl, m_pad, rows0, rmax, 0);
+    sw_load_lds(lds[0][2], Ph, m_pad, cols0, rmax, 0);
+    sw_load_lds(lds[0][3], Pl, m_pad, cols0, rmax, 0);
+    SwAcc acc;
+    if (interior) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    acc.v[a][b][e] =
+                        C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] * up;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t i = row0 + 16 * b + r;
+                    const int64_t j = col0 + 16 * a + 4 * g + e;
+                    acc.v[a][b][e] = (live && i < m && j <= i) ? C[i + j * ldc] * up : 0.0f;
+                }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    const int64_t nchunks = kb_pad / SB_KC;
+    int st = 0;
+    for (int64_t c = 0; c < nchunks; ++c) {
+        if (c + 1 < nchunks) {
+            sw_load_lds(lds[st ^ 1][0], Ph, m_pad, rows0, rmax, c + 1);
+            sw_load_lds(lds[st ^ 1][1], Pl, m_pad, rows0, rmax, c + 1);
+            sw_load_lds(lds[st ^ 1][2], Ph, m_pad, cols0, rmax, c + 1);
+            sw_load_lds(lds[st ^ 1][3], Pl, m_pad, cols0, rmax, c + 1);
+        }
+        f16x8_t sh[4], sl[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            sh[b] = sb_neg(sw_frag(lds[st][0], sr + 16 * b + r, g));
+            sl[b] = sb_neg(sw_frag(lds[st][1], sr + 16 * b + r, g));
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const f16x8_t fh = sw_frag(lds[st][2], fr + 16 * a + r, g);
+            const f16x8_t fl = sw_frag(lds[st][3], fr + 16 * a + r, g);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                acc.v[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fl, sh[b], acc.v[a][b], 0, 0, 0);
+                acc.v[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fh, sl[b], acc.v[a][b], 0, 0, 0);
+                acc.v[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fh, sh[b], acc.v[a][b], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        st ^= 1;
+    }
+
+    if (!live) return;
+    if (interior) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    C[(row0 + 16 * b + r) + (col0 + 16 * a + 4 * g + e) * ldc] =
+                        acc.v[a][b][e] * down;
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t i = row0 + 16 * b + r;
+            if (i >= m) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t j = col0 + 16 * a + 4 * g + e;
+                if (j > i) continue;
+                C[i + j * ldc] = acc.v[a][b][e] * down;
+            }
+        }
+}
+
 size_t syrk_f16_workspace(int64_t n, int64_t kb) {
     const int64_t m_pad = round_up(n > 0 ? n : 1, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
@@ -378,6 +560,18 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
                          (long long)off, (long long)m);
     const sb_elem* Ph = sb_panels(const_cast<void*>(ws)) + off * 8;
     const sb_elem* Pl = sb_panels(const_cast<void*>(ws)) + m_pad * kb_pad + off * 8;
+    if (ncols == 0 && m >= SW_MIN) {
+        const int64_t tw = ceil_div(m, (int64_t)SW_T);
+        const int64_t grid = ceil_div(sw_super_count(tw), (int64_t)SB_XCDS) * SB_XCDS * SW_SUPER *
+                             SW_SUPER;
+        if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_update: grid too large");
+        TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
+        hipLaunchKernelGGL(syrk_f16x3_wide_kernel, dim3((unsigned)grid), dim3(SW_THREADS), 0, s,
+                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, C, ldc, info,
+                           static_cast<const unsigned*>(ws));
+        QMLH_CHECK_LAUNCH("syrk_f16x3_wide launch");
+        return QMLH_OK;
+    }
     const int64_t tm = ceil_div(m, (int64_t)SB_T);
     int64_t grid;
     double flops;

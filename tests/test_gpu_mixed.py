@@ -49,10 +49,11 @@ def test_spotrf_fp32_factor_vs_lapack(n):
     assert np.array_equal(np.tril(Ad.cpu().numpy(), -1), np.tril(A, -1))
 
 
-@pytest.mark.parametrize("n", [257, 700, 1500, 5000])
+@pytest.mark.parametrize("n", [257, 700, 1500, 5000, 9001])
 def test_spotrf_f16x3_factor_accuracy(n):
     # Split-FP16 trailing updates: each update ~2^-22 relative, so the factor agrees with
-    # LAPACK's to ~1e-5 normwise on a well-conditioned matrix.
+    # LAPACK's to ~1e-5 normwise on a well-conditioned matrix. From order 4096 the trailing
+    # updates run on 256 x 256 tiles (n = 9001: every step's tiles are ragged at the edge).
     from scipy.linalg import lapack
 
     from qml_amd import _lib
