@@ -15,17 +15,23 @@ template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
                void* split_ws = nullptr, int64_t r = 0, double pivot_floor = 0.0);
 
+// Workspace of the split: a header (the scale), then regions of f16x3_region_bytes(n, kb)
+// each holding the hi/lo panels of one split of up to n rows and kb features.
+size_t f16x3_region_bytes(int64_t n, int64_t kb);
 size_t syrk_f16_workspace(int64_t n, int64_t kb);
 // Record the scale of the split panels (from the largest diagonal entry of the matrix about
 // to be factored) in ws; before the first split.
 int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t s);
-// Split P (m x kb, FP32 column-major) into the scaled FP16 hi/lo panels in ws.
-int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s);
+// Split P (m x kb, FP32 column-major) into the scaled FP16 hi/lo panels of the region that
+// starts `region` bytes after the header of ws.
+int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s,
+                size_t region);
 // C -= P P^T in split FP16 from a split of m_split rows, for the m rows starting at row off
 // (a multiple of 128): ncols == 0 the lower triangle of the m x m block, ncols > 0 the entries
 // i >= j of its first ncols columns.
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
-                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s);
+                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
+                  size_t region);
 // C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three FP16 products.
 int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s);

@@ -3,9 +3,10 @@
 // Replaces LAPACK dpotrf("U") / dpotrs("U", n, 1) as called at qml/fcho_solve.f90:35,46 (see
 // the header for why "U" on the Fortran view is the lower triangle here).
 //
-// Factorisation, outer block nb (256; 512 with split-FP16 trailing updates), per step k
+// Factorisation, outer block nb (512; 768 with split-FP16 trailing updates), per step k
 // (from order 4096 on with look-ahead: the next block column is factored on a second stream
-// while the rest of the trailing update runs, see potrf_lookahead):
+// while the rest of the trailing update runs, see potrf_lookahead; the split-FP16 factor
+// works in groups of two block columns, see potrf_lookahead_split):
 //   for each 64-wide sub-panel of the nb-wide block column (left to right):
 //     1. the 64 x 64 diagonal block is factored in the registers of one wavefront (FP64
 //        arithmetic), then every row of the column below is solved against it (256 rows per
@@ -242,7 +243,7 @@ static constexpr int64_t PF_EXT_SIDE_MIN = 20000;
 struct LookaheadCtx {
     int device = -1;
     hipStream_t s2 = nullptr, s3 = nullptr;
-    hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr, ev_e = nullptr;
+    hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr, ev_e = nullptr, ev_b = nullptr;
 };
 
 static std::mutex g_la_mu;
@@ -266,13 +267,13 @@ static int lookahead_acquire(LookaheadCtx** out) {
     int rc = check_hip(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking), "potrf: side stream");
     if (rc == QMLH_OK)
         rc = check_hip(hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking), "potrf: third stream");
-    hipEvent_t* evs[4] = {&c->ev_a, &c->ev_p, &c->ev_x, &c->ev_e};
-    for (int i = 0; i < 4 && rc == QMLH_OK; ++i)
+    hipEvent_t* evs[5] = {&c->ev_a, &c->ev_p, &c->ev_x, &c->ev_e, &c->ev_b};
+    for (int i = 0; i < 5 && rc == QMLH_OK; ++i)
         rc = check_hip(hipEventCreateWithFlags(evs[i], hipEventDisableTiming), "potrf: event");
     if (rc != QMLH_OK) {  // nothing was enqueued on them yet
         if (c->s2) (void)hipStreamDestroy(c->s2);
         if (c->s3) (void)hipStreamDestroy(c->s3);
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 5; ++i)
             if (*evs[i]) (void)hipEventDestroy(*evs[i]);
         delete c;
         return rc;
@@ -291,11 +292,11 @@ static void lookahead_release(LookaheadCtx* c) {
 
 template <typename T>
 static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
-                           hipStream_t s, void* split_ws, int64_t nb, double pf) {
+                           hipStream_t s, int64_t nb, double pf) {
     LookaheadCtx* ctx = nullptr;
     QMLH_TRY(lookahead_acquire(&ctx));
-    hipStream_t s2 = ctx->s2, s3 = ctx->s3;
-    hipEvent_t ev_a = ctx->ev_a, ev_p = ctx->ev_p, ev_x = ctx->ev_x, ev_e = ctx->ev_e;
+    hipStream_t s2 = ctx->s2;
+    hipEvent_t ev_a = ctx->ev_a, ev_p = ctx->ev_p;
     int rc = factor_block_column<T>(A, lda, n + r, 0, n < nb ? n : nb, info, s, nullptr, pf);
     for (int64_t k = 0; rc == QMLH_OK && k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
@@ -304,54 +305,120 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
         const int64_t kb2 = m < nb ? m : nb;  // next block column
         T* P = A + (k + kb) + k * lda;
         T* C = P + kb * lda;  // A(k + kb, k + kb)
-        const bool split = std::is_same<T, float>::value && split_ws && m > kb2;
-        hipEvent_t rows_wait = nullptr;  // what the next panel's rows wait for
-        if (split) {
-            // split-FP16 trailing updates: split the panel once; the next block column gets
-            // the same split-FP16 update as the rest, then the rest of the trailing matrix
-            // reuses the split from row kb2 on. The extra rows stay out of the split (their
-            // entries are not bounded by the diagonal that sets the split's scale): their part
-            // of the next block column is an FP32 product on a third stream, which the next
-            // panel waits for only before its rows are solved.
-            float* Pf = reinterpret_cast<float*>(P);
-            float* Cf = reinterpret_cast<float*>(C);
-            if (r > 0) {
-                rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
-                if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
-                if (rc == QMLH_OK)
-                    rc = launch_update<T>(UPD_FULL, P + m, lda, P, lda, r, kb2, kb, C + m, lda,
-                                          info, s3);
-                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
-                rows_wait = ev_e;
-            }
-            if (rc == QMLH_OK) rc = f16x3_split(Pf, lda, m, kb, split_ws, s);
-            if (rc == QMLH_OK)
-                rc = f16x3_update(split_ws, m, kb, 0, m, kb2, Cf, lda, info, s);
-        } else {
-            rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
-        }
+        rc = launch_update<T>(UPD_TRAP, P, lda, P, lda, m + r, kb2, kb, C, lda, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
         if (rc == QMLH_OK)
-            rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2, rows_wait, pf);
+            rc = factor_block_column<T>(A, lda, n + r, k + kb, kb2, info, s2, nullptr, pf);
         const bool ext_side = m >= PF_EXT_SIDE_MIN;
         if (rc == QMLH_OK && ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s2);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
-        if (rc == QMLH_OK && m > kb2) {
-            if (split)
-                rc = f16x3_update(split_ws, m, kb, kb2, m - kb2, 0,
-                                   reinterpret_cast<float*>(C + kb2 + kb2 * lda), lda, info, s);
-            else
-                rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s,
-                                      split_ws);
-        }
+        if (rc == QMLH_OK && m > kb2)
+            rc = trailing_syrk<T>(P + kb2, lda, m - kb2, kb, C + kb2 + kb2 * lda, info, s,
+                                  nullptr);
         if (rc == QMLH_OK && !ext_side)
             rc = extra_rows_update<T>(A, lda, n, r, k, kb, k + kb + kb2, info, s);
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
     }
-    // `s` already waits for the last panel (and the third stream's last work precedes the
-    // last panel's rows), so every later use of these streams is ordered after this one.
+    // `s` already waits for the last panel, so every later use of these streams is ordered
+    // after this one.
+    lookahead_release(ctx);
+    return rc;
+}
+
+// Split-FP16 look-ahead in groups of two block columns. The panels of a group are adjacent
+// columns, so the group's rows below it form one column-major block of 2 nb columns: it is
+// split once, and the next group's columns and the rest of the trailing matrix each take one
+// update with 2 nb features, which halves the reads and writes of the FP32 trailing matrix per
+// product against one update per block column. Per group:
+//   caller's stream: split of the group (all rows below it, region 1) -> update of the next
+//     group's first block column -> [event a] -> update of its second block column ->
+//     [event b] -> update of the rest of the trailing matrix;
+//   side stream, after event a: the next group -- its first block column factored, split
+//     (region 0), then after event b the second block column updated from it and factored.
+// The caller's stream then waits for the side stream before the next group's split. With r
+// extra rows (A[n:n+r, :]) the next group's part of their update (an FP32 product) runs on a
+// third stream that the side stream waits for before solving rows, and the rest of their
+// update right after it on the third stream (it writes only extra rows of later columns,
+// which only later work on that stream reads, and whose solve waits for its next event).
+static constexpr int64_t PF_GROUP = 2 * PF_NB_SPLIT;
+
+static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB_SPLIT); }
+
+// Factor the w (<= 2 nb) columns from k, whose updates from earlier columns are applied (to
+// the second block column once b_ready has fired, when given).
+static int factor_group_split(float* A, int64_t lda, int64_t n, int64_t r, int64_t k, int64_t w,
+                              int32_t* info, hipStream_t st, hipEvent_t rows_wait,
+                              hipEvent_t b_ready, void* ws, double pf) {
+    const int64_t wa = w < PF_NB_SPLIT ? w : PF_NB_SPLIT;
+    QMLH_TRY(factor_block_column<float>(A, lda, n + r, k, wa, info, st, rows_wait, pf));
+    const int64_t wb = w - wa;
+    if (wb <= 0) return QMLH_OK;
+    const int64_t ma = n - k - wa;  // rows of the matrix below the first block column
+    float* Pa = A + (k + wa) + k * lda;
+    float* Cb = Pa + wa * lda;  // A(k + wa, k + wa)
+    QMLH_TRY(f16x3_split(Pa, lda, ma, wa, ws, st, 0));
+    if (b_ready) QMLH_TRY(check_hip(hipStreamWaitEvent(st, b_ready, 0), "potrf: wait"));
+    QMLH_TRY(f16x3_update(ws, ma, wa, 0, ma, wb, Cb, lda, info, st, 0));
+    if (r > 0)
+        QMLH_TRY(launch_update<float>(UPD_FULL, Pa + ma, lda, Pa, lda, r, wb, wa, Cb + ma, lda,
+                                      info, st));
+    return factor_block_column<float>(A, lda, n + r, k + wa, wb, info, st, nullptr, pf);
+}
+
+static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
+                                 hipStream_t s, void* ws, double pf) {
+    LookaheadCtx* ctx = nullptr;
+    QMLH_TRY(lookahead_acquire(&ctx));
+    hipStream_t s2 = ctx->s2, s3 = ctx->s3;
+    hipEvent_t ev_a = ctx->ev_a, ev_p = ctx->ev_p, ev_x = ctx->ev_x, ev_e = ctx->ev_e;
+    hipEvent_t ev_b = ctx->ev_b;
+    const size_t region = split_group_region(n);
+    int64_t k = 0, w = n < PF_GROUP ? n : PF_GROUP;
+    int rc = factor_group_split(A, lda, n, r, 0, w, info, s, nullptr, nullptr, ws, pf);
+    while (rc == QMLH_OK && k + w < n) {
+        const int64_t m = n - k - w;                  // rows below the group
+        const int64_t w2 = m < PF_GROUP ? m : PF_GROUP;  // the next group
+        float* P = A + (k + w) + k * lda;             // the group's rows below it (m x w)
+        float* C = P + w * lda;                       // A(k + w, k + w)
+        hipEvent_t rows_wait = nullptr;
+        if (r > 0) {
+            // the extra rows' update by this group, all on the third stream: the next group's
+            // columns (which that group's rows solve waits for), then the rest
+            rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
+            if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
+            if (rc == QMLH_OK)
+                rc = launch_update<float>(UPD_FULL, P + m, lda, P, lda, r, w2, w, C + m, lda,
+                                          info, s3);
+            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+            if (rc == QMLH_OK)
+                rc = extra_rows_update<float>(A, lda, n, r, k, w, k + w + w2, info, s3);
+            rows_wait = ev_e;
+        }
+        const int64_t wa2 = w2 < PF_NB_SPLIT ? w2 : PF_NB_SPLIT;  // next group's first block column
+        if (rc == QMLH_OK) rc = f16x3_split(P, lda, m, w, ws, s, region);
+        if (rc == QMLH_OK) rc = f16x3_update(ws, m, w, 0, m, wa2, C, lda, info, s, region);
+        if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
+        if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
+        if (rc == QMLH_OK && w2 > wa2) {
+            rc = f16x3_update(ws, m, w, wa2, m - wa2, w2 - wa2, C + wa2 + wa2 * lda, lda, info, s,
+                              region);
+            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_b, s), "potrf: record");
+        }
+        if (rc == QMLH_OK)
+            rc = factor_group_split(A, lda, n, r, k + w, w2, info, s2, rows_wait,
+                                    w2 > wa2 ? ev_b : nullptr, ws, pf);
+        if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
+        if (rc == QMLH_OK && m > w2)
+            rc = f16x3_update(ws, m, w, w2, m - w2, 0, C + w2 + w2 * lda, lda, info, s, region);
+        if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s, ev_p, 0), "potrf: wait");
+        k += w;
+        w = w2;
+    }
+    // `s` already waits for the last group, whose rows waited for the third stream's last
+    // work (the last group's "rest" is empty), so every later use of these streams is ordered
+    // after this one.
     lookahead_release(ctx);
     return rc;
 }
@@ -370,9 +437,12 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     const bool split16 = std::is_same<T, float>::value && split_ws;
     const int64_t nb = split16 ? PF_NB_SPLIT : PF_NB;
     if (split16) QMLH_TRY(f16x3_prepare(reinterpret_cast<const float*>(A), n, lda, split_ws, s));
-    if (n >= PF_LOOKAHEAD_MIN)
-        return potrf_lookahead<T>(A, n, lda, r, info, s, split16 ? split_ws : nullptr, nb,
-                                  pivot_floor);
+    if (n >= PF_LOOKAHEAD_MIN) {
+        if constexpr (std::is_same<T, float>::value)
+            if (split16)
+                return potrf_lookahead_split(A, n, lda, r, info, s, split_ws, pivot_floor);
+        return potrf_lookahead<T>(A, n, lda, r, info, s, nb, pivot_floor);
+    }
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
         QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s, nullptr, pivot_floor));
@@ -812,8 +882,13 @@ extern "C" int qmlh_spotrf(float* A, int64_t n, int64_t lda, int32_t* info,
     return qmlh::potrf_impl<float>(A, n, lda, info, (hipStream_t)stream);
 }
 
+// The split of one block column (region 0); from the look-ahead order on also the split of a
+// group of two (region 1).
 extern "C" size_t qmlh_spotrf_f16x3_workspace(int64_t n) {
-    return n > 0 ? qmlh::syrk_f16_workspace(n, qmlh::PF_NB_SPLIT) : 0;
+    using namespace qmlh;
+    if (n <= 0) return 0;
+    const size_t base = syrk_f16_workspace(n, PF_NB_SPLIT);
+    return n >= PF_LOOKAHEAD_MIN ? base + f16x3_region_bytes(n, PF_GROUP) : base;
 }
 
 extern "C" int qmlh_spotrf_f16x3(float* A, int64_t n, int64_t lda, int32_t* info, void* ws,

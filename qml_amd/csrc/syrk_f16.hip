@@ -517,10 +517,14 @@ __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
         }
 }
 
-size_t syrk_f16_workspace(int64_t n, int64_t kb) {
+size_t f16x3_region_bytes(int64_t n, int64_t kb) {
     const int64_t m_pad = round_up(n > 0 ? n : 1, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
-    return SB_HDR_BYTES + (size_t)(2 * m_pad * kb_pad) * sizeof(sb_elem);
+    return (size_t)(2 * m_pad * kb_pad) * sizeof(sb_elem);
+}
+
+size_t syrk_f16_workspace(int64_t n, int64_t kb) {
+    return SB_HDR_BYTES + f16x3_region_bytes(n, kb);
 }
 
 int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t s) {
@@ -533,15 +537,16 @@ int f16x3_prepare(const float* A, int64_t n, int64_t lda, void* ws, hipStream_t 
     return QMLH_OK;
 }
 
-static sb_elem* sb_panels(void* ws) {
-    return reinterpret_cast<sb_elem*>(static_cast<char*>(ws) + SB_HDR_BYTES);
+static sb_elem* sb_panels(void* ws, size_t region) {
+    return reinterpret_cast<sb_elem*>(static_cast<char*>(ws) + SB_HDR_BYTES + region);
 }
 
-int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s) {
+int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hipStream_t s,
+                size_t region) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
-    sb_elem* Ph = sb_panels(ws);
+    sb_elem* Ph = sb_panels(ws, region);
     sb_elem* Pl = Ph + m_pad * kb_pad;
     hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)(kb_pad / 8)),
                        dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl,
@@ -551,15 +556,16 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
 }
 
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
-                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s) {
+                  int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
+                  size_t region) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m_split, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
     if (off % SB_T != 0 || off + round_up(m, SB_T) > m_pad)
         return set_error(QMLH_ERR_ARG, "f16x3_update: rows %lld + %lld outside the split",
                          (long long)off, (long long)m);
-    const sb_elem* Ph = sb_panels(const_cast<void*>(ws)) + off * 8;
-    const sb_elem* Pl = sb_panels(const_cast<void*>(ws)) + m_pad * kb_pad + off * 8;
+    const sb_elem* Ph = sb_panels(const_cast<void*>(ws), region) + off * 8;
+    const sb_elem* Pl = sb_panels(const_cast<void*>(ws), region) + m_pad * kb_pad + off * 8;
     if (ncols == 0 && m >= SW_MIN) {
         const int64_t tw = ceil_div(m, (int64_t)SW_T);
         const int64_t grid = ceil_div(sw_super_count(tw), (int64_t)SB_XCDS) * SB_XCDS * SW_SUPER *
@@ -596,8 +602,8 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
 int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
-    QMLH_TRY(f16x3_split(P, ldp, m, kb, ws, s));
-    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s);
+    QMLH_TRY(f16x3_split(P, ldp, m, kb, ws, s, 0));
+    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s, 0);
 }
 
 }  // namespace qmlh
