@@ -275,7 +275,8 @@ int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_
  * NULL) of A that also overwrites V (n x r, column-major, ldv >= n) with L^-1 V in the same
  * pass: V^T is placed in the r rows below the matrix (lda >= n + r; A[n:n+r, :] is
  * overwritten), solved as extra panel rows while the trailing updates run (their own updates in
- * FP32 MFMA), and copied back. The factor of A is the same, bit for bit, as without V. On a
+ * FP32 MFMA; with f16x3 from order 4096 and r <= 1536, split-FP16 products with a scale of
+ * their own, sharing the factor's workspace), and copied back. The factor of A is the same, bit for bit, as without V. On a
  * failed factorisation (info != 0) V is unspecified. Replaces qmlh_spotrf* + the solve of
  * qmlh_woodbury_setup. pivot_floor > 0 selects static pivoting, for factors that only
  * precondition: a finite pivot at or below pivot_floor is replaced by pivot_floor (the factor

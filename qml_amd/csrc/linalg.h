@@ -32,6 +32,12 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
                   int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
                   size_t region);
+// C (r x m) -= X P^T: X (r x kb, FP32 column-major) split with its own scale into the region at
+// region_x (its entries need not be bounded by the factor's diagonal), P the split of m rows in
+// the region at region_p (kb features, as f16x3_split left it); three FP16 products per term.
+int f16x3_gemm_rows(const float* X, int64_t ldx, int64_t r, int64_t kb, void* ws,
+                    size_t region_x, size_t region_p, int64_t m, float* C, int64_t ldc,
+                    const int32_t* info, hipStream_t s);
 // C (m x m, lower) -= P P^T, P (m x kb) FP32 column-major, as three FP16 products.
 int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int64_t ldc,
                 const int32_t* info, void* ws, hipStream_t s);

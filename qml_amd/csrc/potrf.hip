@@ -341,7 +341,8 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
 // extra rows (A[n:n+r, :]) the next group's part of their update (an FP32 product) runs on a
 // third stream that the side stream waits for before solving rows, and the rest of their
 // update right after it on the third stream (it writes only extra rows of later columns,
-// which only later work on that stream reads, and whose solve waits for its next event).
+// which only later work on that stream reads, and whose solve waits for its next event); for
+// r <= 2 nb both are one split-FP16 product (f16x3_gemm_rows).
 static constexpr int64_t PF_GROUP = 2 * PF_NB_SPLIT;
 
 static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB_SPLIT); }
@@ -375,6 +376,7 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
     hipEvent_t ev_a = ctx->ev_a, ev_p = ctx->ev_p, ev_x = ctx->ev_x, ev_e = ctx->ev_e;
     hipEvent_t ev_b = ctx->ev_b;
     const size_t region = split_group_region(n);
+    const size_t region_x = region + f16x3_region_bytes(n - PF_GROUP, PF_GROUP);
     int64_t k = 0, w = n < PF_GROUP ? n : PF_GROUP;
     int rc = factor_group_split(A, lda, n, r, 0, w, info, s, nullptr, nullptr, ws, pf);
     while (rc == QMLH_OK && k + w < n) {
@@ -383,21 +385,31 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
         float* P = A + (k + w) + k * lda;             // the group's rows below it (m x w)
         float* C = P + w * lda;                       // A(k + w, k + w)
         hipEvent_t rows_wait = nullptr;
-        if (r > 0) {
-            // the extra rows' update by this group, all on the third stream: the next group's
-            // columns (which that group's rows solve waits for), then the rest
+        if (rc == QMLH_OK) rc = f16x3_split(P, lda, m, w, ws, s, region);
+        if (rc == QMLH_OK && r > 0) {
+            // the extra rows' update by this group, on the third stream: with r <= 2 nb one
+            // split-FP16 product over all later columns (the extra rows split with their own
+            // scale into the tail of the group's region, which a group split never reaches:
+            // it has at most n - 2 nb rows); else FP32 products, the next group's columns
+            // first. The next group's rows solve waits for the event after them.
             rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
             if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
-            if (rc == QMLH_OK)
-                rc = launch_update<float>(UPD_FULL, P + m, lda, P, lda, r, w2, w, C + m, lda,
-                                          info, s3);
-            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
-            if (rc == QMLH_OK)
-                rc = extra_rows_update<float>(A, lda, n, r, k, w, k + w + w2, info, s3);
+            if (r <= PF_GROUP) {
+                if (rc == QMLH_OK)
+                    rc = f16x3_gemm_rows(P + m, lda, r, w, ws, region_x, region, m, C + m, lda,
+                                         info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+            } else {
+                if (rc == QMLH_OK)
+                    rc = launch_update<float>(UPD_FULL, P + m, lda, P, lda, r, w2, w, C + m, lda,
+                                              info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+                if (rc == QMLH_OK)
+                    rc = extra_rows_update<float>(A, lda, n, r, k, w, k + w + w2, info, s3);
+            }
             rows_wait = ev_e;
         }
         const int64_t wa2 = w2 < PF_NB_SPLIT ? w2 : PF_NB_SPLIT;  // next group's first block column
-        if (rc == QMLH_OK) rc = f16x3_split(P, lda, m, w, ws, s, region);
         if (rc == QMLH_OK) rc = f16x3_update(ws, m, w, 0, m, wa2, C, lda, info, s, region);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");

@@ -62,7 +62,32 @@ __device__ __forceinline__ int sb_scale_exp(const unsigned* hdr) {
     return e < -40 ? -40 : (e > 40 ? 40 : e);
 }
 
+// hdr[1] holds the bits of max |X| over a block X split with its own scale (the extra rows of
+// a factorisation, whose entries the diagonal does not bound): 2^e with e = 14 - (ilogb(max) + 1)
+// puts every entry below 2^14.
+__device__ __forceinline__ int sb_scale_exp_abs(const unsigned* hdr) {
+    const float amax = __uint_as_float(hdr[1]);
+    if (!(amax > 0.0f) || !(amax < 3.0e38f)) return 0;
+    const int e = 14 - (ilogbf(amax) + 1);
+    return e < -40 ? -40 : (e > 40 ? 40 : e);
+}
+
 __global__ void sb_hdr_clear_kernel(unsigned* hdr) { hdr[threadIdx.x] = 0u; }
+
+__global__ void sb_word_clear_kernel(unsigned* hdr, int idx) { hdr[idx] = 0u; }
+
+// hdr[1] = bits of max |X(i, j)| over the finite entries of the r x w block (blockIdx.y = j).
+__global__ __launch_bounds__(256) void sb_maxabs_kernel(const float* __restrict__ X, int64_t ldx,
+                                                        int64_t r, unsigned* __restrict__ hdr) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float v = i < r ? fabsf(X[i + (int64_t)blockIdx.y * ldx]) : 0.0f;
+    unsigned m = v < 3.0e38f ? __float_as_uint(v) : 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(hdr + 1, m);
+}
 
 // hdr[0] = bits of max_i A(i, i) over the positive diagonal entries (positive floats order
 // like their bit patterns).
@@ -81,18 +106,19 @@ __global__ __launch_bounds__(256) void sb_diag_max_kernel(const float* __restric
 }
 
 // Ph, Pl <- split of P (column-major FP32, m x kb); rows m .. m_pad-1 and features
-// kb .. kb_pad-1 are zero. One lane per (row, group of 8 features): 8 strided loads (lanes
+// kb .. kb_pad-1 are zero. Scale from the diagonal (hdr[0]) or, abs_scale, from max |P| (hdr[1]). One lane per (row, group of 8 features): 8 strided loads (lanes
 // run down the rows, so each is coalesced) and one 16-byte store into each of Ph and Pl.
 __global__ __launch_bounds__(256) void split_panel_kernel(const float* __restrict__ P,
                                                           int64_t ldp, int64_t m, int64_t m_pad,
                                                           int64_t kb, int64_t kb_pad,
                                                           sb_elem* __restrict__ Ph,
                                                           sb_elem* __restrict__ Pl,
-                                                          const unsigned* __restrict__ hdr) {
+                                                          const unsigned* __restrict__ hdr,
+                                                          int abs_scale) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t k0 = (int64_t)blockIdx.y * 8;
     if (row >= m_pad || k0 >= kb_pad) return;
-    const float sc = ldexpf(1.0f, sb_scale_exp(hdr));
+    const float sc = ldexpf(1.0f, abs_scale ? sb_scale_exp_abs(hdr) : sb_scale_exp(hdr));
     // loads from clamped indices (no per-load branch), zero selected afterwards
     const int64_t rr = row < m ? row : m - 1;
     float raw[8];
@@ -550,7 +576,7 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
     sb_elem* Pl = Ph + m_pad * kb_pad;
     hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(m_pad, 256), (unsigned)(kb_pad / 8)),
                        dim3(256), 0, s, P, ldp, m, m_pad, kb, kb_pad, Ph, Pl,
-                       static_cast<const unsigned*>(ws));
+                       static_cast<const unsigned*>(ws), 0);
     QMLH_CHECK_LAUNCH("split_panel launch");
     return QMLH_OK;
 }
@@ -594,6 +620,100 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
     hipLaunchKernelGGL(syrk_f16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Ph, Pl, m,
                        m_pad, kb_pad, ncols, C, ldc, info, static_cast<const unsigned*>(ws));
     QMLH_CHECK_LAUNCH("syrk_f16x3 launch");
+    return QMLH_OK;
+}
+
+// C(i, j) -= sum_k X(i, k) P(j, k) for all i < r, j < m: X (r x kb) split with its own scale
+// (hdr[1]), P a split of m rows (hdr[0]); the products carry 2^(e_x + e_p). 128 x 128 tiles, one
+// per workgroup, the X tiles of one P column block adjacent in launch order (they share its
+// chunks).
+__global__ __launch_bounds__(256, 2) void gemm_f16x3_kernel(
+    const sb_elem* __restrict__ Xh, const sb_elem* __restrict__ Xl, int64_t r, int64_t x_pad,
+    const sb_elem* __restrict__ Ph, const sb_elem* __restrict__ Pl, int64_t m, int64_t m_pad,
+    int64_t kb_pad, float* __restrict__ C, int64_t ldc, const int32_t* __restrict__ info,
+    const unsigned* __restrict__ hdr) {
+    __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SB_PANEL];
+    if (info && *info) return;
+    const int64_t tr = (r + SB_T - 1) / SB_T;
+    const int64_t bi = blockIdx.x % tr, bj = blockIdx.x / tr;
+    const int e2 = sb_scale_exp(hdr) + sb_scale_exp_abs(hdr);
+    const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
+    const int64_t rows0 = bi * SB_T, cols0 = bj * SB_T;
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int rr = lane & 15, g = lane >> 4;
+    const int fr = 64 * (w & 1), sr = 64 * (w >> 1);
+    const int64_t col0 = cols0 + fr, row0 = rows0 + sr;
+    const bool live = row0 < r && col0 < m;
+    const bool interior = row0 + 63 < r && col0 + 63 < m;
+    sb_load_lds(lds[0][0], Xh, x_pad, rows0, 0);
+    sb_load_lds(lds[0][1], Xl, x_pad, rows0, 0);
+    sb_load_lds(lds[0][2], Ph, m_pad, cols0, 0);
+    sb_load_lds(lds[0][3], Pl, m_pad, cols0, 0);
+    SbAcc acc;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = row0 + 16 * b + rr, j = col0 + 16 * a + 4 * g + e;
+                acc.v[a][b][e] = (interior || (i < r && j < m)) ? C[i + j * ldc] * up : 0.0f;
+            }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int64_t nchunks = kb_pad / SB_KC;
+    int st = 0;
+    for (int64_t c = 0; c < nchunks; ++c) {
+        if (c + 1 < nchunks) {
+            sb_load_lds(lds[st ^ 1][0], Xh, x_pad, rows0, c + 1);
+            sb_load_lds(lds[st ^ 1][1], Xl, x_pad, rows0, c + 1);
+            sb_load_lds(lds[st ^ 1][2], Ph, m_pad, cols0, c + 1);
+            sb_load_lds(lds[st ^ 1][3], Pl, m_pad, cols0, c + 1);
+        }
+        SbFrags F;
+        sb_read_frags(F, lds[st], fr, sr, lane);
+        sb_products(acc, F);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        st ^= 1;
+    }
+    if (!live) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = row0 + 16 * b + rr, j = col0 + 16 * a + 4 * g + e;
+                if (interior || (i < r && j < m)) C[i + j * ldc] = acc.v[a][b][e] * down;
+            }
+}
+
+int f16x3_gemm_rows(const float* X, int64_t ldx, int64_t r, int64_t kb, void* ws,
+                    size_t region_x, size_t region_p, int64_t m, float* C, int64_t ldc,
+                    const int32_t* info, hipStream_t s) {
+    if (r <= 0 || m <= 0 || kb <= 0) return QMLH_OK;
+    unsigned* hdr = static_cast<unsigned*>(ws);
+    const int64_t x_pad = round_up(r, SB_T), m_pad = round_up(m, SB_T);
+    const int64_t kb_pad = round_up(kb, SB_KC);
+    sb_elem* Xh = sb_panels(ws, region_x);
+    sb_elem* Xl = Xh + x_pad * kb_pad;
+    hipLaunchKernelGGL(sb_word_clear_kernel, dim3(1), dim3(1), 0, s, hdr, 1);
+    hipLaunchKernelGGL(sb_maxabs_kernel, dim3((unsigned)ceil_div(r, 256), (unsigned)kb), dim3(256),
+                       0, s, X, ldx, r, hdr);
+    hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(x_pad, 256), (unsigned)(kb_pad / 8)),
+                       dim3(256), 0, s, X, ldx, r, x_pad, kb, kb_pad, Xh, Xl,
+                       static_cast<const unsigned*>(ws), 1);
+    QMLH_CHECK_LAUNCH("extra rows split launch");
+    const sb_elem* Ph = sb_panels(ws, region_p);
+    const sb_elem* Pl = Ph + m_pad * kb_pad;
+    const int64_t grid = ceil_div(r, (int64_t)SB_T) * ceil_div(m, (int64_t)SB_T);
+    if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_gemm_rows: grid too large");
+    TimerScope ts(QMLH_TIMER_GEMM, 2.0 * (double)r * (double)m * (double)kb, s);
+    hipLaunchKernelGGL(gemm_f16x3_kernel, dim3((unsigned)grid), dim3(256), 0, s, Xh, Xl, r, x_pad,
+                       Ph, Pl, m, m_pad, kb_pad, C, ldc, info, static_cast<const unsigned*>(ws));
+    QMLH_CHECK_LAUNCH("gemm_f16x3 launch");
     return QMLH_OK;
 }
 
