@@ -285,6 +285,21 @@ int qmlh_woodbury_setup(const float *L, int64_t n, int64_t ldl, float *V, int64_
 int qmlh_spotrf_lowrank(float *A, int64_t n, int64_t lda, float *V, int64_t ldv, int64_t r,
                         int32_t f16x3, double pivot_floor, int32_t *info, void *ws,
                         size_t ws_bytes, qmlh_stream_t stream);
+/* qmlh_spotrf_lowrank with shift on breakdown (breakdown_shift > 0; for factors that only
+ * precondition; no counterpart in the reference, which has no low-precision factor): a pivot
+ * at or below breakdown_pivot (>= 0; for A = B + lambda I with B positive semidefinite every
+ * exact pivot is at least lambda, so a smaller one is rounding) counts as a breakdown; where a
+ * 64 x 64 diagonal block has one it is factored again with its diagonal raised by
+ * breakdown_shift (then 10 breakdown_shift), and every later diagonal block is raised by the
+ * shift that succeeded, so one pass yields the exact factor of
+ * A + diag(0, .., 0, s1, .., s2, ..) (zero before the first failing block, nondecreasing)
+ * instead of stopping at the first failed pivot. *shift_dev (device, one double) ends as the
+ * last shift (0: no breakdown). A block that fails at 10 breakdown_shift sets info as a failed
+ * pivot does. breakdown_shift = 0: exactly qmlh_spotrf_lowrank. */
+int qmlh_spotrf_lowrank_shift(float *A, int64_t n, int64_t lda, float *V, int64_t ldv,
+                              int64_t r, int32_t f16x3, double pivot_floor,
+                              double breakdown_pivot, double breakdown_shift, double *shift_dev,
+                              int32_t *info, void *ws, size_t ws_bytes, qmlh_stream_t stream);
 /* C = chol(I + Y^T Y) (r x r, lower, FP64; r <= 1024) for Y (n x r, column-major FP32): the
  * rest of qmlh_woodbury_setup once Y = L^-1 V is known. The Gram matrix is summed in FP64 over
  * row slices in a fixed order (deterministic); ws holds

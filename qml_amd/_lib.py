@@ -88,6 +88,8 @@ _SIGNATURES = {
     "qmlh_woodbury_workspace": (_sz, [_i64]),
     "qmlh_spotrf_lowrank": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i32, _f64, _vp,
                                            _vp, _sz, _vp]),
+    "qmlh_spotrf_lowrank_shift": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _i32, _f64,
+                                                 _f64, _f64, _vp, _vp, _vp, _sz, _vp]),
     "qmlh_woodbury_capacitance_workspace": (_sz, [_i64, _i64]),
     "qmlh_woodbury_capacitance": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp, _sz,
                                                  _vp]),

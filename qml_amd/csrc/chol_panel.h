@@ -83,7 +83,9 @@ __device__ __forceinline__ void stage_lower64(double (*s)[LDS_LD], const T* __re
 // c, with the Schur value left on that diagonal entry (as LAPACK leaves it); the value is the
 // same in every lane. floor > 0 (static pivoting, for factors that only precondition): a
 // finite pivot at or below floor is replaced by floor instead of failing; NaN still fails.
-__device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w, double floor = 0.0) {
+// fail_at >= 0: a pivot at or below it fails (0: LAPACK's rule).
+__device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w, double floor = 0.0,
+                                            double fail_at = 0.0) {
     __shared__ double colbuf[CP_B];
     __shared__ int s_fail;
     const int t = threadIdx.x;
@@ -101,7 +103,7 @@ __device__ __forceinline__ int potf2_wave64(double* __restrict__ sp, int w, doub
             if (c >= w || f) break;
             double piv = readlane_f64(a[c], c);
             if (!(piv > floor) && floor > 0.0 && piv == piv) piv = floor;
-            if (!(piv > 0.0)) {
+            if (!(piv > fail_at)) {
                 f = c + 1;
                 break;
             }
@@ -226,15 +228,36 @@ __device__ __forceinline__ void trsm_row(T* __restrict__ B, int64_t ldb, uint32_
 // L back, set info = col_base + c + 1 on a failed pivot. `info` already set: nothing is done.
 // sp: CP_PACK doubles of LDS (potf2_body brings its own; a kernel that has LDS to spare at
 // that point passes it).
+// Shift on breakdown (shift_acc non-null, bshift > 0; for factors that only precondition): a
+// pivot at or below bpivot counts as failed, the block's diagonal first gets *shift_acc (the
+// shift earlier blocks settled on), a failed pivot re-factors the block from A with its whole
+// diagonal raised by bshift, then 10 bshift (no further: a larger shift would precondition a
+// different matrix), and the shift that succeeds is left in *shift_acc for the later blocks. The factor is then the exact Cholesky factor of A + diag(0, .., 0, s1, .., s2, ..):
+// zero before the first failing block, nondecreasing after. A block that fails at 10 bshift
+// sets info. Diagonal blocks are factored one at a time in column order, each after the
+// one before it, so *shift_acc needs no atomics.
 template <typename T>
 __device__ __forceinline__ void potf2_body_sp(T* __restrict__ A, int64_t lda, int w,
                                               int64_t col_base, int32_t* __restrict__ info,
-                                              double* __restrict__ sp, double floor = 0.0) {
+                                              double* __restrict__ sp, double floor = 0.0,
+                                              double bshift = 0.0,
+                                              double* __restrict__ shift_acc = nullptr,
+                                              double bpivot = 0.0) {
     if (*info) return;
     const int t = threadIdx.x;
-    stage_packed64(sp, A, lda, w);
-    __syncthreads();
-    const int fail = potf2_wave64(sp, w, floor);
+    const bool shifting = shift_acc != nullptr && bshift > 0.0;
+    double shift = shifting ? *shift_acc : 0.0;
+    int fail = 0;
+    for (int attempt = 0;; ++attempt) {
+        stage_packed64(sp, A, lda, w);
+        __syncthreads();
+        if (shift > 0.0 && t < w) sp[pk(t, t)] += shift;
+        __syncthreads();
+        fail = potf2_wave64(sp, w, floor, shifting ? bpivot : 0.0);
+        if (!fail || !shifting || shift >= 10.0 * bshift) break;
+        shift = shift > 0.0 ? 10.0 * shift : bshift;  // uniform: every lane sees the same fail
+        __syncthreads();  // every lane has staged before sp is reloaded
+    }
 #pragma unroll
     for (int q = 0; q < CP_B * CP_B / 256; ++q) {
         const int idx = t + 256 * q;
@@ -242,14 +265,17 @@ __device__ __forceinline__ void potf2_body_sp(T* __restrict__ A, int64_t lda, in
         if (i < w && j < w && i >= j) A[i + (int64_t)j * lda] = (T)sp[pk(i, j)];
     }
     if (t == 0 && fail) *info = (int32_t)(col_base + fail);
+    if (t == 0 && shifting && !fail) *shift_acc = shift;
 }
 
 template <typename T>
 __device__ __forceinline__ void potf2_body(T* __restrict__ A, int64_t lda, int w,
                                            int64_t col_base, int32_t* __restrict__ info,
-                                           double floor = 0.0) {
+                                           double floor = 0.0, double bshift = 0.0,
+                                           double* __restrict__ shift_acc = nullptr,
+                                           double bpivot = 0.0) {
     __shared__ double sp[CP_PACK];
-    potf2_body_sp<T>(A, lda, w, col_base, info, sp, floor);
+    potf2_body_sp<T>(A, lda, w, col_base, info, sp, floor, bshift, shift_acc, bpivot);
 }
 
 // Rows of a sub-panel, workgroup `local` of the launch: rows 256 local .. of the m rows below

@@ -10,10 +10,21 @@ namespace qmlh {
 // For float, a non-null split_ws (qmlh_spotrf_f16x3_workspace(n) bytes) selects split-FP16
 // trailing updates instead of FP32 MFMA ones. r > 0: the r rows below the matrix
 // (A[n:n+r, 0:n], lda >= n + r) are solved along, ending as W L^-T.
-// pivot_floor > 0: static pivoting (a pivot at or below it is replaced by it, not a failure).
+// Pivot handling of the diagonal blocks. floor > 0: static pivoting (a pivot at or below it is
+// replaced by it, not a failure). shift_acc non-null and bshift > 0: shift on breakdown (a
+// pivot at or below bpivot fails; a block with a failed pivot is re-factored with its diagonal
+// raised by bshift, then 10 bshift, and every later block gets the same shift; *shift_acc, zero
+// on entry, ends as the last shift; a block that fails at 10 bshift sets info).
+struct PivotRule {
+    double floor = 0.0;
+    double bshift = 0.0;
+    double* shift_acc = nullptr;
+    double bpivot = 0.0;
+};
+
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s,
-               void* split_ws = nullptr, int64_t r = 0, double pivot_floor = 0.0);
+               void* split_ws = nullptr, int64_t r = 0, PivotRule pr = PivotRule());
 
 // Workspace of the split: a header (the scale), then regions of f16x3_region_bytes(n, kb)
 // each holding the hi/lo panels of one split of up to n rows and kb features.

@@ -58,7 +58,7 @@ template <typename T, int MODE, int LB = LAYOUT_COL>
 __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
     int64_t N, int64_t Kd, T* __restrict__ C, int64_t ldc, int32_t* __restrict__ info,
-    int64_t tiles_m, int w_next, int64_t col_base_next, double pivot_floor) {
+    int64_t tiles_m, int w_next, int64_t col_base_next, PivotRule pr) {
     __shared__ __attribute__((aligned(16))) T lds[WG_LDS_ELEMS];
     constexpr bool MASK = MODE != UPD_FULL;
     if (info && *info) return;
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256, 2) void gemm_update_kernel(
         // to the whole workgroup and frees the operand LDS for the factor
         __syncthreads();
         potf2_body_sp<T>(C, ldc, w_next, col_base_next, info, reinterpret_cast<double*>(lds),
-                         pivot_floor);
+                         pr.floor, pr.bshift, pr.shift_acc, pr.bpivot);
     }
 }
 
@@ -104,7 +104,7 @@ template <typename T, int LB = LAYOUT_COL>
 static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t M,
                          int64_t N, int64_t Kd, T* C, int64_t ldc, int32_t* info,
                          hipStream_t s, int timer_class = -1, int w_next = 0,
-                         int64_t col_base_next = 0, double pivot_floor = 0.0) {
+                         int64_t col_base_next = 0, PivotRule pr = PivotRule()) {
     if (M <= 0 || N <= 0 || Kd <= 0) return QMLH_OK;
     const int64_t tm = ceil_div(M, UP_TILE), tn = ceil_div(N, UP_TILE);
     if (timer_class < 0) timer_class = mode == UPD_SYRK ? QMLH_TIMER_SYRK : QMLH_TIMER_GEMM;
@@ -117,27 +117,27 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
         case UPD_SYRK:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0, 0.0);
+                               (int64_t)0, PivotRule());
             break;
         case UPD_SYRK_UT:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_SYRK_UT, LB>), dim3((unsigned)(tm * (tm + 1) / 2)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0, 0.0);
+                               (int64_t)0, PivotRule());
             break;
         case UPD_TRAP:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0, 0.0);
+                               (int64_t)0, PivotRule());
             break;
         case UPD_TRAP_POTF2:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_TRAP_POTF2, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, w_next,
-                               col_base_next, pivot_floor);
+                               col_base_next, pr);
             break;
         default:
             hipLaunchKernelGGL((gemm_update_kernel<T, UPD_FULL, LB>), dim3((unsigned)(tm * tn)),
                                dim3(256), 0, s, A, lda, B, ldb, M, N, Kd, C, ldc, info, tm, 0,
-                               (int64_t)0, 0.0);
+                               (int64_t)0, PivotRule());
     }
     QMLH_CHECK_LAUNCH("gemm_update launch");
     return QMLH_OK;
@@ -147,8 +147,8 @@ static int launch_update(int mode, const T* A, int64_t lda, const T* B, int64_t 
 template <typename T>
 __global__ __launch_bounds__(256) void potf2_kernel(T* __restrict__ A, int64_t lda, int w,
                                                     int64_t col_base, int32_t* __restrict__ info,
-                                                    double pivot_floor) {
-    potf2_body<T>(A, lda, w, col_base, info, pivot_floor);
+                                                    PivotRule pr) {
+    potf2_body<T>(A, lda, w, col_base, info, pr.floor, pr.bshift, pr.shift_acc, pr.bpivot);
 }
 
 template <typename T>
@@ -169,7 +169,7 @@ __global__ void clear_info_kernel(int32_t* info) { *info = 0; }
 template <typename T>
 static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64_t kb,
                                int32_t* info, hipStream_t s, hipEvent_t rows_wait = nullptr,
-                               double pivot_floor = 0.0) {
+                               PivotRule pr = PivotRule()) {
     for (int64_t jj = 0; jj < kb; jj += PF_IB) {
         const int64_t c = k + jj;
         const int w = (int)(kb - jj < PF_IB ? kb - jj : PF_IB);
@@ -177,7 +177,7 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
         const int64_t m = rows - c - w;  // rows below the diagonal block
         if (jj == 0) {
             hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, s, Acc, lda, w, c, info,
-                               pivot_floor);
+                               pr);
             QMLH_CHECK_LAUNCH("potf2 launch");
         }
         if (jj == 0 && rows_wait)  // the rows' last update may still run on another stream
@@ -191,8 +191,7 @@ static int factor_block_column(T* A, int64_t lda, int64_t rows, int64_t k, int64
         if (ncols > 0) {
             const int w2 = (int)(ncols < PF_IB ? ncols : PF_IB);
             QMLH_TRY(launch_update<T>(UPD_TRAP_POTF2, Acc + w, lda, Acc + w, lda, m, ncols, w,
-                                      Acc + w + w * lda, lda, info, s, -1, w2, c + w,
-                                      pivot_floor));
+                                      Acc + w + w * lda, lda, info, s, -1, w2, c + w, pr));
         }
     }
     return QMLH_OK;
@@ -292,7 +291,7 @@ static void lookahead_release(LookaheadCtx* c) {
 
 template <typename T>
 static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
-                           hipStream_t s, int64_t nb, double pf) {
+                           hipStream_t s, int64_t nb, PivotRule pf) {
     LookaheadCtx* ctx = nullptr;
     QMLH_TRY(lookahead_acquire(&ctx));
     hipStream_t s2 = ctx->s2;
@@ -351,7 +350,7 @@ static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB
 // the second block column once b_ready has fired, when given).
 static int factor_group_split(float* A, int64_t lda, int64_t n, int64_t r, int64_t k, int64_t w,
                               int32_t* info, hipStream_t st, hipEvent_t rows_wait,
-                              hipEvent_t b_ready, void* ws, double pf) {
+                              hipEvent_t b_ready, void* ws, PivotRule pf) {
     const int64_t wa = w < PF_NB_SPLIT ? w : PF_NB_SPLIT;
     QMLH_TRY(factor_block_column<float>(A, lda, n + r, k, wa, info, st, rows_wait, pf));
     const int64_t wb = w - wa;
@@ -369,7 +368,7 @@ static int factor_group_split(float* A, int64_t lda, int64_t n, int64_t r, int64
 }
 
 static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, int32_t* info,
-                                 hipStream_t s, void* ws, double pf) {
+                                 hipStream_t s, void* ws, PivotRule pf) {
     LookaheadCtx* ctx = nullptr;
     QMLH_TRY(lookahead_acquire(&ctx));
     hipStream_t s2 = ctx->s2, s3 = ctx->s3;
@@ -437,7 +436,7 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
 
 template <typename T>
 int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void* split_ws,
-               int64_t r, double pivot_floor) {
+               int64_t r, PivotRule pr) {
     if (n < 0 || r < 0) return set_error(QMLH_ERR_ARG, "potrf: n < 0 or r < 0");
     if (lda < (n + r > 1 ? n + r : 1)) return set_error(QMLH_ERR_ARG, "potrf: lda < n + r");
     if (!info) return set_error(QMLH_ERR_ARG, "potrf: info is null");
@@ -452,12 +451,12 @@ int potrf_impl(T* A, int64_t n, int64_t lda, int32_t* info, hipStream_t s, void*
     if (n >= PF_LOOKAHEAD_MIN) {
         if constexpr (std::is_same<T, float>::value)
             if (split16)
-                return potrf_lookahead_split(A, n, lda, r, info, s, split_ws, pivot_floor);
-        return potrf_lookahead<T>(A, n, lda, r, info, s, nb, pivot_floor);
+                return potrf_lookahead_split(A, n, lda, r, info, s, split_ws, pr);
+        return potrf_lookahead<T>(A, n, lda, r, info, s, nb, pr);
     }
     for (int64_t k = 0; k < n; k += nb) {
         const int64_t kb = n - k < nb ? n - k : nb;
-        QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s, nullptr, pivot_floor));
+        QMLH_TRY(factor_block_column<T>(A, lda, n + r, k, kb, info, s, nullptr, pr));
         const int64_t m = n - k - kb;
         T* P = A + (k + kb) + k * lda;
         QMLH_TRY(trailing_syrk<T>(P, lda, m, kb, P + kb * lda, info, s, split16 ? split_ws : nullptr));
@@ -866,9 +865,11 @@ __global__ __launch_bounds__(256) void negate_symmetrize_kernel(const double* __
 }
 
 template int potrf_impl<double>(double*, int64_t, int64_t, int32_t*, hipStream_t, void*,
-                                int64_t, double);
+                                int64_t, PivotRule);
 template int potrf_impl<float>(float*, int64_t, int64_t, int32_t*, hipStream_t, void*, int64_t,
-                               double);
+                               PivotRule);
+
+__global__ void set_f64_kernel(double* p, double v) { *p = v; }
 template int potrs_impl<double>(const double*, int64_t, int64_t, double*, double*, int32_t*,
                                 hipStream_t);
 template int potrs_impl<float>(const float*, int64_t, int64_t, double*, double*, int32_t*,
@@ -913,9 +914,11 @@ extern "C" int qmlh_spotrf_f16x3(float* A, int64_t n, int64_t lda, int32_t* info
     return potrf_impl<float>(A, n, lda, info, (hipStream_t)stream, ws);
 }
 
-extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
-                                   int64_t r, int32_t f16x3, double pivot_floor, int32_t* info,
-                                   void* ws, size_t ws_bytes, qmlh_stream_t stream) {
+static int spotrf_lowrank_impl(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
+                               int64_t r, int32_t f16x3, double pivot_floor,
+                               double breakdown_pivot, double breakdown_shift,
+                               double* shift_dev, int32_t* info,
+                               void* ws, size_t ws_bytes, hipStream_t s) {
     using namespace qmlh;
     if (n < 0 || r < 0 || lda < n + r || (r > 0 && ldv < n))
         return set_error(QMLH_ERR_ARG, "spotrf_lowrank: bad shape (lda >= n + r, ldv >= n)");
@@ -927,12 +930,42 @@ extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, i
         if (reinterpret_cast<uintptr_t>(ws) & 15)
             return set_error(QMLH_ERR_ARG, "spotrf_lowrank: workspace must be 16-byte aligned");
     }
-    hipStream_t s = (hipStream_t)stream;
     if (!(pivot_floor >= 0.0)) return set_error(QMLH_ERR_ARG, "spotrf_lowrank: pivot_floor < 0");
+    if (!(breakdown_shift >= 0.0) || !(breakdown_pivot >= 0.0) ||
+        (breakdown_shift > 0.0 && !shift_dev))
+        return set_error(QMLH_ERR_ARG,
+                         "spotrf_lowrank: breakdown_shift or breakdown_pivot < 0, or no shift output");
     if (n > 0) QMLH_TRY(transpose_f32(V, ldv, n, r, A + n, lda, s));
-    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, f16x3 ? ws : nullptr, r, pivot_floor));
+    PivotRule pr;
+    pr.floor = pivot_floor;
+    if (breakdown_shift > 0.0) {
+        pr.bshift = breakdown_shift;
+        pr.shift_acc = shift_dev;
+        pr.bpivot = breakdown_pivot;
+        hipLaunchKernelGGL(set_f64_kernel, dim3(1), dim3(1), 0, s, shift_dev, 0.0);
+        QMLH_CHECK_LAUNCH("spotrf_lowrank: clear shift");
+    }
+    QMLH_TRY(potrf_impl<float>(A, n, lda, info, s, f16x3 ? ws : nullptr, r, pr));
     if (n > 0) QMLH_TRY(transpose_f32(A + n, lda, r, n, V, ldv, s));
     return QMLH_OK;
+}
+
+extern "C" int qmlh_spotrf_lowrank(float* A, int64_t n, int64_t lda, float* V, int64_t ldv,
+                                   int64_t r, int32_t f16x3, double pivot_floor, int32_t* info,
+                                   void* ws, size_t ws_bytes, qmlh_stream_t stream) {
+    return spotrf_lowrank_impl(A, n, lda, V, ldv, r, f16x3, pivot_floor, 0.0, 0.0, nullptr, info,
+                               ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int qmlh_spotrf_lowrank_shift(float* A, int64_t n, int64_t lda, float* V,
+                                         int64_t ldv, int64_t r, int32_t f16x3,
+                                         double pivot_floor, double breakdown_pivot,
+                                         double breakdown_shift, double* shift_dev,
+                                         int32_t* info, void* ws, size_t ws_bytes,
+                                         qmlh_stream_t stream) {
+    return spotrf_lowrank_impl(A, n, lda, V, ldv, r, f16x3, pivot_floor, breakdown_pivot,
+                               breakdown_shift, shift_dev, info, ws, ws_bytes,
+                               (hipStream_t)stream);
 }
 
 // Workspace: an n-vector, from order TRSV_SYNCFREE_MIN the FP64 inverses of the factor's 64-

@@ -9,6 +9,7 @@ kernel is never materialised and K never crosses PCIe.
 
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -50,6 +51,9 @@ class FitReport:
     # normwise backward error |y - A alpha| / (|A|_inf |alpha| + |y|) of the mixed fit, when
     # its recomputed residual is above the tolerance (-1: the residual test passed)
     backward_error: float = -1.0
+    # mixed fit: one entry per factorisation attempt, (shift used, factor info, capacitance
+    # info; None when not reached)
+    attempts: list = field(default_factory=list)
 
 
 class KRR:
@@ -105,7 +109,8 @@ class KRR:
         self.lowrank = lowrank
         self.lowrank_cols = int(lowrank_cols)
         # Static pivoting of the low-precision factor: pivots at or below
-        # pivot_floor * (lambda + shift) are raised to it (0: a breakdown retries with a shift).
+        # pivot_floor * (lambda + shift) are raised to it (0: a breakdown shifts the rest of the
+        # factor, see _fit_mixed).
         if not pivot_floor >= 0.0:
             raise ValueError("pivot_floor must be >= 0")
         self.pivot_floor = float(pivot_floor)
@@ -193,11 +198,19 @@ class KRR:
         else:
             self.build_kernel(Xd, n, d, out=K, stream=stream)
             add_diagonal_device(K, n, self.lam, stream)
-        # Low-precision factor and Woodbury term; when the factor breaks down (or the
-        # capacitance matrix of the Woodbury term is not numerically positive definite), retry
-        # with a diagonal shift (the CG on the FP64 matrix still converges to the unshifted
-        # solution, only more slowly).
+        # Low-precision factor and Woodbury term. Where FP32 rounding of the factor can reach
+        # lambda -- lambda below 0.2 u32 max_i B_ii sqrt(n), the scale of the rounding that the
+        # trailing updates accumulate -- the first factorisation shifts on breakdown: a pivot
+        # at or below 2 lambda counts as failed, a diagonal block with one is factored again
+        # with its diagonal raised (by 2 lambda, then 20 lambda) and every later block keeps that
+        # shift, so one pass yields the factor of K + diag(0, .., 0, s, ..) instead of a
+        # breakdown near the end and a second factorisation. The CG on the FP64 matrix still
+        # converges to the unshifted solution. (With lambda well above that scale pivots just
+        # above lambda are genuine and a shift would only slow the CG: there the factor runs
+        # unshifted.) When the factor fails even so, or the capacitance matrix of the Woodbury
+        # term is not numerically positive definite, retry from the matrix with a full shift.
         shifts = [0.0] + [max(self.lam, 1e-12) * 10.0 ** k for k in range(1, 9)]
+        shift_dev = torch.zeros(1, dtype=torch.float64, device=dev)
         bws = None
         Yw = None  # the Woodbury term handed to the CG: V (FP32) or V64, after the solve
         for attempt, shift in enumerate(shifts):
@@ -212,14 +225,23 @@ class KRR:
             if split16 and bws is None:
                 nb = L.qmlh_spotrf_f16x3_workspace(n)
                 bws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=dev)
-            _lib.check(L.qmlh_spotrf_lowrank(_lib.ptr(B), n, ldb, _lib.ptr(V) if rx else None, n,
-                                             rx, int(split16),
-                                             self.pivot_floor * (self.lam + shift),
-                                             _lib.ptr(info),
-                                             _lib.ptr(bws) if split16 else None,
-                                             bws.numel() * 8 if split16 else 0, h))
+            bshift = 0.0
+            if attempt == 0 and self.pivot_floor == 0.0:  # (static pivoting: not combined)
+                s_ = stream if stream is not None else torch.cuda.current_stream()
+                with torch.cuda.stream(s_):
+                    dmax_t = torch.diagonal(B).max()
+                s_.synchronize()
+                dmax = float(dmax_t.item())
+                if self.lam < 0.2 * 2.0 ** -24 * dmax * math.sqrt(n):
+                    bshift = 2.0 * self.lam
+            _lib.check(L.qmlh_spotrf_lowrank_shift(
+                _lib.ptr(B), n, ldb, _lib.ptr(V) if rx else None, n, rx, int(split16),
+                self.pivot_floor * (self.lam + shift), bshift, bshift, _lib.ptr(shift_dev),
+                _lib.ptr(info), _lib.ptr(bws) if split16 else None,
+                bws.numel() * 8 if split16 else 0, h))
             rep.info = _lib.host_int(info, stream)
-            rep.shift = shift
+            rep.shift = float(shift_dev.item()) if attempt == 0 else shift
+            rep.attempts.append((rep.shift, rep.info, None))
             if rep.info != 0:
                 continue
             if not r:
@@ -249,7 +271,9 @@ class KRR:
             _lib.check(cap(_lib.ptr(Yw), n, n, r, _lib.ptr(C), r, _lib.ptr(info), _lib.ptr(gws),
                            gws.numel() * 8, h))
             del gws
-            if _lib.host_int(info, stream) == 0:
+            cinfo = _lib.host_int(info, stream)
+            rep.attempts[-1] = (rep.shift, rep.info, cinfo)
+            if cinfo == 0:
                 break
             rep.info = -1  # capacitance matrix not positive definite: next shift
         if rep.info != 0:

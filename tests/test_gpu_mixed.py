@@ -286,6 +286,57 @@ def test_spotrf_lowrank_factor_and_solve(n, r, split16):
     assert np.linalg.norm(np.tril(Cd.cpu().numpy().T) - Cref) <= 1e-12 * np.linalg.norm(Cref)
 
 
+@pytest.mark.parametrize("n,split16", [(700, 0), (700, 1), (5000, 1)])
+def test_spotrf_shift_on_breakdown(n, split16):
+    # A = W^T S W with S (n/2 x n/2) well conditioned and W = [I C]: rank n/2, the Schur
+    # complement of its leading half exactly zero, so the FP32 factor meets non-positive pivots
+    # (rounding) past column n/2. Shifting on breakdown instead factors A + D in one pass, D
+    # diagonal: zero before the first failing 64 x 64 block, then that block's shift (1e-2 or
+    # 1e-1), nondecreasing, and the last one reported. L L^T reproduces A + D to the FP32
+    # factor's backward error, about n u max|A|.
+    from qml_amd import _lib
+
+    L = _lib.lib()
+    rng = np.random.default_rng(n + 3)
+    k = n // 2
+    M = rng.standard_normal((k, k)) / np.sqrt(k)
+    S = M @ M.T + np.eye(k)
+    W = np.hstack([np.eye(k), rng.standard_normal((k, n - k)) / np.sqrt(k)])
+    A = (W.T @ S @ W).astype(np.float32)
+    r = 0
+    ws = torch.empty(L.qmlh_spotrf_f16x3_workspace(n) // 8 + 1, dtype=torch.float64,
+                     device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    shift = torch.full((1,), -1.0, dtype=torch.float64, device="cuda")
+    args = (ws.data_ptr() if split16 else None, ws.numel() * 8 if split16 else 0, None)
+    Ad = torch.from_numpy(A.copy()).cuda()
+    _lib.check(L.qmlh_spotrf_lowrank(Ad.data_ptr(), n, n, None, n, r, split16, 0.0,
+                                     info.data_ptr(), *args))
+    assert int(info.item()) > 0  # the plain factor breaks down
+    Ad = torch.from_numpy(A.copy()).cuda()
+    _lib.check(L.qmlh_spotrf_lowrank_shift(Ad.data_ptr(), n, n, None, n, r, split16, 0.0, 0.0,
+                                           1e-2, shift.data_ptr(), info.data_ptr(), *args))
+    assert int(info.item()) == 0
+    s_last = float(shift.item())
+    assert s_last >= 1e-2
+    Lg = np.tril(Ad.cpu().numpy().T).astype(np.float64)
+    E = Lg @ Lg.T - A.astype(np.float64)
+    d = np.diag(E)
+    off = E - np.diag(d)
+    tol = 2.0 * n * 2.0 ** -24 * np.max(np.abs(A))
+    assert np.max(np.abs(off)) <= tol, (np.max(np.abs(off)), tol)
+    i0 = int(np.argmax(d > 0.5e-2))  # first raised entry: the start of a 64-row block
+    assert i0 % 64 == 0 and i0 >= 64, i0
+    assert np.max(np.abs(d[:i0])) <= tol
+    levels = np.array([1e-2, 1e-1])
+    lv = levels[np.argmin(np.abs(np.log10(np.maximum(d[i0:], 1e-30))[:, None]
+                                 - np.log10(levels)[None, :]), axis=1)]
+    assert np.max(np.abs(d[i0:] - lv)) <= tol + 0.01 * lv.max()  # each raised entry: a shift
+    assert np.all(np.diff(lv) >= 0) and lv[-1] == pytest.approx(s_last)
+    for b in range(i0 // 64, (n + 63) // 64):  # one shift per 64 x 64 block
+        assert np.all(lv[b * 64 - i0:min(n, b * 64 + 64) - i0] == lv[b * 64 - i0])
+
+
 @pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "f16x3"),
                                                        ("gaussian", 4000.0, 1e-6, 3000, "fp32"),
                                                        ("laplacian", 724.0, 1e-8, 2000, "fp32"),
