@@ -122,6 +122,12 @@ enum {
     QMLH_PAIR_L2 = 3
 };
 enum { QMLH_SYMMETRIC = 1 };
+/* QMLH_LOWER (with QMLH_SYMMETRIC, qmlh_gaussian_mfma only): write only the lower triangle
+ * i >= j, computing the tiles on or below the diagonal with coalesced stores; the upper
+ * triangle of K is left untouched. For callers that only read the lower triangle (a Cholesky
+ * factorisation of K + lambda I); no counterpart in the reference, whose kernels return the
+ * whole matrix. */
+enum { QMLH_LOWER = 2 };
 int qmlh_pairwise(int32_t kind, const double *A, int64_t na, int64_t lda, const double *B,
                   int64_t nb, int64_t ldb, int64_t d, double sigma, double *K, int64_t ldk,
                   int32_t flags, qmlh_stream_t stream);
@@ -162,7 +168,8 @@ int qmlh_row_sqnorms(const double *X, int64_t n, int64_t ldx, int64_t d, double 
  * 4 u (|A_i|^2 + |B_j|^2) / (2 sigma^2) in the exponent; the drop-in picks this form only
  * where that bound is below 1e-13. Requires lda, ldb multiples of 4, d <= lda, ldb and the
  * pad columns [d, round_up(d, 16)) zero (or readable and zero in both operands).
- * With QMLH_SYMMETRIC the diagonal is exactly 1. */
+ * With QMLH_SYMMETRIC the diagonal is exactly 1; with QMLH_SYMMETRIC | QMLH_LOWER only the
+ * lower triangle is written. */
 int qmlh_gaussian_mfma(const double *A, int64_t na, int64_t lda, const double *normsA,
                        const double *B, int64_t nb, int64_t ldb, const double *normsB,
                        int64_t d, double sigma, double *K, int64_t ldk, int32_t flags,

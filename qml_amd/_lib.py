@@ -19,6 +19,7 @@ QMLH_OK = 0
 PAIR_LAPLACIAN, PAIR_GAUSSIAN, PAIR_MANHATTAN, PAIR_L2 = 0, 1, 2, 3
 PAIR_SARGAN, PAIR_MATERN_L2, PAIR_P_INT, PAIR_P_REAL = 4, 5, 6, 7
 SYMMETRIC = 1
+LOWER = 2
 CM_MAX_SIZE = 256
 ACM_MAX_ATOMS = 256
 ACM_ROW_NORM, ACM_DISTANCE = 0, 1

@@ -87,10 +87,15 @@ __global__ __launch_bounds__(256, 2) void gaussian_mfma_kernel(
     const double* __restrict__ A, int64_t lda, int64_t na, const double* __restrict__ nA,
     const double* __restrict__ B, int64_t ldb, int64_t nb, const double* __restrict__ nB,
     int64_t d, double scale, double* __restrict__ K, int64_t ldk, int symmetric,
-    int64_t tiles_m) {
+    int64_t tiles_m, int lower) {
     __shared__ __attribute__((aligned(16))) double lds[WG_LDS_ELEMS];
     int64_t bi, bj;
     tile_coords(blockIdx.x, symmetric, tiles_m, bi, bj);
+    if (lower) {  // the transposed tile: on or below the diagonal, written as is
+        const int64_t tmp = bi;
+        bi = bj;
+        bj = tmp;
+    }
     Operand<double, LAYOUT_ROW> opA{A, lda, na, d};
     Operand<double, LAYOUT_ROW> opB{B, ldb, nb, d};
     WaveTile<double> t;
@@ -104,7 +109,7 @@ __global__ __launch_bounds__(256, 2) void gaussian_mfma_kernel(
     const int64_t row0 = bi * GK_TILE + 64 * (w >> 1);
     if (row0 >= na || col0 >= nb) return;
     const bool diag_tile = symmetric && bi == bj;
-    const bool mirror = symmetric && bi != bj;
+    const bool mirror = symmetric && !lower && bi != bj;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -116,13 +121,13 @@ __global__ __launch_bounds__(256, 2) void gaussian_mfma_kernel(
             for (int e = 0; e < 4; ++e) {
                 const int64_t j = col0 + tile_row<double>(a, e);
                 if (j >= nb) continue;
-                if (diag_tile && i > j) continue;
+                if (diag_tile && (lower ? i < j : i > j)) continue;
                 double d2 = ni + nB[j] - 2.0 * t.acc[a][b][e];
                 d2 = d2 > 0.0 ? d2 : 0.0;
                 if (symmetric && i == j) d2 = 0.0;
                 const double v = exp(scale * d2);
                 K[i + j * ldk] = v;
-                if (mirror || (diag_tile && i < j)) K[j + i * ldk] = v;
+                if (mirror || (diag_tile && !lower && i < j)) K[j + i * ldk] = v;
             }
         }
     }
@@ -249,7 +254,9 @@ extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, cons
     if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15)
         return set_error(QMLH_ERR_ARG, "gaussian_mfma: A and B must be 16-byte aligned");
     bool sym = (flags & QMLH_SYMMETRIC) != 0;
+    const bool lower = (flags & QMLH_LOWER) != 0;
     if (sym && na != nb) return set_error(QMLH_ERR_ARG, "gaussian_mfma: symmetric needs na == nb");
+    if (lower && !sym) return set_error(QMLH_ERR_ARG, "gaussian_mfma: QMLH_LOWER needs QMLH_SYMMETRIC");
     if (na == 0 || nb == 0) return QMLH_OK;
     if (!A || !B || !K || !normsA || !normsB)
         return set_error(QMLH_ERR_ARG, "gaussian_mfma: null pointer");
@@ -262,7 +269,7 @@ extern "C" int qmlh_gaussian_mfma(const double* A, int64_t na, int64_t lda, cons
     TimerScope ts(QMLH_TIMER_GAUSS, work, (hipStream_t)stream);
     hipLaunchKernelGGL(gaussian_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0,
                        (hipStream_t)stream, A, lda, na, normsA, B, ldb, nb, normsB, d,
-                       -0.5 / (sigma * sigma), K, ldk, sym ? 1 : 0, tm);
+                       -0.5 / (sigma * sigma), K, ldk, sym ? 1 : 0, tm, lower ? 1 : 0);
     QMLH_CHECK_LAUNCH("gaussian_mfma launch");
     return QMLH_OK;
 }

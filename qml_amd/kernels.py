@@ -54,12 +54,14 @@ def row_sqnorms_device(Xd, n: int, d: int, stream=None):
 
 
 def gaussian_mfma_device(Ad, nA, na: int, Bd, nB, nb: int, d: int, sigma: float, symmetric=False,
-                         out=None, stream=None):
+                         out=None, stream=None, lower=False):
+    """lower (with symmetric): write only the lower triangle of the column-major K (the upper one
+    is left as it is), for callers that only read it."""
     K = _empty_kernel_device(na, nb) if out is None else out
+    flags = (_lib.SYMMETRIC if symmetric else 0) | (_lib.LOWER if symmetric and lower else 0)
     _lib.check(_lib.lib().qmlh_gaussian_mfma(
         _lib.ptr(Ad), na, Ad.shape[1], _lib.ptr(nA), _lib.ptr(Bd), nb, Bd.shape[1], _lib.ptr(nB),
-        d, float(sigma), _lib.ptr(K), na, _lib.SYMMETRIC if symmetric else 0,
-        _lib.stream_handle(stream)))
+        d, float(sigma), _lib.ptr(K), na, flags, _lib.stream_handle(stream)))
     return K
 
 

@@ -121,15 +121,16 @@ class KRR:
         self.d = 0
 
     # -- kernel matrix ------------------------------------------------------------------------
-    def build_kernel(self, Xd, n, d, norms=None, out=None, stream=None):
-        """Symmetric (n, n) training kernel, column-major, as an (n, n) device tensor."""
+    def build_kernel(self, Xd, n, d, norms=None, out=None, stream=None, lower=False):
+        """Symmetric (n, n) training kernel, column-major, as an (n, n) device tensor (lower: the
+        MFMA Gaussian writes only the lower triangle, which is all a Cholesky reads)."""
         if self.kernel == "gaussian":
             if norms is None:
                 norms = row_sqnorms_device(Xd, n, d, stream)
             mx = device_max(norms)
             if expansion_ok(mx, mx, d, self.sigma):
                 return gaussian_mfma_device(Xd, norms, n, Xd, norms, n, d, self.sigma, True, out,
-                                            stream)
+                                            stream, lower=lower)
             # outside the expansion's error bound: the direct-difference form
             return pairwise_device(_lib.PAIR_GAUSSIAN, Xd, n, Xd, n, d, self.sigma, True, out,
                                    stream)
@@ -147,7 +148,7 @@ class KRR:
         rep = FitReport(n=n, info=0)
         t0 = time.perf_counter()
         norms = row_sqnorms_device(Xd, n, d, stream) if self.kernel == "gaussian" else None
-        K = self.build_kernel(Xd, n, d, norms, stream=stream)
+        K = self.build_kernel(Xd, n, d, norms, stream=stream, lower=True)
         add_diagonal_device(K, n, self.lam, stream)
         info = potrf_device(K, n, stream)
         alpha = torch.as_tensor(y, dtype=torch.float64).to(Xd.device).clone()

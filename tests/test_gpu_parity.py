@@ -205,6 +205,17 @@ def test_gaussian_mfma_expanded_form_vs_direct():
     Ksym = gaussian_mfma_device(Xd, nX, 700, Xd, nX, 700, d, 4000.0, True).cpu().numpy().T
     assert np.array_equal(Ksym, Ksym.T) and np.all(np.diag(Ksym) == 1.0)
     assert _rel(Ksym, oracle.gaussian_kernel(X, X, 4000.0)) <= 1e-12
+    # lower-triangle mode: the same entries bit for bit below and on the diagonal (the tile
+    # that computes them is the transposed one, whose products and sums run in the same
+    # order), the upper triangle untouched
+    import torch
+
+    Kl = torch.full((700, 700), -7.0, dtype=torch.float64, device="cuda")
+    gaussian_mfma_device(Xd, nX, 700, Xd, nX, 700, d, 4000.0, True, out=Kl, lower=True)
+    Kl = Kl.cpu().numpy().T  # the device tensor holds K column-major
+    low = np.tril(np.ones((700, 700), dtype=bool))
+    assert np.array_equal(Kl[low], Ksym[low])
+    assert np.all(Kl[~low] == -7.0)
 
 
 # --------------------------------------------------------------------------------------------
