@@ -122,11 +122,11 @@ enum {
     QMLH_PAIR_L2 = 3
 };
 enum { QMLH_SYMMETRIC = 1 };
-/* QMLH_LOWER (with QMLH_SYMMETRIC, qmlh_gaussian_mfma only): write only the lower triangle
- * i >= j, computing the tiles on or below the diagonal with coalesced stores; the upper
- * triangle of K is left untouched. For callers that only read the lower triangle (a Cholesky
- * factorisation of K + lambda I); no counterpart in the reference, whose kernels return the
- * whole matrix. */
+/* QMLH_LOWER (with QMLH_SYMMETRIC; qmlh_gaussian_mfma, qmlh_pairwise, qmlh_pairwise_ext):
+ * write only the lower triangle i >= j, computing the tiles on or below the diagonal without
+ * the transposed stores; the upper triangle of K is left untouched. For callers that only
+ * read the lower triangle (a Cholesky factorisation of K + lambda I, the CG's matvec); no
+ * counterpart in the reference, whose kernels return the whole matrix. */
 enum { QMLH_LOWER = 2 };
 int qmlh_pairwise(int32_t kind, const double *A, int64_t na, int64_t lda, const double *B,
                   int64_t nb, int64_t ldb, int64_t d, double sigma, double *K, int64_t ldk,
@@ -146,7 +146,7 @@ int qmlh_pairwise(int32_t kind, const double *A, int64_t na, int64_t lda, const 
  *      (qml/fdistance.f90:121-155).
  * kind QMLH_PAIR_P_REAL: params = [p], p > 0, real powers; replaces fp_distance_double
  *      (qml/fdistance.f90:83-118). Both behind qml/distance.py:94-147 p_distance.
- * flags QMLH_SYMMETRIC as for qmlh_pairwise. */
+ * flags QMLH_SYMMETRIC and QMLH_LOWER as for qmlh_pairwise. */
 enum {
     QMLH_PAIR_SARGAN = 4,
     QMLH_PAIR_MATERN_L2 = 5,

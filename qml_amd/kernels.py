@@ -34,13 +34,15 @@ def _empty_kernel_device(na, nb):
 
 
 def pairwise_device(kind: int, Ad, na: int, Bd, nb: int, d: int, sigma: float, symmetric=False,
-                    out=None, stream=None):
-    """Run a VALU pairwise kernel on device rows; returns the (nb, na) tensor = K col-major."""
+                    out=None, stream=None, lower=False):
+    """Run a VALU pairwise kernel on device rows; returns the (nb, na) tensor = K col-major
+    (lower, with symmetric: only the lower triangle i >= j is written)."""
     K = _empty_kernel_device(na, nb) if out is None else out
     L = _lib.lib()
+    flags = (_lib.SYMMETRIC if symmetric else 0) | (_lib.LOWER if lower else 0)
     _lib.check(L.qmlh_pairwise(kind, _lib.ptr(Ad), na, Ad.shape[1], _lib.ptr(Bd), nb, Bd.shape[1],
-                               d, float(sigma), _lib.ptr(K), na,
-                               _lib.SYMMETRIC if symmetric else 0, _lib.stream_handle(stream)))
+                               d, float(sigma), _lib.ptr(K), na, flags,
+                               _lib.stream_handle(stream)))
     return K
 
 

@@ -122,8 +122,8 @@ class KRR:
 
     # -- kernel matrix ------------------------------------------------------------------------
     def build_kernel(self, Xd, n, d, norms=None, out=None, stream=None, lower=False):
-        """Symmetric (n, n) training kernel, column-major, as an (n, n) device tensor (lower: the
-        MFMA Gaussian writes only the lower triangle, which is all a Cholesky reads)."""
+        """Symmetric (n, n) training kernel, column-major, as an (n, n) device tensor (lower: only
+        the lower triangle is written, which is all a Cholesky and the CG's matvec read)."""
         if self.kernel == "gaussian":
             if norms is None:
                 norms = row_sqnorms_device(Xd, n, d, stream)
@@ -133,8 +133,9 @@ class KRR:
                                             stream, lower=lower)
             # outside the expansion's error bound: the direct-difference form
             return pairwise_device(_lib.PAIR_GAUSSIAN, Xd, n, Xd, n, d, self.sigma, True, out,
-                                   stream)
-        return pairwise_device(_lib.PAIR_LAPLACIAN, Xd, n, Xd, n, d, self.sigma, True, out, stream)
+                                   stream, lower=lower)
+        return pairwise_device(_lib.PAIR_LAPLACIAN, Xd, n, Xd, n, d, self.sigma, True, out, stream,
+                               lower=lower)
 
     # -- fit ----------------------------------------------------------------------------------
     def fit_device(self, Xd, d: int, y, raise_on_error: bool = True, stream=None) -> FitReport:
@@ -197,7 +198,7 @@ class KRR:
             V64 = torch.empty((r, n), dtype=torch.float64, device=dev) if y64 else None
             C = torch.empty((r, r), dtype=torch.float64, device=dev)
         else:
-            self.build_kernel(Xd, n, d, out=K, stream=stream)
+            self.build_kernel(Xd, n, d, out=K, stream=stream, lower=True)
             add_diagonal_device(K, n, self.lam, stream)
         # Low-precision factor and Woodbury term. Where FP32 rounding of the factor can reach
         # lambda -- lambda below 0.2 u32 max_i B_ii sqrt(n), the scale of the rounding that the

@@ -164,6 +164,48 @@ def test_pairwise_symmetric_vs_oracle(kind):
         assert _rel(got, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [1, 300, 700])
+def test_pairwise_lower_mode(n):
+    # QMLH_LOWER: the same entries bit for bit on and below the diagonal as the full symmetric
+    # build (the transposed tile sums the same differences in the same feature order), the
+    # upper triangle untouched; also through qmlh_pairwise_ext, and rejected without
+    # QMLH_SYMMETRIC.
+    from qml_amd import _lib
+    from qml_amd.device import rows_to_device
+    from qml_amd.kernels import pairwise_device
+
+    tr, _, size = synthetic.workload("krr_serve_50k", n_train=n, n_test=0)
+    X = oracle.coulomb_matrix_batch(*tr.ragged(), size)
+    d = X.shape[1]
+    Xd = rows_to_device(X)
+    low = np.tril(np.ones((n, n), dtype=bool))
+    L = _lib.lib()
+    prm = np.array([724.0, 0.5, 0.25])
+    fulls = {}
+    for kind, sigma in ((_lib.PAIR_LAPLACIAN, 724.0), (_lib.PAIR_GAUSSIAN, 4000.0),
+                        (_lib.PAIR_L2, 1.0), ("sargan", None)):
+        outs = []
+        for flags in (_lib.SYMMETRIC, _lib.SYMMETRIC | _lib.LOWER):
+            K = torch.full((n, n), -7.0, dtype=torch.float64, device="cuda")
+            if kind == "sargan":
+                _lib.check(L.qmlh_pairwise_ext(_lib.PAIR_SARGAN, _lib.ptr(Xd), n, Xd.shape[1],
+                                               _lib.ptr(Xd), n, Xd.shape[1], d, prm.ctypes.data,
+                                               prm.size, _lib.ptr(K), n, flags, None))
+            else:
+                _lib.check(L.qmlh_pairwise(kind, _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(Xd), n,
+                                           Xd.shape[1], d, sigma, _lib.ptr(K), n, flags, None))
+            outs.append(K.cpu().numpy().T)
+        full, lo = outs
+        fulls[kind] = full
+        assert np.array_equal(full, full.T), kind
+        assert np.array_equal(bits(lo[low]), bits(full[low])), kind
+        assert np.all(lo[~low] == -7.0), kind
+    Kl = pairwise_device(_lib.PAIR_LAPLACIAN, Xd, n, Xd, n, d, 724.0, True, lower=True)
+    assert np.array_equal(bits(Kl.cpu().numpy().T[low]), bits(fulls[_lib.PAIR_LAPLACIAN][low]))
+    assert L.qmlh_pairwise(_lib.PAIR_LAPLACIAN, _lib.ptr(Xd), n, Xd.shape[1], _lib.ptr(Xd), n,
+                           Xd.shape[1], d, 724.0, _lib.ptr(Kl), n, _lib.LOWER, None) != 0
+
+
 def test_pairwise_reference_fortran_vectors(ref_vectors):
     v = ref_vectors
     for kind, key, sig in (("laplacian", "K_laplacian", float(v["sigma_lap"])),
