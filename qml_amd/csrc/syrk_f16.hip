@@ -77,16 +77,36 @@ __global__ void sb_hdr_clear_kernel(unsigned* hdr) { hdr[threadIdx.x] = 0u; }
 __global__ void sb_word_clear_kernel(unsigned* hdr, int idx) { hdr[idx] = 0u; }
 
 // hdr[1] = bits of max |X(i, j)| over the finite entries of the r x w block (blockIdx.y = j).
+// Workgroup (x, y): rows 256 x .., columns SB_MAXABS_COLS y ..; one atomic per workgroup (an
+// atomic per wave and column serialised thousands of updates of the one word).
+static constexpr int SB_MAXABS_COLS = 16;
 __global__ __launch_bounds__(256) void sb_maxabs_kernel(const float* __restrict__ X, int64_t ldx,
-                                                        int64_t r, unsigned* __restrict__ hdr) {
+                                                        int64_t r, int64_t kb,
+                                                        unsigned* __restrict__ hdr) {
+    __shared__ unsigned wmax[4];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const float v = i < r ? fabsf(X[i + (int64_t)blockIdx.y * ldx]) : 0.0f;
-    unsigned m = v < 3.0e38f ? __float_as_uint(v) : 0u;
+    const int64_t c0 = (int64_t)blockIdx.y * SB_MAXABS_COLS;
+    unsigned m = 0u;
+    if (i < r) {
+#pragma unroll 4
+        for (int c = 0; c < SB_MAXABS_COLS; ++c) {
+            if (c0 + c >= kb) break;
+            const float v = fabsf(X[i + (c0 + c) * ldx]);
+            const unsigned u = v < 3.0e38f ? __float_as_uint(v) : 0u;
+            m = u > m ? u : m;
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned o = __shfl_xor(m, off, 64);
         m = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(hdr + 1, m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned w = wmax[0];
+        for (int q = 1; q < 4; ++q) w = wmax[q] > w ? wmax[q] : w;
+        if (w) atomicMax(hdr + 1, w);
+    }
 }
 
 // hdr[0] = bits of max_i A(i, i) over the positive diagonal entries (positive floats order
@@ -700,8 +720,9 @@ int f16x3_gemm_rows(const float* X, int64_t ldx, int64_t r, int64_t kb, void* ws
     sb_elem* Xh = sb_panels(ws, region_x);
     sb_elem* Xl = Xh + x_pad * kb_pad;
     hipLaunchKernelGGL(sb_word_clear_kernel, dim3(1), dim3(1), 0, s, hdr, 1);
-    hipLaunchKernelGGL(sb_maxabs_kernel, dim3((unsigned)ceil_div(r, 256), (unsigned)kb), dim3(256),
-                       0, s, X, ldx, r, hdr);
+    hipLaunchKernelGGL(sb_maxabs_kernel, dim3((unsigned)ceil_div(r, 256),
+                                              (unsigned)ceil_div(kb, SB_MAXABS_COLS)),
+                       dim3(256), 0, s, X, ldx, r, kb, hdr);
     hipLaunchKernelGGL(split_panel_kernel, dim3((unsigned)ceil_div(x_pad, 256), (unsigned)(kb_pad / 8)),
                        dim3(256), 0, s, X, ldx, r, x_pad, kb, kb_pad, Xh, Xl,
                        static_cast<const unsigned*>(ws), 1);
