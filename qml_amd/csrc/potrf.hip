@@ -331,22 +331,17 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
 // split once, and the next group's columns and the rest of the trailing matrix each take one
 // update with 2 nb features, which halves the reads and writes of the FP32 trailing matrix per
 // product against one update per block column. Per group:
-//   caller's stream: split of the group (all rows below it, region 1) -> [event x] -> update
-//     of the next group's first block column -> [event a] -> update of the rest of the
-//     trailing matrix;
-//   third stream, after event x: [the extra rows' update, below] -> update of the next
-//     group's second block column -> [event b] (beside the rest of the trailing update, which
-//     writes other columns: the side stream needs this column only after factoring the
-//     first);
+//   caller's stream: split of the group (all rows below it, region 1) -> update of the next
+//     group's first block column -> [event a] -> update of its second block column ->
+//     [event b] -> update of the rest of the trailing matrix;
 //   side stream, after event a: the next group -- its first block column factored, split
 //     (region 0), then after event b the second block column updated from it and factored.
-// The caller's stream then waits for the side stream before the next group's split (so every
-// reader of region 1 has finished). With r extra rows (A[n:n+r, :]) the next group's part of
-// their update (an FP32 product) runs first on the third stream, and the side stream waits
-// for it before solving rows; the rest of their update follows the second block column's on
-// the third stream (it writes only extra rows of later columns, which only later work on
-// that stream reads, and whose solve waits for its next event); for r <= 2 nb both are one
-// split-FP16 product (f16x3_gemm_rows).
+// The caller's stream then waits for the side stream before the next group's split. With r
+// extra rows (A[n:n+r, :]) the next group's part of their update (an FP32 product) runs on a
+// third stream that the side stream waits for before solving rows, and the rest of their
+// update right after it on the third stream (it writes only extra rows of later columns,
+// which only later work on that stream reads, and whose solve waits for its next event); for
+// r <= 2 nb both are one split-FP16 product (f16x3_gemm_rows).
 static constexpr int64_t PF_GROUP = 2 * PF_NB_SPLIT;
 
 static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB_SPLIT); }
@@ -389,38 +384,39 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
         float* P = A + (k + w) + k * lda;             // the group's rows below it (m x w)
         float* C = P + w * lda;                       // A(k + w, k + w)
         hipEvent_t rows_wait = nullptr;
-        const int64_t wa2 = w2 < PF_NB_SPLIT ? w2 : PF_NB_SPLIT;  // next group's first block column
         if (rc == QMLH_OK) rc = f16x3_split(P, lda, m, w, ws, s, region);
-        if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
-        if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
         if (rc == QMLH_OK && r > 0) {
             // the extra rows' update by this group, on the third stream: with r <= 2 nb one
             // split-FP16 product over all later columns (the extra rows split with their own
             // scale into the tail of the group's region, which a group split never reaches:
             // it has at most n - 2 nb rows); else FP32 products, the next group's columns
             // first. The next group's rows solve waits for the event after them.
+            rc = check_hip(hipEventRecord(ev_x, s), "potrf: record");
+            if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s3, ev_x, 0), "potrf: wait");
             if (r <= PF_GROUP) {
-                rc = f16x3_gemm_rows(P + m, lda, r, w, ws, region_x, region, m, C + m, lda, info,
-                                     s3);
+                if (rc == QMLH_OK)
+                    rc = f16x3_gemm_rows(P + m, lda, r, w, ws, region_x, region, m, C + m, lda,
+                                         info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
             } else {
-                rc = launch_update<float>(UPD_FULL, P + m, lda, P, lda, r, w2, w, C + m, lda,
-                                          info, s3);
+                if (rc == QMLH_OK)
+                    rc = launch_update<float>(UPD_FULL, P + m, lda, P, lda, r, w2, w, C + m, lda,
+                                              info, s3);
+                if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
+                if (rc == QMLH_OK)
+                    rc = extra_rows_update<float>(A, lda, n, r, k, w, k + w + w2, info, s3);
             }
-            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_e, s3), "potrf: record");
             rows_wait = ev_e;
         }
+        const int64_t wa2 = w2 < PF_NB_SPLIT ? w2 : PF_NB_SPLIT;  // next group's first block column
         if (rc == QMLH_OK) rc = f16x3_update(ws, m, w, 0, m, wa2, C, lda, info, s, region);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
         if (rc == QMLH_OK && w2 > wa2) {
-            // the next group's second block column, on the third stream beside the trailing
-            // update: the side stream needs it only after factoring the first
-            rc = f16x3_update(ws, m, w, wa2, m - wa2, w2 - wa2, C + wa2 + wa2 * lda, lda, info,
-                              s3, region);
-            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_b, s3), "potrf: record");
+            rc = f16x3_update(ws, m, w, wa2, m - wa2, w2 - wa2, C + wa2 + wa2 * lda, lda, info, s,
+                              region);
+            if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_b, s), "potrf: record");
         }
-        if (rc == QMLH_OK && r > PF_GROUP)
-            rc = extra_rows_update<float>(A, lda, n, r, k, w, k + w + w2, info, s3);
         if (rc == QMLH_OK)
             rc = factor_group_split(A, lda, n, r, k + w, w2, info, s2, rows_wait,
                                     w2 > wa2 ? ev_b : nullptr, ws, pf);
