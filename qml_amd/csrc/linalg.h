@@ -39,10 +39,11 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
                 size_t region);
 // C -= P P^T in split FP16 from a split of m_split rows, for the m rows starting at row off
 // (a multiple of 128): ncols == 0 the lower triangle of the m x m block, ncols > 0 the entries
-// i >= j of its first ncols columns.
+// i >= j of its first ncols columns. wide_cols: ncols > 0 on 256 x 256 tiles (one workgroup per
+// CU) from order 4096 on, else on 128 x 128 tiles (two per CU).
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
                   int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
-                  size_t region);
+                  size_t region, bool wide_cols = false);
 // C (r x m) -= X P^T: X (r x kb, FP32 column-major) split with its own scale into the region at
 // region_x (its entries need not be bounded by the factor's diagonal), P the split of m rows in
 // the region at region_p (kb features, as f16x3_split left it); three FP16 products per term.

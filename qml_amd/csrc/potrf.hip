@@ -409,12 +409,13 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
             rows_wait = ev_e;
         }
         const int64_t wa2 = w2 < PF_NB_SPLIT ? w2 : PF_NB_SPLIT;  // next group's first block column
-        if (rc == QMLH_OK) rc = f16x3_update(ws, m, w, 0, m, wa2, C, lda, info, s, region);
+        if (rc == QMLH_OK)
+            rc = f16x3_update(ws, m, w, 0, m, wa2, C, lda, info, s, region, true);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_a, s), "potrf: record");
         if (rc == QMLH_OK) rc = check_hip(hipStreamWaitEvent(s2, ev_a, 0), "potrf: wait");
         if (rc == QMLH_OK && w2 > wa2) {
             rc = f16x3_update(ws, m, w, wa2, m - wa2, w2 - wa2, C + wa2 + wa2 * lda, lda, info, s,
-                              region);
+                              region, true);
             if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_b, s), "potrf: record");
         }
         if (rc == QMLH_OK)

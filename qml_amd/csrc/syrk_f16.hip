@@ -404,8 +404,20 @@ __host__ __device__ inline int64_t sw_super_count(int64_t t) {
     return T * (T + 1) / 2;
 }
 
-__device__ __forceinline__ bool sw_tile(int64_t v, int64_t t_tiles, int64_t& bi, int64_t& bj) {
+// Workgroup v -> tile (bi, bj); false for padding and tiles above the diagonal. ncols == 0:
+// the lower triangle in super-blocks as above. ncols > 0 (the first ncols columns of the m
+// rows, a block column of the next panel): v -> (XCD x = v % 8, its l-th workgroup) -> tile row
+// 8 (l / tc) + x, tile column l % tc, so one XCD runs the tc tiles of a tile row together (they
+// share its row panel).
+__device__ __forceinline__ bool sw_tile(int64_t v, int64_t t_tiles, int64_t ncols, int64_t& bi,
+                                        int64_t& bj) {
     const int64_t x = v % SB_XCDS, l = v / SB_XCDS;
+    if (ncols > 0) {
+        const int64_t tc = (ncols + SW_T - 1) / SW_T;
+        bi = (l / tc) * SB_XCDS + x;
+        bj = l % tc;
+        return bi < t_tiles && bj <= bi;
+    }
     const int64_t sbi = (l / (SW_SUPER * SW_SUPER)) * SB_XCDS + x;
     const int64_t lt = l % (SW_SUPER * SW_SUPER);
     if (sbi >= sw_super_count(t_tiles)) return false;
@@ -449,18 +461,20 @@ struct SwAcc {
     f32x4_t v[8][4];
 };
 
-// C(i, j) -= sum_k P(i, k) P(j, k), i >= j, over the lower triangle of the m x m block; Ph / Pl
-// at row `off` of the split, rmax = the last row of the split relative to that.
+// C(i, j) -= sum_k P(i, k) P(j, k), i >= j, over the lower triangle of the m x m block, or with
+// ncols > 0 over its first ncols columns; Ph / Pl at row `off` of the split, rmax = the last row
+// of the split relative to that.
 __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
     const sb_elem* __restrict__ Ph, const sb_elem* __restrict__ Pl, int64_t m, int64_t m_pad,
-    int64_t rmax, int64_t kb_pad, float* __restrict__ C, int64_t ldc,
+    int64_t rmax, int64_t kb_pad, int64_t ncols, float* __restrict__ C, int64_t ldc,
     const int32_t* __restrict__ info, const unsigned* __restrict__ hdr) {
     // buffers x {row hi, row lo, col hi, col lo}
     __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SW_PANEL];
     if (info && *info) return;
     const int64_t t_tiles = (m + SW_T - 1) / SW_T;
+    const int64_t nc = ncols > 0 ? ncols : m;  // columns of the update
     int64_t bi, bj;
-    if (!sw_tile(blockIdx.x, t_tiles, bi, bj)) return;
+    if (!sw_tile(blockIdx.x, t_tiles, ncols, bi, bj)) return;
     const int e2 = 2 * sb_scale_exp(hdr);
     const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
     const int64_t rows0 = bi * SW_T, cols0 = bj * SW_T;
@@ -473,8 +487,8 @@ __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
     const int fr = 128 * (w & 1), sr = 64 * (w >> 1);
     const int64_t col0 = cols0 + fr;
     const int64_t row0 = rows0 + sr;
-    const bool live = row0 < m && col0 < m && col0 <= row0 + 63;
-    const bool interior = live && row0 + 63 < m && col0 + 127 < m && col0 + 127 <= row0;
+    const bool live = row0 < m && col0 < nc && col0 <= row0 + 63;
+    const bool interior = live && row0 + 63 < m && col0 + 127 < nc && col0 + 127 <= row0;
 
     sw_load_lds(lds[0][0], Ph, m_pad, rows0, rmax, 0);
     sw_load_lds(lds[0][1], Pl, m_pad, rows0, rmax, 0);
@@ -499,7 +513,8 @@ __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
                 for (int e = 0; e < 4; ++e) {
                     const int64_t i = row0 + 16 * b + r;
                     const int64_t j = col0 + 16 * a + 4 * g + e;
-                    acc.v[a][b][e] = (live && i < m && j <= i) ? C[i + j * ldc] * up : 0.0f;
+                    acc.v[a][b][e] =
+                        (live && i < m && j < nc && j <= i) ? C[i + j * ldc] * up : 0.0f;
                 }
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -557,7 +572,7 @@ __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int64_t j = col0 + 16 * a + 4 * g + e;
-                if (j > i) continue;
+                if (j > i || j >= nc) continue;
                 C[i + j * ldc] = acc.v[a][b][e] * down;
             }
         }
@@ -603,7 +618,7 @@ int f16x3_split(const float* P, int64_t ldp, int64_t m, int64_t kb, void* ws, hi
 
 int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64_t m,
                   int64_t ncols, float* C, int64_t ldc, const int32_t* info, hipStream_t s,
-                  size_t region) {
+                  size_t region, bool wide_cols) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     const int64_t m_pad = round_up(m_split, SB_T);
     const int64_t kb_pad = round_up(kb, SB_KC);
@@ -619,9 +634,23 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
         if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_update: grid too large");
         TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
         hipLaunchKernelGGL(syrk_f16x3_wide_kernel, dim3((unsigned)grid), dim3(SW_THREADS), 0, s,
-                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, C, ldc, info,
+                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, (int64_t)0, C, ldc, info,
                            static_cast<const unsigned*>(ws));
         QMLH_CHECK_LAUNCH("syrk_f16x3_wide launch");
+        return QMLH_OK;
+    }
+    if (ncols > 0 && m >= SW_MIN && wide_cols) {
+        const int64_t tc = ceil_div(ncols, (int64_t)SW_T);
+        const int64_t grid = ceil_div(ceil_div(m, (int64_t)SW_T), (int64_t)SB_XCDS) * SB_XCDS * tc;
+        if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_update: grid too large");
+        TimerScope ts(QMLH_TIMER_GEMM,
+                      ((double)m * (double)ncols - (double)ncols * (double)(ncols - 1) / 2.0) *
+                          2.0 * (double)kb,
+                      s);
+        hipLaunchKernelGGL(syrk_f16x3_wide_kernel, dim3((unsigned)grid), dim3(SW_THREADS), 0, s,
+                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, ncols, C, ldc, info,
+                           static_cast<const unsigned*>(ws));
+        QMLH_CHECK_LAUNCH("syrk_f16x3_wide columns launch");
         return QMLH_OK;
     }
     const int64_t tm = ceil_div(m, (int64_t)SB_T);
@@ -744,7 +773,7 @@ int syrk_f16x3(const float* P, int64_t ldp, int64_t m, int64_t kb, float* C, int
                 const int32_t* info, void* ws, hipStream_t s) {
     if (m <= 0 || kb <= 0) return QMLH_OK;
     QMLH_TRY(f16x3_split(P, ldp, m, kb, ws, s, 0));
-    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s, 0);
+    return f16x3_update(ws, m, kb, 0, m, 0, C, ldc, info, s, 0, false);
 }
 
 }  // namespace qmlh
