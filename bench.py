@@ -36,7 +36,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0}
 SYRK_KERNEL = {
     "fp64": ("gemm_update_kernel<double, lower> (Cholesky trailing update, FP64 MFMA)", "fp64"),
     "fp32": ("gemm_update_kernel<float, lower> (FP32 MFMA trailing update)", "fp32"),
-    "f16x3": ("syrk_f16x3_wide_kernel (trailing update of two block columns, 1536 features, "
+    "f16x3": ("syrk_f16x3_wide_kernel<false> (trailing update of two block columns, 1536 features, "
               "on 256 x 256 tiles as 3 FP16 MFMA products of the scaled split panel, FP32 "
               "accumulate)", "fp16"),
 }

@@ -462,8 +462,10 @@ struct SwAcc {
 };
 
 // C(i, j) -= sum_k P(i, k) P(j, k), i >= j, over the lower triangle of the m x m block, or with
-// ncols > 0 over its first ncols columns; Ph / Pl at row `off` of the split, rmax = the last row
-// of the split relative to that.
+// ncols > 0 over its first ncols columns (COLS: a template argument so that the two uses carry
+// different names in kernel traces); Ph / Pl at row `off` of the split, rmax = the last row of
+// the split relative to that.
+template <bool COLS>
 __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
     const sb_elem* __restrict__ Ph, const sb_elem* __restrict__ Pl, int64_t m, int64_t m_pad,
     int64_t rmax, int64_t kb_pad, int64_t ncols, float* __restrict__ C, int64_t ldc,
@@ -472,9 +474,9 @@ __global__ __launch_bounds__(SW_THREADS, 1) void syrk_f16x3_wide_kernel(
     __shared__ __attribute__((aligned(16))) sb_elem lds[2][4][SW_PANEL];
     if (info && *info) return;
     const int64_t t_tiles = (m + SW_T - 1) / SW_T;
-    const int64_t nc = ncols > 0 ? ncols : m;  // columns of the update
+    const int64_t nc = COLS ? ncols : m;  // columns of the update
     int64_t bi, bj;
-    if (!sw_tile(blockIdx.x, t_tiles, ncols, bi, bj)) return;
+    if (!sw_tile(blockIdx.x, t_tiles, COLS ? ncols : 0, bi, bj)) return;
     const int e2 = 2 * sb_scale_exp(hdr);
     const float up = ldexpf(1.0f, e2), down = ldexpf(1.0f, -e2);
     const int64_t rows0 = bi * SW_T, cols0 = bj * SW_T;
@@ -633,9 +635,9 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
                              SW_SUPER;
         if (grid > 0x7fffffffLL) return set_error(QMLH_ERR_ARG, "f16x3_update: grid too large");
         TimerScope ts(QMLH_TIMER_SYRK, (double)m * (double)(m + 1) * (double)kb, s);
-        hipLaunchKernelGGL(syrk_f16x3_wide_kernel, dim3((unsigned)grid), dim3(SW_THREADS), 0, s,
-                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, (int64_t)0, C, ldc, info,
-                           static_cast<const unsigned*>(ws));
+        hipLaunchKernelGGL(syrk_f16x3_wide_kernel<false>, dim3((unsigned)grid), dim3(SW_THREADS),
+                           0, s, Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, (int64_t)0, C, ldc,
+                           info, static_cast<const unsigned*>(ws));
         QMLH_CHECK_LAUNCH("syrk_f16x3_wide launch");
         return QMLH_OK;
     }
@@ -647,8 +649,8 @@ int f16x3_update(const void* ws, int64_t m_split, int64_t kb, int64_t off, int64
                       ((double)m * (double)ncols - (double)ncols * (double)(ncols - 1) / 2.0) *
                           2.0 * (double)kb,
                       s);
-        hipLaunchKernelGGL(syrk_f16x3_wide_kernel, dim3((unsigned)grid), dim3(SW_THREADS), 0, s,
-                           Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, ncols, C, ldc, info,
+        hipLaunchKernelGGL(syrk_f16x3_wide_kernel<true>, dim3((unsigned)grid), dim3(SW_THREADS),
+                           0, s, Ph, Pl, m, m_pad, m_pad - off - 1, kb_pad, ncols, C, ldc, info,
                            static_cast<const unsigned*>(ws));
         QMLH_CHECK_LAUNCH("syrk_f16x3_wide columns launch");
         return QMLH_OK;
