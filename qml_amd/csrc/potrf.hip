@@ -243,6 +243,7 @@ struct LookaheadCtx {
     int device = -1;
     hipStream_t s2 = nullptr, s3 = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_x = nullptr, ev_e = nullptr, ev_b = nullptr;
+    hipEvent_t ev_fa = nullptr, ev_r = nullptr;
 };
 
 static std::mutex g_la_mu;
@@ -266,13 +267,13 @@ static int lookahead_acquire(LookaheadCtx** out) {
     int rc = check_hip(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking), "potrf: side stream");
     if (rc == QMLH_OK)
         rc = check_hip(hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking), "potrf: third stream");
-    hipEvent_t* evs[5] = {&c->ev_a, &c->ev_p, &c->ev_x, &c->ev_e, &c->ev_b};
-    for (int i = 0; i < 5 && rc == QMLH_OK; ++i)
+    hipEvent_t* evs[7] = {&c->ev_a, &c->ev_p, &c->ev_x, &c->ev_e, &c->ev_b, &c->ev_fa, &c->ev_r};
+    for (int i = 0; i < 7 && rc == QMLH_OK; ++i)
         rc = check_hip(hipEventCreateWithFlags(evs[i], hipEventDisableTiming), "potrf: event");
     if (rc != QMLH_OK) {  // nothing was enqueued on them yet
         if (c->s2) (void)hipStreamDestroy(c->s2);
         if (c->s3) (void)hipStreamDestroy(c->s3);
-        for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < 7; ++i)
             if (*evs[i]) (void)hipEventDestroy(*evs[i]);
         delete c;
         return rc;
@@ -347,10 +348,15 @@ static constexpr int64_t PF_GROUP = 2 * PF_NB_SPLIT;
 static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB_SPLIT); }
 
 // Factor the w (<= 2 nb) columns from k, whose updates from earlier columns are applied (to
-// the second block column once b_ready has fired, when given).
+// the second block column once b_ready has fired, when given). With r extra rows and a third
+// stream sx (events ev_fa, ev_r), their update of the second block column from the first runs
+// on sx beside the split-FP16 update of the matrix rows (it reads the factored first block
+// column and writes only extra rows), and the second block column's factor waits for it.
 static int factor_group_split(float* A, int64_t lda, int64_t n, int64_t r, int64_t k, int64_t w,
                               int32_t* info, hipStream_t st, hipEvent_t rows_wait,
-                              hipEvent_t b_ready, void* ws, PivotRule pf) {
+                              hipEvent_t b_ready, void* ws, PivotRule pf,
+                              hipStream_t sx = nullptr, hipEvent_t ev_fa = nullptr,
+                              hipEvent_t ev_r = nullptr) {
     const int64_t wa = w < PF_NB_SPLIT ? w : PF_NB_SPLIT;
     QMLH_TRY(factor_block_column<float>(A, lda, n + r, k, wa, info, st, rows_wait, pf));
     const int64_t wb = w - wa;
@@ -358,12 +364,23 @@ static int factor_group_split(float* A, int64_t lda, int64_t n, int64_t r, int64
     const int64_t ma = n - k - wa;  // rows of the matrix below the first block column
     float* Pa = A + (k + wa) + k * lda;
     float* Cb = Pa + wa * lda;  // A(k + wa, k + wa)
+    const bool side = r > 0 && sx && ev_fa && ev_r;
+    if (side) {
+        QMLH_TRY(check_hip(hipEventRecord(ev_fa, st), "potrf: record"));
+        QMLH_TRY(check_hip(hipStreamWaitEvent(sx, ev_fa, 0), "potrf: wait"));
+        QMLH_TRY(launch_update<float>(UPD_FULL, Pa + ma, lda, Pa, lda, r, wb, wa, Cb + ma, lda,
+                                      info, sx));
+        QMLH_TRY(check_hip(hipEventRecord(ev_r, sx), "potrf: record"));
+    }
     QMLH_TRY(f16x3_split(Pa, lda, ma, wa, ws, st, 0));
     if (b_ready) QMLH_TRY(check_hip(hipStreamWaitEvent(st, b_ready, 0), "potrf: wait"));
     QMLH_TRY(f16x3_update(ws, ma, wa, 0, ma, wb, Cb, lda, info, st, 0));
-    if (r > 0)
+    if (side) {
+        QMLH_TRY(check_hip(hipStreamWaitEvent(st, ev_r, 0), "potrf: wait"));
+    } else if (r > 0) {
         QMLH_TRY(launch_update<float>(UPD_FULL, Pa + ma, lda, Pa, lda, r, wb, wa, Cb + ma, lda,
                                       info, st));
+    }
     return factor_block_column<float>(A, lda, n + r, k + wa, wb, info, st, nullptr, pf);
 }
 
@@ -420,7 +437,8 @@ static int potrf_lookahead_split(float* A, int64_t n, int64_t lda, int64_t r, in
         }
         if (rc == QMLH_OK)
             rc = factor_group_split(A, lda, n, r, k + w, w2, info, s2, rows_wait,
-                                    w2 > wa2 ? ev_b : nullptr, ws, pf);
+                                    w2 > wa2 ? ev_b : nullptr, ws, pf, s3, ctx->ev_fa,
+                                    ctx->ev_r);
         if (rc == QMLH_OK) rc = check_hip(hipEventRecord(ev_p, s2), "potrf: record");
         if (rc == QMLH_OK && m > w2)
             rc = f16x3_update(ws, m, w, w2, m - w2, 0, C + w2 + w2 * lda, lda, info, s, region);
