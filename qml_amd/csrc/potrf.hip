@@ -342,7 +342,8 @@ static int potrf_lookahead(T* A, int64_t n, int64_t lda, int64_t r, int32_t* inf
 // third stream that the side stream waits for before solving rows, and the rest of their
 // update right after it on the third stream (it writes only extra rows of later columns,
 // which only later work on that stream reads, and whose solve waits for its next event); for
-// r <= 2 nb both are one split-FP16 product (f16x3_gemm_rows).
+// r <= 2 nb both are one split-FP16 product (f16x3_gemm_rows). The third stream also takes the
+// extra rows' update of the next group's second block column from its first (factor_group_split).
 static constexpr int64_t PF_GROUP = 2 * PF_NB_SPLIT;
 
 static size_t split_group_region(int64_t n) { return f16x3_region_bytes(n, PF_NB_SPLIT); }
