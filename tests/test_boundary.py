@@ -44,6 +44,13 @@ def test_argument_errors_are_reported_without_touching_the_gpu():
     assert rc == -1 and b"lda" in L.qmlh_last_error()
     rc = L.qmlh_gaussian_mfma(None, 4, 6, None, None, 4, 8, None, 4, 1.0, None, 4, 0, None)
     assert rc == -1 and b"multiples of 4" in L.qmlh_last_error()
+    # the lower-triangle flag only makes sense for a symmetric call
+    rc = L.qmlh_pairwise(0, None, 4, 4, None, 4, 4, 4, 1.0, None, 4, _lib.LOWER, None)
+    assert rc == -1 and b"QMLH_LOWER needs QMLH_SYMMETRIC" in L.qmlh_last_error()
+    prm = (ctypes.c_double * 1)(1.0)
+    rc = L.qmlh_pairwise_ext(_lib.PAIR_P_REAL, None, 4, 4, None, 4, 4, 4, prm, 1, None, 4,
+                             _lib.LOWER, None)
+    assert rc == -1 and b"QMLH_LOWER needs QMLH_SYMMETRIC" in L.qmlh_last_error()
     assert L.qmlh_predict_workspace(10, 1000) == 10 * 8 * 8
 
 
