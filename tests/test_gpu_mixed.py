@@ -340,7 +340,9 @@ def test_spotrf_shift_on_breakdown(n, split16):
 @pytest.mark.parametrize("kernel,sigma,lam,n,factor", [("gaussian", 4000.0, 1e-6, 3000, "f16x3"),
                                                        ("gaussian", 4000.0, 1e-6, 3000, "fp32"),
                                                        ("laplacian", 724.0, 1e-8, 2000, "fp32"),
-                                                       ("laplacian", 724.0, 1e-8, 2000, "f16x3")])
+                                                       ("laplacian", 724.0, 1e-8, 2000, "f16x3"),
+                                                       # grouped look-ahead, no extra rows
+                                                       ("laplacian", 724.0, 1e-8, 6000, "f16x3")])
 def test_krr_mixed_matches_fp64_and_oracle(kernel, sigma, lam, n, factor):
     from qml_amd.krr import KRR
 
